@@ -1,0 +1,515 @@
+// C ABI of the prover (include/gkrx.h): contexts, circuits, prove, verify.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gkrx.h"
+#include "host_hash.hpp"
+#include "internal.hpp"
+#include "verifier.hpp"
+
+struct gkrx_ctx {
+    gkrx_config cfg;
+    int device = 0;
+    uint32_t R = 3;
+    uint32_t L = 8;
+    hipStream_t stream = nullptr;
+    gkrx::DevScratch scratch;
+    gkrx::ProveTimings last;
+    std::mutex mu;
+};
+
+struct gkrx_circuit {
+    gkrx_ctx *ctx = nullptr;
+    uint32_t D = 0;
+    std::vector<uint32_t> n, m;
+    std::vector<std::vector<uint32_t>> mul, add;  // canonical coefficients
+    gkrx::DevCircuit dev;
+    std::vector<uint32_t> witness;
+    bool has_witness = false;
+    bool evaluated = false;
+};
+
+namespace gkrx {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+}  // namespace gkrx
+
+using namespace gkrx;
+
+extern "C" const char *gkrx_last_error(void) { return g_err.c_str(); }
+
+extern "C" void gkrx_config_default(gkrx_config *cfg)
+{
+    if (!cfg) return;
+    cfg->field_type = GKRX_FIELD_M31;
+    cfg->fs_hash = GKRX_FS_SHA256;
+    cfg->pc_type = GKRX_PC_RAW;
+    cfg->security_bits = 100;
+    cfg->grinding_bits = 10;
+    cfg->nb_parallel = 8;
+}
+
+static int field_bits(int field_type)
+{
+    switch (field_type) {
+    case GKRX_FIELD_M31: return 31;
+    case GKRX_FIELD_BABYBEAR: return 31;
+    case GKRX_FIELD_BN254: return 254;
+    default: return -1;
+    }
+}
+
+extern "C" int gkrx_config_num_repetitions(const gkrx_config *cfg)
+{
+    if (!cfg) return GKRX_ERR_ARG;
+    const int fs = field_bits(cfg->field_type);
+    if (fs <= 0) return GKRX_ERR_CONFIG;
+    const int num = cfg->security_bits - cfg->grinding_bits;
+    return num / fs + (num % fs != 0 ? 1 : 0);
+}
+
+extern "C" int gkrx_ctx_create(const gkrx_config *cfg, int device, gkrx_ctx **out)
+{
+    if (!cfg || !out) return GKRX_ERR_ARG;
+    // Prover::Prover asserts M31 / Raw (LinearGKR.hpp:44-46); both transcript hashes run here.
+    if (cfg->field_type != GKRX_FIELD_M31 || cfg->pc_type != GKRX_PC_RAW ||
+        (cfg->fs_hash != GKRX_FS_SHA256 && cfg->fs_hash != GKRX_FS_KECCAK256) ||
+        (cfg->nb_parallel != 8 && cfg->nb_parallel != 16) || cfg->grinding_bits < 0 || cfg->grinding_bits > 24) {
+        set_error("unsupported config");
+        return GKRX_ERR_CONFIG;
+    }
+    const int R = gkrx_config_num_repetitions(cfg);
+    if (R < 1 || R > 3) {
+        set_error("num_repetitions must be 1..3 (the reference keeps 3 helpers, sumcheck.hpp:65)");
+        return GKRX_ERR_CONFIG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        set_error("no usable HIP device");
+        return GKRX_ERR_DEVICE;
+    }
+    GKRX_HIP_CHECK(hipSetDevice(device));
+    gkrx_ctx *c = new (std::nothrow) gkrx_ctx;
+    if (!c) return GKRX_ERR_NOMEM;
+    c->cfg = *cfg;
+    c->device = device;
+    c->R = (uint32_t)R;
+    c->L = (uint32_t)cfg->nb_parallel;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreate failed");
+        return GKRX_ERR_DEVICE;
+    }
+    *out = c;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_ctx_destroy(gkrx_ctx *ctx)
+{
+    if (!ctx) return GKRX_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_free_scratch(ctx->scratch);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GKRX_OK;
+}
+
+// next_pow_of_2 + _compute_nb_vars (include/utils/myutil.hpp:38-48, circuit.hpp:124-149)
+static uint32_t nb_vars_of(uint32_t max_id)
+{
+    if (max_id == 0) return 0;
+    uint32_t p = ((max_id & (max_id - 1)) == 0) ? max_id : (1u << (32 - __builtin_clz(max_id)));
+    return (uint32_t)__builtin_ctz(p);
+}
+
+// M31(uint32) folds once (include/field/M31.hpp:58-62); the library keeps canonical
+// coefficients.  0xffffffff folds to 2^31, where the reference's signed AVX lane
+// multiply is not a field multiply, so it is rejected.
+static bool canon_coef(uint32_t raw, uint32_t &out)
+{
+    uint32_t f = (raw & 0x7fffffffu) + (raw >> 31);
+    if (f > 0x7fffffffu) return false;
+    out = f == 0x7fffffffu ? 0 : f;
+    return true;
+}
+
+static int finish_circuit(gkrx_ctx *ctx, gkrx_circuit *c)
+{
+    const uint32_t D = c->D;
+    c->n.assign(D, 0);
+    c->m.assign(D, 0);
+    for (uint32_t l = 0; l < D; l++) {
+        uint32_t mo = 0, mi = 0;
+        const auto &mu = c->mul[l];
+        for (size_t g = 0; g < mu.size() / 4; g++) {
+            mi = std::max({mi, mu[4 * g], mu[4 * g + 1]});
+            mo = std::max(mo, mu[4 * g + 2]);
+        }
+        const auto &ad = c->add[l];
+        for (size_t g = 0; g < ad.size() / 3; g++) {
+            mi = std::max(mi, ad[3 * g]);
+            mo = std::max(mo, ad[3 * g + 1]);
+        }
+        c->n[l] = nb_vars_of(mi);
+        c->m[l] = nb_vars_of(mo);
+    }
+    for (uint32_t l = 0; l < D; l++) {
+        const uint32_t n = c->n[l], m = c->m[l];
+        if (n < 1 || n > kMaxVars || m < 1 || m > kMaxVars) {
+            set_error("layer " + std::to_string(l) + ": need 1..30 input and output variables");
+            return GKRX_ERR_CIRCUIT;
+        }
+        if (l + 1 < D && c->n[l + 1] != m) {
+            set_error("layer " + std::to_string(l) + ": output variables differ from the next layer's inputs");
+            return GKRX_ERR_CIRCUIT;
+        }
+        auto &mu = c->mul[l];
+        for (size_t g = 0; g < mu.size() / 4; g++) {
+            if ((mu[4 * g] >> n) || (mu[4 * g + 1] >> n) || (mu[4 * g + 2] >> m) || !canon_coef(mu[4 * g + 3], mu[4 * g + 3])) {
+                set_error("layer " + std::to_string(l) + ": mul gate index or coefficient out of range");
+                return GKRX_ERR_CIRCUIT;
+            }
+        }
+        auto &ad = c->add[l];
+        for (size_t g = 0; g < ad.size() / 3; g++) {
+            if ((ad[3 * g] >> n) || (ad[3 * g + 1] >> m) || !canon_coef(ad[3 * g + 2], ad[3 * g + 2])) {
+                set_error("layer " + std::to_string(l) + ": add gate index or coefficient out of range");
+                return GKRX_ERR_CIRCUIT;
+            }
+        }
+    }
+    GKRX_HIP_CHECK(hipSetDevice(ctx->device));
+    int rc = dev_upload_circuit(c->dev, ctx->L, c->n, c->m, c->mul, c->add);
+    if (rc) return rc;
+    c->ctx = ctx;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_circuit_from_arrays(gkrx_ctx *ctx, uint32_t nb_layers, const uint32_t *n_mul,
+                                        const uint32_t *const *mul, const uint32_t *n_add,
+                                        const uint32_t *const *add, gkrx_circuit **out)
+{
+    if (!ctx || !out || nb_layers == 0 || !n_mul || !mul || !n_add || !add) return GKRX_ERR_ARG;
+    gkrx_circuit *c = new (std::nothrow) gkrx_circuit;
+    if (!c) return GKRX_ERR_NOMEM;
+    c->D = nb_layers;
+    c->mul.resize(nb_layers);
+    c->add.resize(nb_layers);
+    for (uint32_t l = 0; l < nb_layers; l++) {
+        c->mul[l].assign(mul[l], mul[l] + (size_t)4 * n_mul[l]);
+        c->add[l].assign(add[l], add[l] + (size_t)3 * n_add[l]);
+    }
+    int rc = finish_circuit(ctx, c);
+    if (rc) {
+        dev_free_circuit(c->dev);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return GKRX_OK;
+}
+
+// Circuit::load_extracted_gates (include/circuit/circuit.hpp:152-215)
+extern "C" int gkrx_circuit_load_text(gkrx_ctx *ctx, const char *mul_path, const char *add_path, gkrx_circuit **out)
+{
+    if (!ctx || !mul_path || !add_path || !out) return GKRX_ERR_ARG;
+    FILE *f = fopen(mul_path, "r");
+    if (!f) {
+        set_error(std::string("cannot open ") + mul_path);
+        return GKRX_ERR_IO;
+    }
+    uint32_t nb = 0;
+    int ch, last = '\n';
+    while ((ch = fgetc(f)) != EOF) {
+        if (ch == '\n') nb++;
+        last = ch;
+    }
+    if (last != '\n') nb++;
+    rewind(f);
+    if (nb == 0) {
+        fclose(f);
+        set_error("empty mul file");
+        return GKRX_ERR_CIRCUIT;
+    }
+    gkrx_circuit *c = new (std::nothrow) gkrx_circuit;
+    if (!c) {
+        fclose(f);
+        return GKRX_ERR_NOMEM;
+    }
+    c->D = nb;
+    c->mul.resize(nb);
+    c->add.resize(nb);
+    auto fail = [&](const char *msg) {
+        if (f) fclose(f);
+        delete c;
+        set_error(msg);
+        return GKRX_ERR_CIRCUIT;
+    };
+    for (uint32_t i = 0; i < nb; i++) {
+        auto &mu = c->mul[nb - 1 - i];  // line i -> layers[D - 1 - i]
+        unsigned long ng;
+        if (fscanf(f, "%lu", &ng) != 1) return fail("malformed mul file");
+        mu.resize(4 * ng);
+        for (unsigned long g = 0; g < ng; g++) {
+            unsigned long i0, i1, o;
+            unsigned coef;
+            if (fscanf(f, "%lu%lu%lu%u", &i0, &i1, &o, &coef) != 4) return fail("malformed mul gate");
+            if ((i0 | i1 | o) >> 32) return fail("gate index above 2^32");
+            mu[4 * g] = (uint32_t)i0;
+            mu[4 * g + 1] = (uint32_t)i1;
+            mu[4 * g + 2] = (uint32_t)o;
+            mu[4 * g + 3] = coef;
+        }
+    }
+    fclose(f);
+    f = fopen(add_path, "r");
+    if (!f) {
+        delete c;
+        set_error(std::string("cannot open ") + add_path);
+        return GKRX_ERR_IO;
+    }
+    for (uint32_t i = 0; i < nb; i++) {
+        auto &ad = c->add[nb - 1 - i];
+        unsigned long ng;
+        if (fscanf(f, "%lu", &ng) != 1) return fail("malformed add file");
+        ad.resize(3 * ng);
+        for (unsigned long g = 0; g < ng; g++) {
+            unsigned long i0, o;
+            unsigned coef;
+            if (fscanf(f, "%lu%lu%u", &i0, &o, &coef) != 3) return fail("malformed add gate");
+            if ((i0 | o) >> 32) return fail("gate index above 2^32");
+            ad[3 * g] = (uint32_t)i0;
+            ad[3 * g + 1] = (uint32_t)o;
+            ad[3 * g + 2] = coef;
+        }
+    }
+    fclose(f);
+    f = nullptr;
+    int rc = finish_circuit(ctx, c);
+    if (rc) {
+        dev_free_circuit(c->dev);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_circuit_shape(const gkrx_circuit *c, uint32_t *nb_layers, uint32_t *nb_in_vars,
+                                  uint32_t *nb_out_vars)
+{
+    if (!c) return GKRX_ERR_ARG;
+    if (nb_layers) *nb_layers = c->D;
+    for (uint32_t l = 0; l < c->D; l++) {
+        if (nb_in_vars) nb_in_vars[l] = c->n[l];
+        if (nb_out_vars) nb_out_vars[l] = c->m[l];
+    }
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_circuit_write_text(const gkrx_circuit *c, const char *mul_path, const char *add_path)
+{
+    if (!c || !mul_path || !add_path) return GKRX_ERR_ARG;
+    for (int which = 0; which < 2; which++) {
+        FILE *f = fopen(which == 0 ? mul_path : add_path, "w");
+        if (!f) {
+            set_error("cannot write circuit file");
+            return GKRX_ERR_IO;
+        }
+        for (uint32_t i = 0; i < c->D; i++) {
+            const uint32_t l = c->D - 1 - i;
+            const auto &g = which == 0 ? c->mul[l] : c->add[l];
+            const size_t w = which == 0 ? 4 : 3;
+            fprintf(f, "%zu", g.size() / w);
+            for (size_t k = 0; k < g.size(); k += w) {
+                if (w == 4) fprintf(f, " %u %u %u %u", g[k], g[k + 1], g[k + 2], g[k + 3]);
+                else fprintf(f, " %u %u %u", g[k], g[k + 1], g[k + 2]);
+            }
+            fputc('\n', f);
+        }
+        fclose(f);
+    }
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_circuit_set_witness(gkrx_circuit *c, const uint32_t *lanes, size_t n_u32)
+{
+    if (!c || !lanes) return GKRX_ERR_ARG;
+    const size_t need = ((size_t)1 << c->n[0]) * c->dev.L;
+    if (n_u32 != need) {
+        set_error("witness must hold 2^nb_input_vars(0) elements of L lanes");
+        return GKRX_ERR_ARG;
+    }
+    for (size_t i = 0; i < n_u32; i++)
+        if (lanes[i] >= 0x7fffffffu) {
+            set_error("witness values must be canonical M31 (< 2^31 - 1)");
+            return GKRX_ERR_ARG;
+        }
+    std::lock_guard<std::mutex> lk(c->ctx->mu);
+    GKRX_HIP_CHECK(hipSetDevice(c->ctx->device));
+    c->witness.assign(lanes, lanes + n_u32);
+    GKRX_HIP_CHECK(hipMemcpyAsync(c->dev.values, c->witness.data(), n_u32 * 4, hipMemcpyHostToDevice, c->ctx->stream));
+    GKRX_HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+    c->has_witness = true;
+    c->evaluated = false;
+    return GKRX_OK;
+}
+
+static int evaluate_locked(gkrx_circuit *c)
+{
+    if (!c->has_witness) {
+        set_error("no witness set");
+        return GKRX_ERR_STATE;
+    }
+    int rc = dev_forward(c->dev, c->ctx->stream);
+    if (rc) return rc;
+    c->evaluated = true;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_circuit_evaluate(gkrx_circuit *c)
+{
+    if (!c) return GKRX_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->ctx->mu);
+    GKRX_HIP_CHECK(hipSetDevice(c->ctx->device));
+    return evaluate_locked(c);
+}
+
+extern "C" int gkrx_circuit_get_layer_values(gkrx_circuit *c, uint32_t layer, uint32_t *out, size_t n_u32)
+{
+    if (!c || !out || layer > c->D) return GKRX_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->ctx->mu);
+    if (!c->evaluated) {
+        set_error("circuit not evaluated");
+        return GKRX_ERR_STATE;
+    }
+    const uint32_t vars = layer < c->D ? c->n[layer] : c->m[c->D - 1];
+    const size_t need = ((size_t)1 << vars) * c->dev.L;
+    if (n_u32 < need) return GKRX_ERR_ARG;
+    const uint64_t off = layer < c->D ? c->dev.layers[layer].v_off : c->dev.layers[c->D - 1].out_off;
+    GKRX_HIP_CHECK(hipSetDevice(c->ctx->device));
+    GKRX_HIP_CHECK(hipMemcpyAsync(out, c->dev.values + off, need * 4, hipMemcpyDeviceToHost, c->ctx->stream));
+    GKRX_HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_circuit_get_output(gkrx_circuit *c, uint32_t *out, size_t n_u32)
+{
+    if (!c) return GKRX_ERR_ARG;
+    return gkrx_circuit_get_layer_values(c, c->D, out, n_u32);
+}
+
+extern "C" int gkrx_circuit_destroy(gkrx_circuit *c)
+{
+    if (!c) return GKRX_ERR_ARG;
+    if (c->ctx) {
+        std::lock_guard<std::mutex> lk(c->ctx->mu);
+        (void)hipSetDevice(c->ctx->device);
+        (void)hipStreamSynchronize(c->ctx->stream);
+        dev_free_circuit(c->dev);
+    }
+    delete c;
+    return GKRX_OK;
+}
+
+static size_t tail_bytes(const gkrx_ctx *ctx, const gkrx_circuit *c)
+{
+    const size_t S = (size_t)ctx->L * 4;
+    size_t t = 32;
+    for (uint32_t l = 0; l < c->D; l++) t += ((size_t)6 * c->n[l] * ctx->R + 2 * ctx->R) * S;
+    return t;
+}
+
+extern "C" size_t gkrx_proof_size(const gkrx_ctx *ctx, const gkrx_circuit *c)
+{
+    if (!ctx || !c) return 0;
+    return ((size_t)1 << c->n[0]) * ctx->L * 4 + tail_bytes(ctx, c);
+}
+
+extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap, size_t *proof_len,
+                          uint32_t *claimed_v)
+{
+    if (!ctx || !c || !proof || !claimed_v || c->ctx != ctx) return GKRX_ERR_ARG;
+    const size_t total = gkrx_proof_size(ctx, c);
+    if (proof_cap < total) {
+        set_error("proof buffer too small");
+        return GKRX_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    GKRX_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t tb = tail_bytes(ctx, c);
+    int rc = dev_ensure_scratch(ctx->scratch, c->dev, ctx->R, tb);
+    if (rc) return rc;
+    if (!c->evaluated) {
+        rc = evaluate_locked(c);  // asynchronous: overlaps the host transcript prefix below
+        if (rc) return rc;
+    }
+    const size_t S = (size_t)ctx->L * 4;
+    const size_t commit_len = ((size_t)1 << c->n[0]) * S;
+    const uint32_t mlast = c->m[c->D - 1];
+    std::vector<uint32_t> rz1((size_t)ctx->R * mlast);
+    uint32_t tstate[10];
+    uint8_t grind[32];
+    host_transcript_prefix(ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0,
+                           reinterpret_cast<const uint8_t *>(c->witness.data()), commit_len, ctx->R, mlast,
+                           (uint32_t)ctx->cfg.grinding_bits, grind, rz1.data(), tstate);
+    const auto t1 = std::chrono::steady_clock::now();
+    rc = dev_prove(c->dev, ctx->scratch, ctx->R, ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0, rz1.data(), tstate,
+                   grind, 32, ctx->stream);
+    if (rc) return rc;
+    uint32_t tfinal[10];
+    memcpy(proof, c->witness.data(), commit_len);  // RAW commitment bytes (raw.hpp:19-27)
+    GKRX_HIP_CHECK(hipMemcpyAsync(proof + commit_len, ctx->scratch.tail, tb, hipMemcpyDeviceToHost, ctx->stream));
+    GKRX_HIP_CHECK(hipMemcpyAsync(claimed_v, ctx->scratch.claims, (size_t)ctx->R * ctx->L * 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    GKRX_HIP_CHECK(hipMemcpyAsync(tfinal, ctx->scratch.tstate, sizeof tfinal, hipMemcpyDeviceToHost, ctx->stream));
+    GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto t2 = std::chrono::steady_clock::now();
+    if (tfinal[0] != tb) {
+        set_error("device transcript length mismatch: " + std::to_string(tfinal[0]) + " != " + std::to_string(tb));
+        return GKRX_ERR_DEVICE;
+    }
+    *proof_len = total;
+    ctx->last.host_prefix_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v, const uint8_t *proof,
+                           size_t proof_len)
+{
+    if (!ctx || !c || !claimed_v || !proof) return GKRX_ERR_ARG;
+    HostCircuitView v{c->D, c->n.data(), c->m.data(), c->mul, c->add};
+    return host_verify(v, ctx->L, ctx->R, ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0,
+                       (uint32_t)ctx->cfg.grinding_bits, claimed_v, proof, proof_len);
+}
+
+extern "C" int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3)
+{
+    if (!ctx || !ms3) return GKRX_ERR_ARG;
+    ms3[0] = ctx->last.host_prefix_ms;
+    ms3[1] = ctx->last.device_ms;
+    ms3[2] = ctx->last.total_ms;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_hash(int fs_hash, const uint8_t *in, size_t len, uint8_t out[32])
+{
+    if ((!in && len) || !out) return GKRX_ERR_ARG;
+    if (fs_hash != GKRX_FS_SHA256 && fs_hash != GKRX_FS_KECCAK256) return GKRX_ERR_CONFIG;
+    host_hash(fs_hash == GKRX_FS_KECCAK256 ? 1 : 0, in, len, out);
+    return GKRX_OK;
+}

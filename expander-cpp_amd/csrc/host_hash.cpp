@@ -1,0 +1,258 @@
+// Host transcript hashes.  The transcript prefix of a proof -- the hash of the RAW
+// commitment (the whole input layer, include/LinearGKR/LinearGKR.hpp:62-69), grinding
+// (:13-32) and the output-layer challenges (include/LinearGKR/gkr.hpp:42-49) -- is a
+// serial chain that no amount of device parallelism shortens, so it runs on a host core
+// (SHA-256 with the x86 SHA extensions when present) while the device evaluates the
+// circuit.  Every later challenge is drawn on the device.
+#include <immintrin.h>
+
+#include <cpuid.h>
+#include <cstdint>
+#include <cstring>
+
+#include "host_hash.hpp"
+
+namespace gkrx {
+
+namespace {
+
+const uint32_t kK[64] = {
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+
+inline uint32_t ror(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+void sha256_blocks_portable(uint32_t s[8], const uint8_t *p, size_t nblocks)
+{
+    for (size_t b = 0; b < nblocks; b++, p += 64) {
+        uint32_t w[64];
+        for (int t = 0; t < 16; t++)
+            w[t] = ((uint32_t)p[4 * t] << 24) | ((uint32_t)p[4 * t + 1] << 16) | ((uint32_t)p[4 * t + 2] << 8) |
+                   p[4 * t + 3];
+        for (int t = 16; t < 64; t++)
+            w[t] = w[t - 16] + (ror(w[t - 15], 7) ^ ror(w[t - 15], 18) ^ (w[t - 15] >> 3)) + w[t - 7] +
+                   (ror(w[t - 2], 17) ^ ror(w[t - 2], 19) ^ (w[t - 2] >> 10));
+        uint32_t a = s[0], bb = s[1], c = s[2], d = s[3], e = s[4], f = s[5], g = s[6], h = s[7];
+        for (int t = 0; t < 64; t++) {
+            uint32_t t1 = h + (ror(e, 6) ^ ror(e, 11) ^ ror(e, 25)) + ((e & f) ^ (~e & g)) + kK[t] + w[t];
+            uint32_t t2 = (ror(a, 2) ^ ror(a, 13) ^ ror(a, 22)) + ((a & bb) ^ (a & c) ^ (bb & c));
+            h = g; g = f; f = e; e = d + t1; d = c; c = bb; bb = a; a = t1 + t2;
+        }
+        s[0] += a; s[1] += bb; s[2] += c; s[3] += d; s[4] += e; s[5] += f; s[6] += g; s[7] += h;
+    }
+}
+
+// SHA-256 compression with the x86 SHA extensions (sha256rnds2 / msg1 / msg2), following
+// the register arrangement Intel documents for them (state as ABEF / CDGH halves).
+__attribute__((target("sha,sse4.1,ssse3"))) void sha256_blocks_shani(uint32_t s[8], const uint8_t *p,
+                                                                       size_t nblocks)
+{
+    const __m128i kBswap = _mm_set_epi64x(0x0c0d0e0f08090a0bULL, 0x0405060700010203ULL);
+    __m128i tmp = _mm_loadu_si128(reinterpret_cast<const __m128i *>(&s[0]));
+    __m128i st1 = _mm_loadu_si128(reinterpret_cast<const __m128i *>(&s[4]));
+    tmp = _mm_shuffle_epi32(tmp, 0xB1);
+    st1 = _mm_shuffle_epi32(st1, 0x1B);
+    __m128i st0 = _mm_alignr_epi8(tmp, st1, 8);  // ABEF
+    st1 = _mm_blend_epi16(st1, tmp, 0xF0);       // CDGH
+    for (size_t b = 0; b < nblocks; b++, p += 64) {
+        const __m128i save0 = st0, save1 = st1;
+        __m128i w[4];
+        for (int i = 0; i < 4; i++)
+            w[i] = _mm_shuffle_epi8(_mm_loadu_si128(reinterpret_cast<const __m128i *>(p + 16 * i)), kBswap);
+        for (int g = 0; g < 16; g++) {
+            if (g >= 4) {
+                __m128i x = _mm_sha256msg1_epu32(w[g & 3], w[(g + 1) & 3]);
+                x = _mm_add_epi32(x, _mm_alignr_epi8(w[(g + 3) & 3], w[(g + 2) & 3], 4));
+                w[g & 3] = _mm_sha256msg2_epu32(x, w[(g + 3) & 3]);
+            }
+            __m128i msg = _mm_add_epi32(w[g & 3], _mm_loadu_si128(reinterpret_cast<const __m128i *>(&kK[4 * g])));
+            st1 = _mm_sha256rnds2_epu32(st1, st0, msg);
+            msg = _mm_shuffle_epi32(msg, 0x0E);
+            st0 = _mm_sha256rnds2_epu32(st0, st1, msg);
+        }
+        st0 = _mm_add_epi32(st0, save0);
+        st1 = _mm_add_epi32(st1, save1);
+    }
+    tmp = _mm_shuffle_epi32(st0, 0x1B);
+    st1 = _mm_shuffle_epi32(st1, 0xB1);
+    st0 = _mm_blend_epi16(tmp, st1, 0xF0);
+    st1 = _mm_alignr_epi8(st1, tmp, 8);
+    _mm_storeu_si128(reinterpret_cast<__m128i *>(&s[0]), st0);
+    _mm_storeu_si128(reinterpret_cast<__m128i *>(&s[4]), st1);
+}
+
+bool cpu_has_sha()
+{
+    unsigned a, b, c, d;
+    if (!__get_cpuid_count(7, 0, &a, &b, &c, &d)) return false;
+    return (b >> 29) & 1u;
+}
+
+const bool kHasSha = cpu_has_sha();
+
+void sha256_blocks(uint32_t s[8], const uint8_t *p, size_t nblocks)
+{
+    if (kHasSha) sha256_blocks_shani(s, p, nblocks);
+    else sha256_blocks_portable(s, p, nblocks);
+}
+
+const uint64_t kRC[24] = {
+    0x0000000000000001ULL, 0x0000000000008082ULL, 0x800000000000808aULL, 0x8000000080008000ULL,
+    0x000000000000808bULL, 0x0000000080000001ULL, 0x8000000080008081ULL, 0x8000000000008009ULL,
+    0x000000000000008aULL, 0x0000000000000088ULL, 0x0000000080008009ULL, 0x000000008000000aULL,
+    0x000000008000808bULL, 0x800000000000008bULL, 0x8000000000008089ULL, 0x8000000000008003ULL,
+    0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
+    0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
+
+inline uint64_t rol(uint64_t x, int n) { return (x << n) | (x >> (64 - n)); }
+
+void keccak_f(uint64_t A[25])
+{
+    for (int round = 0; round < 24; round++) {
+        uint64_t C[5];
+        for (int x = 0; x < 5; x++) C[x] = A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20];
+        uint64_t D[5];
+        for (int x = 0; x < 5; x++) D[x] = C[(x + 4) % 5] ^ rol(C[(x + 1) % 5], 1);
+        uint64_t B[25];
+        B[0] = A[0] ^ D[0];
+        B[10] = rol(A[1] ^ D[1], 1);
+        B[20] = rol(A[2] ^ D[2], 62);
+        B[5] = rol(A[3] ^ D[3], 28);
+        B[15] = rol(A[4] ^ D[4], 27);
+        B[16] = rol(A[5] ^ D[0], 36);
+        B[1] = rol(A[6] ^ D[1], 44);
+        B[11] = rol(A[7] ^ D[2], 6);
+        B[21] = rol(A[8] ^ D[3], 55);
+        B[6] = rol(A[9] ^ D[4], 20);
+        B[7] = rol(A[10] ^ D[0], 3);
+        B[17] = rol(A[11] ^ D[1], 10);
+        B[2] = rol(A[12] ^ D[2], 43);
+        B[12] = rol(A[13] ^ D[3], 25);
+        B[22] = rol(A[14] ^ D[4], 39);
+        B[23] = rol(A[15] ^ D[0], 41);
+        B[8] = rol(A[16] ^ D[1], 45);
+        B[18] = rol(A[17] ^ D[2], 15);
+        B[3] = rol(A[18] ^ D[3], 21);
+        B[13] = rol(A[19] ^ D[4], 8);
+        B[14] = rol(A[20] ^ D[0], 18);
+        B[24] = rol(A[21] ^ D[1], 2);
+        B[9] = rol(A[22] ^ D[2], 61);
+        B[19] = rol(A[23] ^ D[3], 56);
+        B[4] = rol(A[24] ^ D[4], 14);
+        for (int y = 0; y < 25; y += 5)
+            for (int x = 0; x < 5; x++) A[y + x] = B[y + x] ^ (~B[y + (x + 1) % 5] & B[y + (x + 2) % 5]);
+        A[0] ^= kRC[round];
+    }
+}
+
+}  // namespace
+
+void host_sha256(const uint8_t *in, size_t len, uint8_t out[32])
+{
+    uint32_t s[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a, 0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+    const size_t full = len / 64;
+    sha256_blocks(s, in, full);
+    uint8_t tail[128] = {0};
+    const size_t rem = len - 64 * full;
+    memcpy(tail, in + 64 * full, rem);
+    tail[rem] = 0x80;
+    const size_t tl = rem + 9 <= 64 ? 64 : 128;
+    const uint64_t bits = (uint64_t)len * 8;
+    for (int i = 0; i < 8; i++) tail[tl - 1 - i] = (uint8_t)(bits >> (8 * i));
+    sha256_blocks(s, tail, tl / 64);
+    for (int i = 0; i < 8; i++) {
+        out[4 * i] = (uint8_t)(s[i] >> 24);
+        out[4 * i + 1] = (uint8_t)(s[i] >> 16);
+        out[4 * i + 2] = (uint8_t)(s[i] >> 8);
+        out[4 * i + 3] = (uint8_t)s[i];
+    }
+}
+
+void host_keccak256(const uint8_t *in, size_t len, uint8_t out[32])
+{
+    constexpr size_t kRate = 136;
+    uint64_t A[25] = {0};
+    size_t off = 0;
+    while (len - off >= kRate) {
+        for (size_t i = 0; i < kRate / 8; i++) {
+            uint64_t w;
+            memcpy(&w, in + off + 8 * i, 8);
+            A[i] ^= w;
+        }
+        keccak_f(A);
+        off += kRate;
+    }
+    uint8_t blk[kRate] = {0};
+    memcpy(blk, in + off, len - off);
+    blk[len - off] ^= 0x01;
+    blk[kRate - 1] ^= 0x80;
+    for (size_t i = 0; i < kRate / 8; i++) {
+        uint64_t w;
+        memcpy(&w, blk + 8 * i, 8);
+        A[i] ^= w;
+    }
+    keccak_f(A);
+    memcpy(out, A, 32);
+}
+
+void host_hash(int kind, const uint8_t *in, size_t len, uint8_t out[32])
+{
+    uint8_t tmp[32];
+    if (kind == 1) host_keccak256(in, len, tmp);
+    else host_sha256(in, len, tmp);
+    memcpy(out, tmp, 32);
+}
+
+bool host_sha_extensions() { return kHasSha; }
+
+// M31::from_bytes of the first four digest bytes (include/field/M31.hpp:111-116)
+static uint32_t digest_to_m31(const uint8_t d[32])
+{
+    uint32_t x;
+    memcpy(&x, d, 4);
+    x = (x & 0x7fffffffu) + (x >> 31);
+    if (x >= 0x7fffffffu) x -= 0x7fffffffu;
+    return x;
+}
+
+void host_transcript_prefix(int kind, const uint8_t *commit, size_t commit_len, uint32_t R, uint32_t mlast,
+                            uint32_t grinding_bits, uint8_t grind_out[32], uint32_t *rz1, uint32_t tstate[10])
+{
+    uint8_t digest[32];
+    // grind: challenge_fs(256 / 31 = 8); the first hashes the commitment bytes, the
+    // others re-hash the digest; each is written as 4 LE bytes (ceil(31, 8) = 4).
+    uint8_t hb[32] = {0};
+    host_hash(kind, commit, commit_len, digest);
+    for (int i = 0; i < 8; i++) {
+        if (i > 0) host_hash(kind, digest, 32, digest);
+        uint32_t h = digest_to_m31(digest);
+        memcpy(hb + 4 * i, &h, 4);
+    }
+    for (uint32_t i = 0; i < (1u << grinding_bits); i++) host_hash(kind, hb, 32, hb);
+    memcpy(grind_out, hb, 32);
+    // output-layer challenges rz1[j][i], var-outer / rep-inner; the first hashes the
+    // 32 grind bytes, the rest re-hash the digest.
+    bool pending = true;
+    for (uint32_t i = 0; i < mlast; i++)
+        for (uint32_t j = 0; j < R; j++) {
+            if (pending) {
+                host_hash(kind, hb, 32, digest);
+                pending = false;
+            } else {
+                host_hash(kind, digest, 32, digest);
+            }
+            rz1[(size_t)j * mlast + i] = digest_to_m31(digest);
+        }
+    // transcript state relative to the tail that starts at the grind bytes
+    tstate[0] = 32;
+    tstate[1] = pending ? 0 : 32;
+    memcpy(&tstate[2], digest, 32);
+}
+
+}  // namespace gkrx

@@ -1,0 +1,23 @@
+// Host transcript hashes and the host part of the transcript (see host_hash.cpp).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace gkrx {
+
+void host_sha256(const uint8_t *in, size_t len, uint8_t out[32]);
+void host_keccak256(const uint8_t *in, size_t len, uint8_t out[32]);
+// kind: 0 = SHA-256, 1 = Keccak-256; out may alias in.
+void host_hash(int kind, const uint8_t *in, size_t len, uint8_t out[32]);
+bool host_sha_extensions();
+
+// Prover::prove up to the first layer sumcheck (include/LinearGKR/LinearGKR.hpp:62-69,
+// grind :13-32, include/LinearGKR/gkr.hpp:42-49): hash of the commitment, grinding,
+// then R * mlast output challenges rz1[j * mlast + i].  tstate receives the transcript
+// state relative to the proof tail that begins with the 32 grind bytes:
+// [0] bytes in the tail, [1] start of the unhashed suffix, [2..9] digest.
+void host_transcript_prefix(int kind, const uint8_t *commit, size_t commit_len, uint32_t R, uint32_t mlast,
+                            uint32_t grinding_bits, uint8_t grind_out[32], uint32_t *rz1, uint32_t tstate[10]);
+
+}  // namespace gkrx

@@ -1,0 +1,95 @@
+// Internal structures shared by the C-ABI layer (capi.cpp) and the device prover
+// (prover_m31.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gkrx.h"
+
+namespace gkrx {
+
+void set_error(const std::string &msg);
+
+#define GKRX_HIP_CHECK(expr)                                                                  \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            ::gkrx::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));             \
+            return GKRX_ERR_DEVICE;                                                           \
+        }                                                                                     \
+    } while (0)
+
+constexpr uint32_t kAddMark = 0xffffffffu;  // "no second input" in a CSR entry
+constexpr int kMaxVars = 30;
+
+// One layer in device form.  Every table is in the reference's natural index order
+// (entry i of a multilinear table is the hypercube point with little-endian bits i).
+struct LayerDev {
+    uint32_t n = 0, m = 0;       // nb_input_vars, nb_output_vars
+    uint64_t v_off = 0;          // u32 offset of the layer input values in the value buffer
+    uint64_t out_off = 0;        // u32 offset of the layer output values
+    // forward CSR by output index: entry {in0, in1 or kAddMark, coef}
+    uint32_t *fw_ptr = nullptr, *fw_ent = nullptr;
+    // g_x CSR by first input x: entry {in1 or kAddMark, z, coef}
+    uint32_t *gx_ptr = nullptr, *gx_ent = nullptr;
+    // h_y CSR by second input y (mul gates only): entry {x, z, coef}
+    uint32_t *hy_ptr = nullptr, *hy_ent = nullptr;
+};
+
+struct DevCircuit {
+    uint32_t D = 0;
+    uint32_t L = 0;                  // lanes
+    std::vector<LayerDev> layers;
+    uint32_t *csr_pool = nullptr;    // one allocation holding every CSR
+    uint32_t *values = nullptr;      // all layer values, D + 1 tables of L lanes
+    uint64_t values_u32 = 0;
+    uint32_t max_n = 0, max_m = 0;
+};
+
+// Device scratch of one prover context, grown to the largest circuit seen.
+struct DevScratch {
+    uint64_t bytes = 0;
+    void *pool = nullptr;
+    uint32_t *eqz = nullptr;      // R * maxM
+    uint32_t *eqx = nullptr;      // R * maxN
+    uint32_t *hg0 = nullptr;      // R * maxN * L    (phase-1 round-0 hg)
+    uint32_t *hgA = nullptr;      // R * maxN/2 * L
+    uint32_t *hgB = nullptr;      // R * maxN/4 * L
+    uint32_t *fA = nullptr;       // R * maxN/2 * L
+    uint32_t *fB = nullptr;       // R * maxN/4 * L
+    uint32_t *s0 = nullptr;       // R * maxN        (phase-2 scalar h table)
+    uint32_t *sA = nullptr;       // R * maxN/2
+    uint32_t *sB = nullptr;       // R * maxN/4
+    uint32_t *partials = nullptr; // R * maxBlocks * 3L
+    uint32_t *chal = nullptr;     // D * R * 2 * 32 (rx then ry per layer and rep) + rz1 seed
+    uint32_t *ab = nullptr;       // D * 2 (alpha, beta after each layer)
+    uint32_t *claims = nullptr;   // R * L
+    uint32_t *vxy = nullptr;      // R * L (vx) + R * L (vy)
+    uint32_t *tstate = nullptr;   // transcript state: len, hash_start, digest[8]
+    uint32_t *tail = nullptr;     // transcript bytes after the commitment
+    uint64_t tail_cap = 0;        // bytes
+    uint32_t max_n = 0, max_m = 0, L = 0, D = 0;
+};
+
+struct ProveTimings {
+    double host_prefix_ms = 0, device_ms = 0, total_ms = 0;
+};
+
+// Device entry points implemented in prover_m31.hip
+int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &n, const std::vector<uint32_t> &m,
+                       const std::vector<std::vector<uint32_t>> &mul, const std::vector<std::vector<uint32_t>> &add);
+void dev_free_circuit(DevCircuit &dc);
+int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t tail_bytes);
+void dev_free_scratch(DevScratch &s);
+int dev_forward(const DevCircuit &dc, hipStream_t st);
+// Runs the on-device part of the prove: output claims, every layer sumcheck and the
+// transcript from the state in `tstate_host` (len, hash_start, digest) and the tail
+// prefix already in s.tail.  rz1 (R * m_last) seeds the last layer.
+int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, const uint32_t *rz1_host,
+              const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st);
+
+}  // namespace gkrx

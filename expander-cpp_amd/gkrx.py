@@ -1,0 +1,266 @@
+"""ctypes binding of libgkrx.so (the C ABI declared in include/gkrx.h).
+
+Thin host-side mirror of the reference's C++ surface (PolyhedraZK/Expander-cpp):
+``Config`` (include/configuration/config.hpp:26-86), ``Circuit``
+(include/circuit/circuit.hpp:118-305), ``Prover.prove`` and ``Verifier.verify``
+(include/LinearGKR/LinearGKR.hpp:34-150).  Arrays are numpy uint32; proofs are bytes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgkrx.so")
+
+FIELD_M31, FIELD_BABYBEAR, FIELD_BN254 = 0, 1, 2
+FS_SHA256, FS_KECCAK256 = 0, 1
+PC_RAW = 0
+P31 = (1 << 31) - 1
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class GkrxError(RuntimeError):
+    pass
+
+
+class _Config(C.Structure):
+    _fields_ = [("field_type", C.c_int32), ("fs_hash", C.c_int32), ("pc_type", C.c_int32),
+                ("security_bits", C.c_int32), ("grinding_bits", C.c_int32), ("nb_parallel", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libgkrx.so; raises if the native library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GkrxError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.gkrx_last_error.restype = C.c_char_p
+    L.gkrx_config_default.argtypes = [C.POINTER(_Config)]
+    L.gkrx_config_num_repetitions.argtypes = [C.POINTER(_Config)]
+    L.gkrx_ctx_create.argtypes = [C.POINTER(_Config), C.c_int, C.POINTER(vp)]
+    L.gkrx_ctx_destroy.argtypes = [vp]
+    L.gkrx_circuit_load_text.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(vp)]
+    L.gkrx_circuit_from_arrays.argtypes = [vp, C.c_uint32, _u32p, C.POINTER(C.c_void_p), _u32p,
+                                           C.POINTER(C.c_void_p), C.POINTER(vp)]
+    L.gkrx_circuit_shape.argtypes = [vp, C.POINTER(C.c_uint32), vp, vp]
+    L.gkrx_circuit_write_text.argtypes = [vp, C.c_char_p, C.c_char_p]
+    L.gkrx_circuit_set_witness.argtypes = [vp, _u32p, C.c_size_t]
+    L.gkrx_circuit_evaluate.argtypes = [vp]
+    L.gkrx_circuit_get_output.argtypes = [vp, _u32p, C.c_size_t]
+    L.gkrx_circuit_get_layer_values.argtypes = [vp, C.c_uint32, _u32p, C.c_size_t]
+    L.gkrx_circuit_destroy.argtypes = [vp]
+    L.gkrx_proof_size.argtypes = [vp, vp]
+    L.gkrx_proof_size.restype = C.c_size_t
+    L.gkrx_prove.argtypes = [vp, vp, _u8p, C.c_size_t, C.POINTER(C.c_size_t), _u32p]
+    L.gkrx_verify.argtypes = [vp, vp, _u32p, _u8p, C.c_size_t]
+    L.gkrx_last_timing.argtypes = [vp, C.POINTER(C.c_double)]
+    L.gkrx_synth_layer.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u32p,
+                                   C.POINTER(C.c_uint32), _u32p, C.POINTER(C.c_uint32)]
+    L.gkrx_synth_witness_m31.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _u32p]
+    L.gkrx_hash.argtypes = [C.c_int, C.c_char_p, C.c_size_t, C.c_char_p]
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise GkrxError(f"{what} failed ({rc}): {lib().gkrx_last_error().decode(errors='replace')}")
+    return rc
+
+
+# ----------------------------------------------------------------- synthetic inputs
+def synth_layer(seed: int, layer: int, log_width: int, add_permille: int, locality: bool):
+    """Gates of one synthetic layer: (mul[u,4], add[a,3]) uint32 arrays."""
+    W = 1 << log_width
+    mul = np.empty(4 * W, dtype=np.uint32)
+    add = np.empty(3 * W, dtype=np.uint32)
+    nm, na = C.c_uint32(), C.c_uint32()
+    _check(lib().gkrx_synth_layer(seed, layer, log_width, add_permille, int(bool(locality)), mul, C.byref(nm),
+                                  add, C.byref(na)), "gkrx_synth_layer")
+    return mul[:4 * nm.value].reshape(-1, 4).copy(), add[:3 * na.value].reshape(-1, 3).copy()
+
+
+def synth_circuit(seed: int, depth: int, log_width: int, add_permille: int, locality: bool):
+    """List of (mul, add) per layer, layer 0 = input layer."""
+    return [synth_layer(seed, l, log_width, add_permille, locality) for l in range(depth)]
+
+
+def synth_witness(seed: int, n_elems: int, lanes: int) -> np.ndarray:
+    out = np.empty(n_elems * lanes, dtype=np.uint32)
+    _check(lib().gkrx_synth_witness_m31(seed, n_elems, lanes, out), "gkrx_synth_witness_m31")
+    return out
+
+
+def write_text(layers, mul_path: str, add_path: str) -> None:
+    """Write (mul, add) layer lists in the reference text format
+    (Circuit::load_extracted_gates, include/circuit/circuit.hpp:152-215): one line per
+    layer, output layer first."""
+    with open(mul_path, "w") as fm, open(add_path, "w") as fa:
+        for mul, add in reversed(layers):
+            fm.write(" ".join([str(len(mul))] + [str(int(x)) for x in np.asarray(mul).ravel()]) + "\n")
+            fa.write(" ".join([str(len(add))] + [str(int(x)) for x in np.asarray(add).ravel()]) + "\n")
+
+
+def host_hash(fs_hash: int, data: bytes) -> bytes:
+    out = C.create_string_buffer(32)
+    _check(lib().gkrx_hash(fs_hash, data, len(data), out), "gkrx_hash")
+    return out.raw
+
+
+# ----------------------------------------------------------------- API mirror
+class Config:
+    """gkr::Config (include/configuration/config.hpp:26-86); nb_parallel = lanes."""
+
+    def __init__(self, fs_hash: int = FS_SHA256, nb_parallel: int = 8, field_type: int = FIELD_M31,
+                 grinding_bits: int = 10, security_bits: int = 100, pc_type: int = PC_RAW):
+        self._c = _Config()
+        lib().gkrx_config_default(C.byref(self._c))
+        self._c.fs_hash = fs_hash
+        self._c.nb_parallel = nb_parallel
+        self._c.field_type = field_type
+        self._c.grinding_bits = grinding_bits
+        self._c.security_bits = security_bits
+        self._c.pc_type = pc_type
+
+    @property
+    def lanes(self) -> int:
+        return self._c.nb_parallel
+
+    def get_num_repetitions(self) -> int:
+        return _check(lib().gkrx_config_num_repetitions(C.byref(self._c)), "num_repetitions")
+
+
+class Context:
+    """A prover context bound to one GPU (Prover::Prover, LinearGKR.hpp:42-47)."""
+
+    def __init__(self, config: Config, device: int = 0):
+        self.config = config
+        h = C.c_void_p()
+        _check(lib().gkrx_ctx_create(C.byref(config._c), device, C.byref(h)), "gkrx_ctx_create")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().gkrx_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Circuit:
+    """gkr::Circuit on the device."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self._h = handle
+        n = C.c_uint32()
+        _check(lib().gkrx_circuit_shape(handle, C.byref(n), None, None), "shape")
+        self.nb_layers = n.value
+        ni = np.zeros(n.value, dtype=np.uint32)
+        no = np.zeros(n.value, dtype=np.uint32)
+        _check(lib().gkrx_circuit_shape(handle, C.byref(n), ni.ctypes.data, no.ctypes.data), "shape")
+        self.nb_input_vars = [int(x) for x in ni]
+        self.nb_output_vars = [int(x) for x in no]
+
+    @classmethod
+    def load_extracted_gates(cls, ctx: Context, mul_path: str, add_path: str) -> "Circuit":
+        h = C.c_void_p()
+        _check(lib().gkrx_circuit_load_text(ctx._h, mul_path.encode(), add_path.encode(), C.byref(h)),
+               "gkrx_circuit_load_text")
+        return cls(ctx, h)
+
+    @classmethod
+    def from_layers(cls, ctx: Context, layers) -> "Circuit":
+        D = len(layers)
+        muls = [np.ascontiguousarray(m, dtype=np.uint32).ravel() for m, _ in layers]
+        adds = [np.ascontiguousarray(a, dtype=np.uint32).ravel() for _, a in layers]
+        nm = np.array([len(m) // 4 for m in muls], dtype=np.uint32)
+        na = np.array([len(a) // 3 for a in adds], dtype=np.uint32)
+        mp = (C.c_void_p * D)(*[m.ctypes.data for m in muls])
+        ap = (C.c_void_p * D)(*[a.ctypes.data for a in adds])
+        h = C.c_void_p()
+        _check(lib().gkrx_circuit_from_arrays(ctx._h, D, nm, mp, na, ap, C.byref(h)), "gkrx_circuit_from_arrays")
+        return cls(ctx, h)
+
+    def set_witness(self, lanes: np.ndarray) -> None:
+        w = np.ascontiguousarray(lanes, dtype=np.uint32).ravel()
+        _check(lib().gkrx_circuit_set_witness(self._h, w, w.size), "gkrx_circuit_set_witness")
+
+    def evaluate(self) -> None:
+        _check(lib().gkrx_circuit_evaluate(self._h), "gkrx_circuit_evaluate")
+
+    def layer_values(self, layer: int) -> np.ndarray:
+        L = self.ctx.config.lanes
+        vars_ = self.nb_input_vars[layer] if layer < self.nb_layers else self.nb_output_vars[-1]
+        out = np.empty((1 << vars_) * L, dtype=np.uint32)
+        _check(lib().gkrx_circuit_get_layer_values(self._h, layer, out, out.size), "get_layer_values")
+        return out
+
+    def output(self) -> np.ndarray:
+        return self.layer_values(self.nb_layers)
+
+    def write_text(self, mul_path: str, add_path: str) -> None:
+        _check(lib().gkrx_circuit_write_text(self._h, mul_path.encode(), add_path.encode()), "write_text")
+
+    def close(self):
+        if self._h:
+            lib().gkrx_circuit_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Prover:
+    """gkr::Prover::prove (LinearGKR.hpp:58-90): returns (claimed_v[R, L], proof bytes)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def proof_size(self, circuit: Circuit) -> int:
+        return lib().gkrx_proof_size(self.ctx._h, circuit._h)
+
+    def prove(self, circuit: Circuit, out: np.ndarray | None = None):
+        size = self.proof_size(circuit)
+        buf = out if out is not None else np.empty(size, dtype=np.uint8)
+        R = self.ctx.config.get_num_repetitions()
+        claimed = np.empty(R * self.ctx.config.lanes, dtype=np.uint32)
+        n = C.c_size_t()
+        _check(lib().gkrx_prove(self.ctx._h, circuit._h, buf, buf.size, C.byref(n), claimed), "gkrx_prove")
+        return claimed.reshape(R, -1), buf[:n.value]
+
+    def last_timing(self):
+        ms = (C.c_double * 3)()
+        _check(lib().gkrx_last_timing(self.ctx._h, ms), "last_timing")
+        return {"host_prefix_ms": ms[0], "device_ms": ms[1], "total_ms": ms[2]}
+
+
+class Verifier:
+    """gkr::Verifier::verify (LinearGKR.hpp:104-149), on the host."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def verify(self, circuit: Circuit, claimed_v: np.ndarray, proof) -> bool:
+        cv = np.ascontiguousarray(claimed_v, dtype=np.uint32).ravel()
+        pb = np.frombuffer(bytes(proof), dtype=np.uint8) if not isinstance(proof, np.ndarray) else proof
+        pb = np.ascontiguousarray(pb, dtype=np.uint8)
+        return _check(lib().gkrx_verify(self.ctx._h, circuit._h, cv, pb, pb.size), "gkrx_verify") == 1

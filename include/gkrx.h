@@ -1,0 +1,124 @@
+/*
+ * gkrx -- C ABI of the MI355X GKR prover.
+ *
+ * The reference (PolyhedraZK/Expander-cpp) is header-only C++ with no ABI.  Each entry
+ * point below replaces one piece of its C++ surface; the citation names the reference
+ * interface it stands in for (paths relative to the reference root).  The C++ drop-in
+ * classes (gkr::Config, gkr::Circuit, gkr::Prover, gkr::Verifier, gkr::Proof) in
+ * include/gkrx/expander.hpp are thin wrappers over these calls.
+ *
+ * Conventions: plain pointers and sizes; every call returns an int status (GKRX_OK = 0,
+ * negative on error) unless noted; gkrx_last_error() gives a message for the calling
+ * thread.  Field elements of M31 with L lanes are L little-endian u32 values (canonical,
+ * < 2^31 - 1), lane 0 first -- the bytes F::to_bytes writes (include/field/M31.hpp:420-426).
+ */
+#ifndef GKRX_H
+#define GKRX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GKRX_OK 0
+#define GKRX_ERR_ARG (-1)     /* bad argument or misuse (the reference asserts) */
+#define GKRX_ERR_CONFIG (-2)  /* unsupported Config (reference: assert in Prover ctor, LinearGKR.hpp:42-47) */
+#define GKRX_ERR_CIRCUIT (-3) /* malformed circuit file or unsupported shape */
+#define GKRX_ERR_IO (-4)      /* file could not be opened */
+#define GKRX_ERR_NOMEM (-5)   /* host or device allocation failed */
+#define GKRX_ERR_DEVICE (-6)  /* HIP runtime error or no usable GPU */
+#define GKRX_ERR_STATE (-7)   /* call order (e.g. prove before a witness is set) */
+
+/* Enumerations mirror include/configuration/config.hpp:5-24. */
+enum { GKRX_PC_RAW = 0, GKRX_PC_KZG = 1, GKRX_PC_ORION = 2, GKRX_PC_FRI = 3 };
+enum { GKRX_FIELD_M31 = 0, GKRX_FIELD_BABYBEAR = 1, GKRX_FIELD_BN254 = 2 };
+enum { GKRX_FS_SHA256 = 0, GKRX_FS_KECCAK256 = 1, GKRX_FS_POSEIDON = 2, GKRX_FS_ANIMOE = 3, GKRX_FS_MIMC7 = 4 };
+
+/* Config fields (include/configuration/config.hpp:26-86).  nb_parallel is the lane
+ * count L of the data-parallel M31 field (8 or 16). */
+typedef struct gkrx_config {
+    int32_t field_type;
+    int32_t fs_hash;
+    int32_t pc_type;
+    int32_t security_bits;
+    int32_t grinding_bits;
+    int32_t nb_parallel;
+} gkrx_config;
+
+typedef struct gkrx_ctx gkrx_ctx;
+typedef struct gkrx_circuit gkrx_circuit;
+
+/* Config::Config (config.hpp:71-81): M31, SHA-256, Raw, 100-bit security, 10 grinding
+ * bits; nb_parallel = 8, the lane count the shipped x86 reference uses (M31.hpp:234-238). */
+void gkrx_config_default(gkrx_config *cfg);
+/* Config::get_num_repetitions (config.hpp:61-65, 82-85). */
+int gkrx_config_num_repetitions(const gkrx_config *cfg);
+
+/* Prover::Prover(const Config&) (include/LinearGKR/LinearGKR.hpp:42-47) plus the device
+ * resources the prover needs: one context per GPU.  device = HIP device ordinal. */
+int gkrx_ctx_create(const gkrx_config *cfg, int device, gkrx_ctx **out);
+int gkrx_ctx_destroy(gkrx_ctx *ctx);
+
+/* Circuit::load_extracted_gates (include/circuit/circuit.hpp:152-215): the text
+ * format (mul file one line per layer, output layer first). */
+int gkrx_circuit_load_text(gkrx_ctx *ctx, const char *mul_path, const char *add_path, gkrx_circuit **out);
+/* Circuit built from gate arrays, layer 0 = input layer as in Circuit::layers.
+ * mul[l]: n_mul[l] x (in0, in1, out, coef); add[l]: n_add[l] x (in0, out, coef).
+ * Variable counts follow Circuit::_compute_nb_vars (circuit.hpp:124-149). */
+int gkrx_circuit_from_arrays(gkrx_ctx *ctx, uint32_t nb_layers, const uint32_t *n_mul, const uint32_t *const *mul,
+                             const uint32_t *n_add, const uint32_t *const *add, gkrx_circuit **out);
+/* CircuitLayer::nb_input_vars / nb_output_vars, per layer (arrays of nb_layers). */
+int gkrx_circuit_shape(const gkrx_circuit *c, uint32_t *nb_layers, uint32_t *nb_in_vars, uint32_t *nb_out_vars);
+/* Writes the circuit in the text format read by load_extracted_gates. */
+int gkrx_circuit_write_text(const gkrx_circuit *c, const char *mul_path, const char *add_path);
+/* layers[0].input_layer_vals.evals = witness: 2^nb_input_vars(0) elements x L lanes (host memory). */
+int gkrx_circuit_set_witness(gkrx_circuit *c, const uint32_t *lanes, size_t n_u32);
+/* Circuit::evaluate (circuit.hpp:273-280), on the device; every layer stays in HBM. */
+int gkrx_circuit_evaluate(gkrx_circuit *c);
+/* layers.back().output_layer_vals.evals (2^nb_output_vars(D-1) x L u32). */
+int gkrx_circuit_get_output(gkrx_circuit *c, uint32_t *out, size_t n_u32);
+/* Input values of layer l (2^nb_input_vars(l) x L u32); l == nb_layers gives the output. */
+int gkrx_circuit_get_layer_values(gkrx_circuit *c, uint32_t layer, uint32_t *out, size_t n_u32);
+int gkrx_circuit_destroy(gkrx_circuit *c);
+
+/* Bytes of the proof Prover::prove returns for this circuit. */
+size_t gkrx_proof_size(const gkrx_ctx *ctx, const gkrx_circuit *c);
+/* Prover::prove (LinearGKR.hpp:58-90): evaluates the circuit if needed and proves it.
+ * proof: caller buffer of at least gkrx_proof_size bytes; *proof_len receives the
+ * length.  claimed_v: R * L u32 (R = gkrx_config_num_repetitions), the first element
+ * of the tuple prove returns. */
+int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap, size_t *proof_len,
+               uint32_t *claimed_v);
+/* Verifier::verify (LinearGKR.hpp:104-149), on the host.  Returns 1 = accepted,
+ * 0 = rejected, negative = error. */
+int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v, const uint8_t *proof,
+                size_t proof_len);
+
+/* Millisecond split of the last gkrx_prove on this context: [0] host transcript prefix
+ * (commitment hash + grinding + output challenges), [1] device work, [2] total. */
+int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3);
+
+/* Thread-local message of the last failed call. */
+const char *gkrx_last_error(void);
+
+/* Synthetic benchmark inputs (no reference counterpart: the reference draws wiring from
+ * SparseCircuitConnection::random, circuit.hpp:40-65, and witnesses from rand(),
+ * circuit.hpp:287-295).  One gate per output index of a 2^log_width layer; add with
+ * probability add_permille / 1000; coefficient 1; with locality, 80% of input indices
+ * fall within +-1024 of the output index; the largest input index is forced to
+ * 2^log_width - 1.  mul_out holds up to 2^log_width x 4 u32, add_out up to 2^log_width x 3. */
+int gkrx_synth_layer(uint64_t seed, uint32_t layer, uint32_t log_width, uint32_t add_permille, uint32_t locality,
+                     uint32_t *mul_out, uint32_t *n_mul, uint32_t *add_out, uint32_t *n_add);
+/* n_elems x lanes M31 values, uniform in [0, p) by rejection. */
+int gkrx_synth_witness_m31(uint64_t seed, uint64_t n_elems, uint32_t lanes, uint32_t *out);
+
+/* Transcript hashes of the library (host implementations), for tests. */
+int gkrx_hash(int fs_hash, const uint8_t *in, size_t len, uint8_t out[32]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GKRX_H */

@@ -1,0 +1,131 @@
+// Driver that runs the REFERENCE prover / verifier, compiled directly from the
+// reference's own sources (include/ and lib/btc_sha256) by oracle/Makefile.
+// TEST INFRASTRUCTURE ONLY: it produces and checks golden fixtures and times the
+// reference CPU prover for the benchmark's cpu_baseline leg.  Nothing in the product
+// links it.
+//
+//   ref_driver prove  <mul.txt> <add.txt> <witness.bin> <proof.out> <claimed.out> [reps]
+//   ref_driver verify <mul.txt> <add.txt> <claimed.bin> <proof.bin>     (exit 0 = accepted)
+//   ref_driver time   <mul.txt> <add.txt> <witness.bin>                 (prints seconds of prove)
+//
+// Build variants (oracle/Makefile):
+//   * as shipped: VectorizedM31 = 8 lanes, SHA-256 transcript (include/hash/hashes.hpp:22-31);
+//   * GKRX_REF_KECCAK: the CSHA256 object inside SHA256Hasher is swapped for a Keccak-256
+//     class with the same Reset/Write/Finalize surface (the oracle's Keccak), so the
+//     reference's own transcript framing runs with the Keccak-256 hash;
+//   * 16 lanes: the recipe rewrites `vectorize_size = 1` to 2 (include/field/M31.hpp:237)
+//     in the preprocessed translation unit, in a pipe, without copying any source.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#ifdef GKRX_REF_KECCAK
+#include "btc_sha256/crypto/sha256.h"
+extern "C" void oracle_keccak256(const unsigned char *in, size_t len, unsigned char out[32]);
+class OracleKeccak256Ctx {
+    std::vector<unsigned char> buf;
+
+  public:
+    OracleKeccak256Ctx &Reset() { buf.clear(); return *this; }
+    OracleKeccak256Ctx &Write(const unsigned char *d, size_t n) { buf.insert(buf.end(), d, d + n); return *this; }
+    void Finalize(unsigned char out[32]) { oracle_keccak256(buf.data(), buf.size(), out); }
+};
+#define CSHA256 OracleKeccak256Ctx
+#endif
+
+#include "LinearGKR/LinearGKR.hpp"
+
+using F = gkr::M31_field::VectorizedM31;
+using FP = gkr::M31_field::M31;
+
+static std::vector<unsigned char> read_file(const char *p)
+{
+    FILE *f = fopen(p, "rb");
+    if (!f) { fprintf(stderr, "cannot open %s\n", p); exit(2); }
+    std::vector<unsigned char> b;
+    unsigned char tmp[1 << 16];
+    size_t n;
+    while ((n = fread(tmp, 1, sizeof tmp, f)) > 0) b.insert(b.end(), tmp, tmp + n);
+    fclose(f);
+    return b;
+}
+
+static void write_file(const char *p, const unsigned char *d, size_t n)
+{
+    FILE *f = fopen(p, "wb");
+    if (!f) { fprintf(stderr, "cannot write %s\n", p); exit(2); }
+    fwrite(d, 1, n, f);
+    fclose(f);
+}
+
+static void set_witness(gkr::Circuit<F, FP> &c, const std::vector<unsigned char> &w)
+{
+    size_t N0 = (size_t)1 << c.layers[0].nb_input_vars;
+    if (w.size() != N0 * sizeof(F)) {
+        fprintf(stderr, "witness has %zu bytes, expected %zu\n", w.size(), N0 * sizeof(F));
+        exit(2);
+    }
+    auto &ev = c.layers[0].input_layer_vals.evals;
+    ev.resize(N0);
+    for (size_t i = 0; i < N0; i++) ev[i].from_bytes(w.data() + i * sizeof(F));
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) return 2;
+    std::string cmd = argv[1];
+    if ((cmd == "prove" && argc < 7) || (cmd == "verify" && argc < 6) || (cmd == "time" && argc < 5)) {
+        fprintf(stderr, "bad arguments\n");
+        return 2;
+    }
+    auto circuit = gkr::Circuit<F, FP>::load_extracted_gates(argv[2], argv[3]);
+    Config config;
+    if (cmd == "prove" || cmd == "time") {
+        set_witness(circuit, read_file(argv[4]));
+        circuit.evaluate();
+        gkr::Prover<F, FP> prover(config);
+        int reps = (cmd == "time" && argc > 5) ? atoi(argv[5]) : 1;
+        double best = 1e30;
+        std::vector<F> claimed;
+        gkr::Proof<F> proof;
+        for (int it = 0; it < reps; it++) {
+            auto t0 = std::chrono::steady_clock::now();
+            prover.prepare_mem(circuit);
+            auto res = prover.prove(circuit);
+            auto t1 = std::chrono::steady_clock::now();
+            double s = std::chrono::duration<double>(t1 - t0).count();
+            if (s < best) best = s;
+            claimed = std::get<0>(res);
+            proof = std::get<1>(res);
+        }
+        if (cmd == "time") {
+            printf("%.6f\n", best);
+            return 0;
+        }
+        write_file(argv[5], proof.bytes.data(), proof.bytes.size());
+        std::vector<unsigned char> cb(claimed.size() * sizeof(F));
+        for (size_t j = 0; j < claimed.size(); j++) claimed[j].to_bytes(cb.data() + j * sizeof(F));
+        write_file(argv[6], cb.data(), cb.size());
+        return 0;
+    }
+    if (cmd == "verify") {
+        size_t N0 = (size_t)1 << circuit.layers[0].nb_input_vars;
+        circuit.layers[0].input_layer_vals.evals.resize(N0);
+        auto cb = read_file(argv[4]);
+        std::vector<F> claimed(cb.size() / sizeof(F));
+        for (size_t j = 0; j < claimed.size(); j++) claimed[j].from_bytes(cb.data() + j * sizeof(F));
+        gkr::Proof<F> proof;
+        proof.bytes = read_file(argv[5]);
+        gkr::Verifier verifier(config);
+        // the reference verifier prints debug sizes on stdout (LinearGKR.hpp:143-144)
+        fflush(stdout);
+        FILE *saved = stdout;
+        (void)saved;
+        bool ok = verifier.verify(circuit, claimed, proof);
+        fprintf(stderr, "reference verifier: %s\n", ok ? "accepted" : "rejected");
+        return ok ? 0 : 3;
+    }
+    return 2;
+}
