@@ -2,12 +2,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gkrx.h"
@@ -23,7 +25,10 @@ struct gkrx_ctx {
     hipStream_t stream = nullptr;
     gkrx::DevScratch scratch;
     gkrx::ProveTimings last;
+    gkrx::Profiler prof;
+    gkrx::BatchScratch batch;
     std::mutex mu;
+    std::atomic<int> live_circuits{0};  // circuits bound to this context
 };
 
 struct gkrx_circuit {
@@ -117,9 +122,14 @@ extern "C" int gkrx_ctx_create(const gkrx_config *cfg, int device, gkrx_ctx **ou
 extern "C" int gkrx_ctx_destroy(gkrx_ctx *ctx)
 {
     if (!ctx) return GKRX_ERR_ARG;
+    if (ctx->live_circuits.load() != 0) {
+        set_error("context still has " + std::to_string(ctx->live_circuits.load()) + " circuit(s); destroy them first");
+        return GKRX_ERR_STATE;
+    }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     dev_free_scratch(ctx->scratch);
+    dev_free_batch(ctx->batch);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GKRX_OK;
@@ -193,6 +203,7 @@ static int finish_circuit(gkrx_ctx *ctx, gkrx_circuit *c)
     int rc = dev_upload_circuit(c->dev, ctx->L, c->n, c->m, c->mul, c->add);
     if (rc) return rc;
     c->ctx = ctx;
+    ctx->live_circuits++;
     return GKRX_OK;
 }
 
@@ -372,7 +383,8 @@ static int evaluate_locked(gkrx_circuit *c)
         set_error("no witness set");
         return GKRX_ERR_STATE;
     }
-    int rc = dev_forward(c->dev, c->ctx->stream);
+    gkrx_ctx *ctx = c->ctx;
+    int rc = dev_forward(c->dev, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr);
     if (rc) return rc;
     c->evaluated = true;
     return GKRX_OK;
@@ -418,6 +430,7 @@ extern "C" int gkrx_circuit_destroy(gkrx_circuit *c)
         (void)hipSetDevice(c->ctx->device);
         (void)hipStreamSynchronize(c->ctx->stream);
         dev_free_circuit(c->dev);
+        c->ctx->live_circuits--;
     }
     delete c;
     return GKRX_OK;
@@ -467,7 +480,7 @@ extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t
                            (uint32_t)ctx->cfg.grinding_bits, grind, rz1.data(), tstate);
     const auto t1 = std::chrono::steady_clock::now();
     rc = dev_prove(c->dev, ctx->scratch, ctx->R, ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0, rz1.data(), tstate,
-                   grind, 32, ctx->stream);
+                   grind, 32, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr);
     if (rc) return rc;
     uint32_t tfinal[10];
     memcpy(proof, c->witness.data(), commit_len);  // RAW commitment bytes (raw.hpp:19-27)
@@ -477,11 +490,94 @@ extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t
     GKRX_HIP_CHECK(hipMemcpyAsync(tfinal, ctx->scratch.tstate, sizeof tfinal, hipMemcpyDeviceToHost, ctx->stream));
     GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     const auto t2 = std::chrono::steady_clock::now();
+    if (ctx->prof.on) ctx->prof.collect();
     if (tfinal[0] != tb) {
         set_error("device transcript length mismatch: " + std::to_string(tfinal[0]) + " != " + std::to_string(tb));
         return GKRX_ERR_DEVICE;
     }
     *proof_len = total;
+    ctx->last.host_prefix_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, uint8_t *const *proofs,
+                                const size_t *proof_caps, size_t *proof_lens, uint32_t *const *claimed_v)
+{
+    if (!ctx || (n && (!circuits || !proofs || !proof_caps || !proof_lens || !claimed_v))) return GKRX_ERR_ARG;
+    for (uint32_t i = 0; i < n; i++) {
+        gkrx_circuit *c = circuits[i];
+        if (!c || c->ctx != ctx || !proofs[i] || !claimed_v[i]) return GKRX_ERR_ARG;
+        if (!c->has_witness) {
+            set_error("batch item " + std::to_string(i) + " has no witness");
+            return GKRX_ERR_STATE;
+        }
+        if (proof_caps[i] < gkrx_proof_size(ctx, c)) {
+            set_error("batch item " + std::to_string(i) + ": proof buffer too small");
+            return GKRX_ERR_ARG;
+        }
+        if (!dev_batch_supported(c->dev)) {
+            set_error("batch item " + std::to_string(i) + ": layers wider than 2^20 (prove it with gkrx_prove)");
+            return GKRX_ERR_CIRCUIT;
+        }
+    }
+    if (n == 0) return GKRX_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    GKRX_HIP_CHECK(hipSetDevice(ctx->device));
+    const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
+    const size_t S = (size_t)ctx->L * 4;
+    struct Prefix {
+        std::vector<uint32_t> rz1;
+        uint32_t tstate[10];
+        uint8_t grind[32];
+    };
+    std::vector<Prefix> pre(n);
+    // the transcript prefixes are independent serial hash chains: one per host thread,
+    // largest commitment first
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return circuits[a]->n[0] > circuits[b]->n[0]; });
+    std::atomic<uint32_t> next{0};
+    auto worker = [&] {
+        for (uint32_t k; (k = next++) < n;) {
+            const uint32_t i = order[k];
+            gkrx_circuit *c = circuits[i];
+            const uint32_t mlast = c->m[c->D - 1];
+            pre[i].rz1.resize((size_t)ctx->R * mlast);
+            host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()),
+                                   ((size_t)1 << c->n[0]) * S, ctx->R, mlast, (uint32_t)ctx->cfg.grinding_bits,
+                                   pre[i].grind, pre[i].rz1.data(), pre[i].tstate);
+        }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < std::min<unsigned>(hw, n); t++) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+    const auto t1 = std::chrono::steady_clock::now();
+    std::vector<BatchItem> items(n);
+    for (uint32_t i = 0; i < n; i++) {
+        gkrx_circuit *c = circuits[i];
+        const size_t commit_len = ((size_t)1 << c->n[0]) * S;
+        memcpy(proofs[i], c->witness.data(), commit_len);
+        items[i] = BatchItem{&c->dev, pre[i].rz1.data(), pre[i].tstate, pre[i].grind, tail_bytes(ctx, c),
+                             proofs[i] + commit_len};
+    }
+    int rc = dev_prove_batch(items, ctx->R, kind, ctx->batch, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr);
+    if (rc) return rc;
+    GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const uint32_t *cl = ctx->batch.claims_host;
+    if (!cl) return GKRX_ERR_NOMEM;
+    for (uint32_t i = 0; i < n; i++) {
+        memcpy(claimed_v[i], cl + (size_t)i * ctx->R * ctx->L, (size_t)ctx->R * ctx->L * 4);
+        proof_lens[i] = gkrx_proof_size(ctx, circuits[i]);
+        circuits[i]->evaluated = true;
+    }
+    if (ctx->prof.on) ctx->prof.collect();
+    const auto t2 = std::chrono::steady_clock::now();
     ctx->last.host_prefix_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
@@ -503,6 +599,26 @@ extern "C" int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3)
     ms3[0] = ctx->last.host_prefix_ms;
     ms3[1] = ctx->last.device_ms;
     ms3[2] = ctx->last.total_ms;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_ctx_profile(gkrx_ctx *ctx, int enable)
+{
+    if (!ctx) return GKRX_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->prof.reset();
+    ctx->prof.on = enable != 0;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_ctx_kernel_stat(gkrx_ctx *ctx, int idx, const char **name, uint64_t *launches, double *ms)
+{
+    if (!ctx || idx < 0 || idx >= kKClassCount) return GKRX_ERR_ARG;
+    if (name) *name = kernel_class_name(idx);
+    if (launches) *launches = ctx->prof.count[idx];
+    if (ms) *ms = ctx->prof.ms[idx];
     return GKRX_OK;
 }
 

@@ -184,4 +184,148 @@ __device__ __forceinline__ void hash_words(int kind, const uint32_t *msg, uint32
     else sha256_words(msg, nwords, out);
 }
 
+// ------------------------------------------------------------ wave-parallel Keccak
+// One wavefront computes one Keccak-f[1600] permutation with the 25 state words spread
+// over lanes, so one VALU instruction advances every word: lane l belongs to row
+// y = l / 8 (rows 0..4 use lanes 0..39) and holds column x = (l % 8 + 4) % 5.  Lanes with
+// l % 8 in 1..5 are the state proper; l % 8 = 0, 6, 7 carry copies of columns 4, 0, 1, so
+// that the x - 1, x + 1 and x + 2 neighbours of every state lane sit in the same
+// 8-lane group and a DPP row shift reaches them.  Lanes 40..63 stay zero.
+//   theta : column parity = XOR of the 5 rows, formed with a DPP row rotate by 8 (rows
+//           2r, 2r+1 share a 16-lane DPP row) and the gfx950 permlane16 / permlane32 swaps;
+//   rho   : per-lane rotate (v_alignbit with a per-lane shift);
+//   pi    : one ds_bpermute (the only LDS-latency step of a round);
+//   chi   : DPP row shifts by 1 and 2 inside the group;
+//   iota  : lane 1 (x = 0, y = 0).
+struct KeccakWaveLane {
+    uint32_t valid;   // all ones for lanes 0..39, zero above
+    uint32_t src4;    // pi: byte address of the source lane
+    uint32_t rsh;     // rho: alignbit shift (32 - r % 32) & 31
+    bool rswap;       // rho: pre-swap halves ((r >= 32) xor (r % 32 == 0))
+    bool copy67;      // lanes l % 8 in {6, 7}: refreshed from l - 5 after chi
+    bool is00;        // the lane of word (0, 0)
+    int word;         // absorb: rate word x + 5y (< 17) or -1
+};
+
+__device__ __forceinline__ KeccakWaveLane keccak_wave_lane()
+{
+    // rho offsets r[x + 5y]
+    const uint8_t kRho[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39,
+                              41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
+    KeccakWaveLane k;
+    const uint32_t lane = __lane_id();
+    const uint32_t y = lane >> 3, j = lane & 7, x = (j + 4) % 5;
+    const bool ok = lane < 40;
+    k.valid = ok ? 0xffffffffu : 0u;
+    // pi: destination (x', y') reads source (3 (y' - 3 x') mod 5, x')
+    const uint32_t xs = (3u * ((y + 15u - 3u * x) % 5u)) % 5u, ys = x;
+    k.src4 = ok ? 4u * (8u * ys + xs + 1u) : 4u * lane;
+    const uint32_t r = ok ? kRho[x + 5 * y] : 0u;
+    k.rsh = (32u - (r & 31u)) & 31u;
+    k.rswap = (r >= 32u) != ((r & 31u) == 0u);
+    k.copy67 = j >= 6;
+    k.is00 = lane == 1;
+    k.word = (ok && x + 5 * y < 17) ? (int)(x + 5 * y) : -1;
+    return k;
+}
+
+__device__ __forceinline__ uint32_t dpp_from_next(uint32_t v) { return __builtin_amdgcn_update_dpp(0u, v, 0x101, 0xf, 0xf, true); }
+__device__ __forceinline__ uint32_t dpp_from_next2(uint32_t v) { return __builtin_amdgcn_update_dpp(0u, v, 0x102, 0xf, 0xf, true); }
+__device__ __forceinline__ uint32_t dpp_from_prev(uint32_t v) { return __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xf, 0xf, true); }
+__device__ __forceinline__ uint32_t dpp_from_prev5(uint32_t v) { return __builtin_amdgcn_update_dpp(0u, v, 0x115, 0xf, 0xf, true); }
+__device__ __forceinline__ uint32_t dpp_ror8(uint32_t v) { return __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xf, 0xf, true); }
+
+// XOR of the 4 rows of 16 lanes (every lane gets the XOR over its lane % 16 position).
+__device__ __forceinline__ uint32_t xor_rows(uint32_t v)
+{
+    auto a = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    const uint32_t p = a[0] ^ a[1];
+    auto b = __builtin_amdgcn_permlane32_swap(p, p, false, false);
+    return b[0] ^ b[1];
+}
+
+__device__ __forceinline__ void keccak_f1600_wave(uint32_t &lo, uint32_t &hi, const KeccakWaveLane &k)
+{
+#pragma unroll 1
+    for (int round = 0; round < 24; round++) {
+        // theta
+        const uint32_t clo = xor_rows(lo ^ dpp_ror8(lo)), chi = xor_rows(hi ^ dpp_ror8(hi));
+        const uint32_t nlo = dpp_from_next(clo), nhi = dpp_from_next(chi);
+        lo ^= dpp_from_prev(clo) ^ ((nlo << 1) | (nhi >> 31));
+        hi ^= dpp_from_prev(chi) ^ ((nhi << 1) | (nlo >> 31));
+        // rho
+        const uint32_t sl = k.rswap ? hi : lo, sh = k.rswap ? lo : hi;
+        const uint32_t rhi = __builtin_amdgcn_alignbit(sh, sl, k.rsh);
+        const uint32_t rlo = __builtin_amdgcn_alignbit(sl, sh, k.rsh);
+        // pi
+        lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)k.src4, (int)rlo) & k.valid;
+        hi = (uint32_t)__builtin_amdgcn_ds_bpermute((int)k.src4, (int)rhi) & k.valid;
+        // chi
+        const uint32_t b1lo = dpp_from_next(lo), b1hi = dpp_from_next(hi);
+        const uint32_t b2lo = dpp_from_next2(lo), b2hi = dpp_from_next2(hi);
+        lo ^= ~b1lo & b2lo;
+        hi ^= ~b1hi & b2hi;
+        // iota
+        if (k.is00) {
+            lo ^= (uint32_t)kKeccakRC[round];
+            hi ^= (uint32_t)(kKeccakRC[round] >> 32);
+        }
+        // the copies of columns 0 and 1 in slots 6 and 7 come from slots 1 and 2
+        const uint32_t clo6 = dpp_from_prev5(lo), chi6 = dpp_from_prev5(hi);
+        if (k.copy67) {
+            lo = clo6;
+            hi = chi6;
+        }
+    }
+}
+
+// Keccak-256 of nwords little-endian words at msg, computed by a whole wavefront (every
+// lane must be active); every lane receives the digest.
+__device__ inline void keccak256_wave(const uint32_t *msg, uint32_t nwords, uint32_t out[8])
+{
+    const KeccakWaveLane k = keccak_wave_lane();
+    uint32_t lo = 0, hi = 0;
+    const uint32_t nbytes = nwords * 4u;
+    const uint32_t nblocks = nbytes / 136 + 1;
+    for (uint32_t b = 0; b < nblocks; b++) {
+        if (k.word >= 0) {
+            const uint32_t idx = b * 34 + 2 * (uint32_t)k.word;
+            uint32_t wl = idx < nwords ? msg[idx] : 0u;
+            uint32_t wh = idx + 1 < nwords ? msg[idx + 1] : 0u;
+            if (b == nblocks - 1) {
+                if (idx == nwords) wl ^= 0x01u;
+                if (idx + 1 == nwords) wh ^= 0x01u;
+                if (k.word == 16) wh ^= 0x80000000u;
+            }
+            lo ^= wl;
+            hi ^= wh;
+        }
+        keccak_f1600_wave(lo, hi, k);
+    }
+    // digest = words (0,0) .. (3,0) = lanes 1 .. 4
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        out[2 * w] = __builtin_amdgcn_readlane(lo, w + 1);
+        out[2 * w + 1] = __builtin_amdgcn_readlane(hi, w + 1);
+    }
+}
+
+// SHA-256 of nwords words by a whole wavefront.  The compression is a serial chain, so
+// every lane runs it on the same data; the pointer is moved to vector registers first,
+// which keeps the arithmetic on the VALU (faster here than the scalar unit, where the
+// compiler would otherwise place uniform work).
+__device__ inline void sha256_wave(const uint32_t *msg, uint32_t nwords, uint32_t out[8])
+{
+    uint64_t p = reinterpret_cast<uint64_t>(msg);
+    asm volatile("" : "+v"(p));
+    sha256_words(reinterpret_cast<const uint32_t *>(p), nwords, out);
+}
+
+// Transcript hash computed by a whole wavefront; every lane receives the digest.
+__device__ __forceinline__ void hash_words_wave(int kind, const uint32_t *msg, uint32_t nwords, uint32_t out[8])
+{
+    if (kind == kHashKeccak256) keccak256_wave(msg, nwords, out);
+    else sha256_wave(msg, nwords, out);
+}
+
 }  // namespace gkrx

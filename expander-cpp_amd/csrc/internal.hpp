@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "gkrx.h"
+#include "profiler.hpp"
 
 namespace gkrx {
 
@@ -48,6 +49,41 @@ struct DevCircuit {
     uint32_t *values = nullptr;      // all layer values, D + 1 tables of L lanes
     uint64_t values_u32 = 0;
     uint32_t max_n = 0, max_m = 0;
+    LayerDev *d_layers = nullptr;    // device copy of `layers` (read by the batch kernel)
+};
+
+// One proof of a batch: its circuit, the host transcript prefix, and where the proof
+// bytes after the commitment go.
+struct BatchItem {
+    const DevCircuit *dc;
+    const uint32_t *rz1;     // R * m_last output challenges
+    const uint32_t *tstate;  // 10 words (length, hash start, digest)
+    const uint8_t *grind;    // 32 grind bytes (the first bytes of the tail)
+    uint64_t tail_bytes;
+    uint8_t *tail_out;       // host destination of the tail
+};
+
+// Device scratch of the batched prover, grown to the largest batch seen.
+struct BatchScratch {
+    uint32_t *pool = nullptr;
+    uint64_t pool_u32 = 0;
+    uint8_t *meta = nullptr;        // job table + order
+    uint64_t meta_bytes = 0;
+    uint8_t *host_meta = nullptr;   // pinned staging of the job table and seeds
+    uint32_t *claims_host = nullptr;
+    uint64_t claims_cap = 0;
+    // pinned host buffer for n claim words (nullptr when allocation fails)
+    uint32_t *host_claims(uint64_t n)
+    {
+        if (n > claims_cap) {
+            if (claims_host) (void)hipHostFree(claims_host);
+            claims_host = nullptr;
+            claims_cap = 0;
+            if (hipHostMalloc(&claims_host, n * 4, hipHostMallocDefault) != hipSuccess) return nullptr;
+            claims_cap = n;
+        }
+        return claims_host;
+    }
 };
 
 // Device scratch of one prover context, grown to the largest circuit seen.
@@ -70,9 +106,13 @@ struct DevScratch {
     uint32_t *claims = nullptr;   // R * L
     uint32_t *vxy = nullptr;      // R * L (vx) + R * L (vy)
     uint32_t *tstate = nullptr;   // transcript state: len, hash_start, digest[8]
+    uint32_t *ticket = nullptr;   // last-block counter of the multi-block reductions (zero at rest)
     uint32_t *tail = nullptr;     // transcript bytes after the commitment
     uint64_t tail_cap = 0;        // bytes
     uint32_t max_n = 0, max_m = 0, L = 0, D = 0;
+    // pinned host staging for the small uploads of a prove (outlives the async copies)
+    uint32_t *stage = nullptr;
+    uint64_t stage_u32 = 0;
 };
 
 struct ProveTimings {
@@ -85,11 +125,20 @@ int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &
 void dev_free_circuit(DevCircuit &dc);
 int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t tail_bytes);
 void dev_free_scratch(DevScratch &s);
-int dev_forward(const DevCircuit &dc, hipStream_t st);
+int dev_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf);
 // Runs the on-device part of the prove: output claims, every layer sumcheck and the
 // transcript from the state in `tstate_host` (len, hash_start, digest) and the tail
 // prefix already in s.tail.  rz1 (R * m_last) seeds the last layer.
 int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, const uint32_t *rz1_host,
-              const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st);
+              const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,
+              Profiler *pf);
+// Proves every item in one launch (one workgroup per proof).  Enqueues the copies of the
+// tails to items[i].tail_out and of the claims (item-major, R * L each) to
+// bs.host_claims(); the caller synchronizes the stream.
+int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_kind, BatchScratch &bs, hipStream_t st,
+                    Profiler *pf);
+// Whether a circuit fits the batch kernel (every layer at most 2^20 wide).
+int dev_batch_supported(const DevCircuit &dc);
+void dev_free_batch(BatchScratch &bs);
 
 }  // namespace gkrx

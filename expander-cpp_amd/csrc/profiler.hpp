@@ -1,0 +1,97 @@
+// Optional per-kernel-class timing with HIP events recorded on the launch stream.
+// Enabled per context (gkrx_ctx_profile); when off, begin/end are no-ops.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace gkrx {
+
+enum KernelClass : int {
+    kKForward = 0,
+    kKEq,
+    kKGx,
+    kKHy,
+    kKDot,
+    kKRoundPh1First,
+    kKRoundPh1,
+    kKRoundPh2First,
+    kKRoundPh2,
+    kKTailPh1,
+    kKTailPh2,
+    kKBatch,
+    kKClassCount
+};
+
+inline const char *kernel_class_name(int c)
+{
+    static const char *names[kKClassCount] = {"k_forward",      "k_eq",      "k_gx",           "k_hy",
+                                              "k_dot",          "k_round_ph1_first", "k_round_ph1",
+                                              "k_round_ph2_first", "k_round_ph2", "k_tail_ph1",
+                                              "k_tail_ph2",     "k_prove_batch"};
+    return (c >= 0 && c < kKClassCount) ? names[c] : "";
+}
+
+struct Profiler {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<int> cls;  // class of event pair i (events 2i, 2i+1)
+    uint64_t count[kKClassCount] = {0};
+    double ms[kKClassCount] = {0};
+
+    hipEvent_t next()
+    {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int c, hipStream_t st)
+    {
+        if (!on) return;
+        hipEvent_t e = next();
+        if (e) {
+            (void)hipEventRecord(e, st);
+            cls.push_back(c);
+        }
+    }
+    void end(hipStream_t st)
+    {
+        if (!on) return;
+        hipEvent_t e = next();
+        if (e) (void)hipEventRecord(e, st);
+    }
+    // after the stream has been synchronized
+    void collect()
+    {
+        for (size_t i = 0; i < cls.size() && 2 * i + 1 < used; i++) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, pool[2 * i], pool[2 * i + 1]) == hipSuccess) {
+                count[cls[i]]++;
+                ms[cls[i]] += t;
+            }
+        }
+        cls.clear();
+        used = 0;
+    }
+    void reset()
+    {
+        cls.clear();
+        used = 0;
+        for (int i = 0; i < kKClassCount; i++) {
+            count[i] = 0;
+            ms[i] = 0;
+        }
+    }
+    ~Profiler()
+    {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace gkrx

@@ -7,8 +7,9 @@
 //   k_gx       <- _prepare_g_x_vals scatters          sumcheck_helper.hpp:198-229
 //   k_hy       <- _prepare_h_y_vals scatter           sumcheck_helper.hpp:233-261
 //   k_round    <- receive_challenge (fold of round k-1) fused with poly_eval_at (round k)
-//                                                     sumcheck_helper.hpp:68-133
-//   k_finish   <- the transcript steps of sumcheck_prove_gkr_layer / gkr_prove
+//                                                     sumcheck_helper.hpp:68-133, on the whole GPU
+//   k_tail     <- the same for the small last rounds of a phase, in one workgroup
+//   transcript_step <- the transcript steps of sumcheck_prove_gkr_layer / gkr_prove
 //                                                     include/LinearGKR/sumcheck.hpp:72-103,
 //                                                     include/LinearGKR/gkr.hpp:58-67
 //   k_dot      <- eval_multilinear for claimed_v      include/poly_commit/poly.hpp:88-104
@@ -35,7 +36,6 @@ namespace gkrx {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kFinishThreads = 1024;
 constexpr uint32_t kMaxBlocksPerRep = 256;
 constexpr uint32_t kChalStride = 64;  // per rep: rx in [0, 32), ry in [32, 64)
 
@@ -137,64 +137,73 @@ __global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, uin
 }
 
 // ------------------------------------------------------------------ g_x and h_y
+// hg_j[x] = sum over the gates with first input x of v[y] * (c * eqZ_j[z]) (mul) or
+// c * eqZ_j[z] (add, broadcast to every lane), for every rep j at once: each wiring entry
+// and each gathered v[y] row is read once for all R repetitions.
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint64_t hg_rs,
                                                  const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz,
                                                  uint64_t eqz_rs, const uint32_t *__restrict__ ptr,
-                                                 const uint32_t *__restrict__ ent, uint32_t N)
+                                                 const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
 {
     constexpr uint32_t LG = L / 4;
-    const uint32_t rep = blockIdx.y;
-    const uint32_t *ez = eqz + rep * eqz_rs;
-    uint32_t *h = hg + rep * hg_rs;
     const uint64_t total = (uint64_t)N * LG;
     for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
         const uint32_t x = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
-        uint4 acc = make_uint4(0, 0, 0, 0);
+        uint4 acc[3] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
         const uint32_t e1 = ptr[x + 1];
         for (uint32_t e = ptr[x]; e < e1; e++) {
             const uint32_t b = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-            const uint32_t ce = m31_mul(c, ez[z]);
+            uint32_t ce[3];
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++) ce[j] = j < R ? m31_mul(c, eqz[j * eqz_rs + z]) : 0u;
             if (b != kAddMark) {
-                acc = add4(acc, mul4s(ld4(v + (uint64_t)b * L + 4 * g), ce));
+                const uint4 vb = ld4(v + (uint64_t)b * L + 4 * g);
+#pragma unroll
+                for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], mul4s(vb, ce[j]));
             } else {
-                acc = add4(acc, make_uint4(ce, ce, ce, ce));
+#pragma unroll
+                for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], make_uint4(ce[j], ce[j], ce[j], ce[j]));
             }
         }
-        st4(h + (uint64_t)x * L + 4 * g, acc);
+#pragma unroll
+        for (uint32_t j = 0; j < 3; j++)
+            if (j < R) st4(hg + j * hg_rs + (uint64_t)x * L + 4 * g, acc[j]);
     }
 }
 
+// s_j[y] = sum over the mul gates with second input y of eqZ_j[z] * eqX_j[x] * c, for every
+// rep j at once (phase-2 hg_j = vx_j * s_j).
 __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint64_t s_rs,
                                                  const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
                                                  const uint32_t *__restrict__ eqx, uint64_t eqx_rs,
                                                  const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
-                                                 uint32_t N)
+                                                 uint32_t N, uint32_t R)
 {
-    const uint32_t rep = blockIdx.y;
-    const uint32_t *ez = eqz + rep * eqz_rs;
-    const uint32_t *ex = eqx + rep * eqx_rs;
-    uint32_t *out = s + rep * s_rs;
     for (uint64_t y = (uint64_t)blockIdx.x * kThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kThreads) {
-        uint32_t acc = 0;
+        uint32_t acc[3] = {0, 0, 0};
         const uint32_t e1 = ptr[y + 1];
         for (uint32_t e = ptr[y]; e < e1; e++) {
             const uint32_t x = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-            acc = m31_add(acc, m31_mul(m31_mul(ez[z], ex[x]), c));
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++)
+                if (j < R) acc[j] = m31_add(acc[j], m31_mul(m31_mul(eqz[j * eqz_rs + z], eqx[j * eqx_rs + x]), c));
         }
-        out[y] = acc;
+#pragma unroll
+        for (uint32_t j = 0; j < 3; j++)
+            if (j < R) s[j * s_rs + y] = acc[j];
     }
 }
 
 // ------------------------------------------------------------------ reductions
-// Reduce 3 x 4 lane sums (u64) across the block.  Threads sharing t % LG hold the same
-// lanes.  Result: partial[q] for q in [0, 3L) with q = pk * L + 4 * g + c.
+// Reduces the 3 x 4 lane sums (u64) of the 256 threads [t0, t0 + 256) that share
+// t % LG.  red is this group's LDS scratch [4][3L]; the result goes to out[q],
+// q = pk * L + 4 * g + c, written by threads t0 .. t0 + 3L - 1.  Every thread of the block
+// must call it (it synchronizes the block).
 template <int L>
-__device__ __forceinline__ void block_reduce_store(uint64_t (&acc)[3][4], uint32_t *__restrict__ dst)
+__device__ __forceinline__ void group_reduce(uint64_t (&acc)[3][4], uint32_t (*red)[3 * L], uint32_t *out)
 {
     constexpr uint32_t LG = L / 4;
-    constexpr int kWaves = kThreads / 64;
-    __shared__ uint32_t red[kWaves][3 * L];
     uint32_t v[3][4];
 #pragma unroll
     for (int pk = 0; pk < 3; pk++)
@@ -207,7 +216,7 @@ __device__ __forceinline__ void block_reduce_store(uint64_t (&acc)[3][4], uint32
 #pragma unroll
             for (int c = 0; c < 4; c++) v[pk][c] = m31_add(v[pk][c], __shfl_xor(v[pk][c], off, 64));
     }
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
     if (lane < LG) {
 #pragma unroll
         for (int pk = 0; pk < 3; pk++)
@@ -215,15 +224,222 @@ __device__ __forceinline__ void block_reduce_store(uint64_t (&acc)[3][4], uint32
             for (int c = 0; c < 4; c++) red[wave][pk * L + 4 * lane + c] = v[pk][c];
     }
     __syncthreads();
-    if (threadIdx.x < 3 * L) {
+    const uint32_t tg = threadIdx.x & 255;
+    if (tg < 3 * L) {
         uint32_t s = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; w++) s = m31_add(s, red[w][threadIdx.x]);
-        dst[threadIdx.x] = s;
+        for (int w = 0; w < 4; w++) s = m31_add(s, red[w][tg]);
+        out[tg] = s;
     }
 }
 
-// ------------------------------------------------------------------ sumcheck round
+// ------------------------------------------------------------------ sumcheck rounds
+// One item of round k: the pair (2i, 2i+1) of the round-k tables for lanes 4g..4g+3.
+// FUSED: the round-k tables are produced here by folding rows 4i..4i+3 of the round-(k-1)
+// tables with challenge r (receive_challenge, sumcheck_helper.hpp:104-133) and stored.
+// Accumulates p0 += f0 h0, p1 += f1 h1, p2 += (f0 + f1)(h0 + h1) (poly_eval_at, :68-102).
+// PH2: h is the phase-2 scalar table s; hg = vx * s is rank one and vx multiplies later.
+template <int L, bool PH2, bool FUSED>
+__device__ __forceinline__ void round_item(const uint32_t *__restrict__ fs, uint32_t *__restrict__ fd,
+                                           const uint32_t *__restrict__ hs, uint32_t *__restrict__ hd, uint64_t i,
+                                           uint32_t g, uint32_t r, uint64_t (&acc)[3][4])
+{
+    uint4 f0, f1, h0, h1;
+    uint32_t s0 = 0, s1 = 0;
+    if (FUSED) {
+        const uint32_t *src = fs + 4 * i * L + 4 * g;
+        const uint4 a0 = ld4(src), a1 = ld4(src + L), a2 = ld4(src + 2 * L), a3 = ld4(src + 3 * L);
+        if (PH2) {
+            const uint4 b = *reinterpret_cast<const uint4 *>(hs + 4 * i);
+            s0 = fold1(b.x, b.y, r);
+            s1 = fold1(b.z, b.w, r);
+            if (g == 0) *reinterpret_cast<uint2 *>(hd + 2 * i) = make_uint2(s0, s1);
+        } else {
+            const uint32_t *hsrc = hs + 4 * i * L + 4 * g;
+            const uint4 c0 = ld4(hsrc), c1 = ld4(hsrc + L), c2 = ld4(hsrc + 2 * L), c3 = ld4(hsrc + 3 * L);
+            h0 = fold4(c0, c1, r);
+            h1 = fold4(c2, c3, r);
+            st4(hd + 2 * i * L + 4 * g, h0);
+            st4(hd + (2 * i + 1) * L + 4 * g, h1);
+        }
+        f0 = fold4(a0, a1, r);
+        f1 = fold4(a2, a3, r);
+        st4(fd + 2 * i * L + 4 * g, f0);
+        st4(fd + (2 * i + 1) * L + 4 * g, f1);
+    } else {
+        f0 = ld4(fs + 2 * i * L + 4 * g);
+        f1 = ld4(fs + (2 * i + 1) * L + 4 * g);
+        if (PH2) {
+            const uint2 b = *reinterpret_cast<const uint2 *>(hs + 2 * i);
+            s0 = b.x;
+            s1 = b.y;
+        } else {
+            h0 = ld4(hs + 2 * i * L + 4 * g);
+            h1 = ld4(hs + (2 * i + 1) * L + 4 * g);
+        }
+    }
+    if (PH2) {
+        h0 = make_uint4(s0, s0, s0, s0);
+        h1 = make_uint4(s1, s1, s1, s1);
+    }
+    const uint4 e0 = mul4(f0, h0), e1 = mul4(f1, h1), e2 = mul4(add4(f0, f1), add4(h0, h1));
+    acc[0][0] += e0.x; acc[0][1] += e0.y; acc[0][2] += e0.z; acc[0][3] += e0.w;
+    acc[1][0] += e1.x; acc[1][1] += e1.y; acc[1][2] += e1.z; acc[1][3] += e1.w;
+    acc[2][0] += e2.x; acc[2][1] += e2.y; acc[2][2] += e2.z; acc[2][3] += e2.w;
+}
+
+// ------------------------------------------------------------------ transcript
+// The transcript (include/fiat_shamir/transcript.hpp:87-175) continues on the device over
+// the proof bytes after the commitment.  tstate: [0] length in bytes, [1] start of the
+// unhashed suffix, [2..9] digest.  A challenge hashes only the bytes appended since the
+// previous challenge (or the previous digest when there are none), so within a round the
+// R repetitions' challenges are independent: each hashes its own 3 field elements.  The
+// exceptions are the last x round, where rep j's message begins with vx_{j-1}
+// (sumcheck.hpp:90-95), and the layer end (vy claims, alpha, beta: sumcheck.hpp:98-103,
+// gkr.hpp:65-66); those run in order on one lane.
+struct TsArgs {
+    uint32_t R;
+    int phase;            // 1 = x variables, 2 = y variables
+    uint32_t k;           // round index within the phase
+    int last;             // k == n - 1
+    uint32_t *chal;       // this layer's challenge slots, rep stride kChalStride
+    uint32_t *vx, *vy;    // R * L each
+    const uint32_t *fin;  // round-k f table (rows 0 and 1 are used after the last round)
+    uint64_t fin_rs;
+    uint32_t *tstate;
+    uint32_t *tail;
+    uint32_t *ab_out;     // alpha, beta drawn after the layer
+    int hash_kind;
+};
+
+__device__ __forceinline__ uint32_t challenge_from(const uint32_t d[8]) { return m31_from_u32_full(d[0]); }
+
+// Message word q (of 3L) of rep j from the round sums: p0, p1 and p2 = 6 p1 + 3 p0 - 2 p2
+// (sumcheck_helper.hpp:100); in phase 2 each sum is first multiplied by vx.
+template <int L>
+__device__ __forceinline__ uint32_t msg_word(const uint32_t *sj, const uint32_t *vxj, int phase, uint32_t q)
+{
+    const uint32_t pk = q / L, c = q % L;
+    uint32_t q0 = sj[c], q1 = sj[L + c], q2 = sj[2 * L + c];
+    if (phase == 2) {
+        const uint32_t v = vxj[c];
+        q0 = m31_mul(q0, v);
+        q1 = m31_mul(q1, v);
+        q2 = m31_mul(q2, v);
+    }
+    if (pk == 0) return q0;
+    if (pk == 1) return q1;
+    return m31_sub(m31_add(m31_mul(q1, 6u), m31_mul(q0, 3u)), m31_mul(q2, 2u));
+}
+
+// Block-wide transcript step for round k of one phase, all reps.  sums: LDS [R][3L].
+// r_out: LDS [R] receives the challenges.  Every hash is computed by a whole wavefront:
+// wave j hashes rep j when the reps are independent.  Every thread of the block must call
+// it; the block has at least 64 R threads.
+template <int L>
+__device__ void transcript_step(const uint32_t *sums, const TsArgs &a, uint32_t *r_out)
+{
+    constexpr uint32_t Q = 3 * L;
+    __shared__ uint32_t ts[10];
+    __shared__ uint32_t dig[8];
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    if (t < 10) ts[t] = a.tstate[t];
+    __syncthreads();
+    if (!(a.phase == 1 && a.last)) {
+        const uint32_t base = ts[0], hs0 = ts[1];
+        for (uint32_t w = t; w < a.R * Q; w += blockDim.x)
+            a.tail[base / 4 + w] = msg_word<L>(sums + (w / Q) * Q, a.vx + (w / Q) * L, a.phase, w % Q);
+        __syncthreads();
+        if (wave < a.R) {
+            const uint32_t j = wave;
+            const uint32_t start = j == 0 ? hs0 : base + j * 4 * Q;
+            const uint32_t end = base + (j + 1) * 4 * Q;
+            uint32_t d[8];
+            hash_words_wave(a.hash_kind, a.tail + start / 4, (end - start) / 4, d);
+            if (lane == 0) {
+                const uint32_t r = challenge_from(d);
+                r_out[j] = r;
+                a.chal[j * kChalStride + (a.phase == 2 ? 32u : 0u) + a.k] = r;
+                if (j == a.R - 1) {
+                    for (int i = 0; i < 8; i++) ts[2 + i] = d[i];
+                    ts[0] = end;
+                    ts[1] = end;
+                }
+            }
+        }
+        __syncthreads();
+        if (a.last) {
+            // phase 2 ends the layer: vy_claim = f after the last fold for every rep, then all
+            // vy claims are appended and alpha, beta drawn (sumcheck.hpp:98-103, gkr.hpp:65-66)
+            const uint32_t len = ts[0];
+            if (t < a.R * L) {
+                const uint32_t j = t / L, c = t % L;
+                const uint32_t *f = a.fin + j * a.fin_rs;
+                const uint32_t v = fold1(f[c], f[L + c], r_out[j]);
+                a.vy[t] = v;
+                a.tail[len / 4 + t] = v;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                uint32_t d[8];
+                hash_words_wave(a.hash_kind, a.tail + len / 4, a.R * L, d);
+                if (lane == 0) {
+                    a.ab_out[0] = challenge_from(d);
+                    for (int i = 0; i < 8; i++) dig[i] = d[i];
+                }
+            }
+            __syncthreads();
+            if (wave == 0) {
+                uint32_t d[8];
+                hash_words_wave(a.hash_kind, dig, 8, d);
+                if (lane == 0) {
+                    a.ab_out[1] = challenge_from(d);
+                    for (int i = 0; i < 8; i++) ts[2 + i] = d[i];
+                    ts[0] = len + 4 * a.R * L;
+                    ts[1] = ts[0];
+                }
+            }
+        }
+    } else {
+        // last x round, in order: rep j hashes (vx_{j-1} ||) its message, then appends vx_j
+        // (sumcheck.hpp:90-95)
+        uint32_t len = ts[0], hs = ts[1];
+        for (uint32_t j = 0; j < a.R; j++) {
+            for (uint32_t q = t; q < Q; q += blockDim.x)
+                a.tail[len / 4 + q] = msg_word<L>(sums + j * Q, a.vx + j * L, 1, q);
+            __syncthreads();
+            if (wave == 0) {
+                uint32_t d[8];
+                hash_words_wave(a.hash_kind, a.tail + hs / 4, (len + 4 * Q - hs) / 4, d);
+                if (lane == 0) {
+                    const uint32_t r = challenge_from(d);
+                    r_out[j] = r;
+                    a.chal[j * kChalStride + a.k] = r;
+                    for (int i = 0; i < 8; i++) ts[2 + i] = d[i];
+                }
+            }
+            __syncthreads();
+            len += 4 * Q;
+            hs = len;
+            if (t < L) {
+                const uint32_t *f = a.fin + j * a.fin_rs;
+                const uint32_t v = fold1(f[t], f[L + t], r_out[j]);
+                a.vx[j * L + t] = v;
+                a.tail[len / 4 + t] = v;
+            }
+            __syncthreads();
+            len += 4 * L;
+        }
+        if (t == 0) {
+            ts[0] = len;
+            ts[1] = hs;
+        }
+    }
+    __syncthreads();
+    if (t < 10) a.tstate[t] = ts[t];
+    __syncthreads();
+}
+
 struct RoundParams {
     const uint32_t *fsrc;  // f_{k-1} (or f_0 when k == 0)
     uint64_t fsrc_rs;      // rep stride in u32 (0 when the source is the shared layer input)
@@ -235,18 +451,22 @@ struct RoundParams {
     uint64_t hdst_rs;
     const uint32_t *rprev; // challenge of round k-1 for rep 0; rep stride kChalStride
     uint32_t *partials;    // [blockIdx.x][rep][3L]
+    uint32_t *ticket;      // zero between launches
     uint32_t items;        // pairs in round k = N_k / 2
-    uint32_t R;
+    TsArgs ts;
 };
 
-// Round k of one sumcheck phase for every repetition (blockIdx.y = rep).
-// FUSED: fold round k-1 with its challenge, store f_k / h_k, and accumulate
-//   p0 = sum f_k[2i] h_k[2i], p1 = sum f_k[2i+1] h_k[2i+1], p2 = sum (f0 + f1)(h0 + h1).
-// PH2: h is the scalar table s (phase-2 hg = vx * s is rank one; vx multiplies later).
+// Round k of one sumcheck phase for every repetition (blockIdx.y = rep), on many blocks.
+// The block that finishes last (ticket) adds the block partials and runs the transcript.
 template <int L, bool PH2, bool FUSED>
 __global__ void __launch_bounds__(kThreads) k_round(RoundParams p)
 {
-    constexpr uint32_t LG = L / 4;
+    constexpr uint32_t LG = L / 4, Q = 3 * L;
+    __shared__ uint32_t red[4][3 * L];
+    __shared__ unsigned long long lsum[3 * Q];
+    __shared__ uint32_t sums[3 * Q], rr[3];
+    __shared__ bool is_last;
+    const uint32_t R = p.ts.R;
     const uint32_t rep = blockIdx.y;
     const uint32_t g = threadIdx.x % LG;
     const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
@@ -254,228 +474,374 @@ __global__ void __launch_bounds__(kThreads) k_round(RoundParams p)
     const uint32_t *hs = p.hsrc + rep * p.hsrc_rs;
     uint32_t *hd = p.hdst + rep * p.hdst_rs;
     const uint32_t r = FUSED ? p.rprev[(uint64_t)rep * kChalStride] : 0u;
-    uint64_t acc[3][4];
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[a][c] = 0;
+    uint64_t acc[3][4] = {};
     const uint64_t total = (uint64_t)p.items * LG;
-    for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
-        const uint64_t i = w / LG;
-        uint4 f0, f1, h0, h1;
-        uint32_t s0 = 0, s1 = 0;
-        if (FUSED) {
-            const uint32_t *src = fs + 4 * i * L + 4 * g;
-            const uint4 a0 = ld4(src), a1 = ld4(src + L), a2 = ld4(src + 2 * L), a3 = ld4(src + 3 * L);
-            f0 = fold4(a0, a1, r);
-            f1 = fold4(a2, a3, r);
-            st4(fd + 2 * i * L + 4 * g, f0);
-            st4(fd + (2 * i + 1) * L + 4 * g, f1);
-            if (PH2) {
-                const uint32_t b0 = hs[4 * i], b1 = hs[4 * i + 1], b2 = hs[4 * i + 2], b3 = hs[4 * i + 3];
-                s0 = fold1(b0, b1, r);
-                s1 = fold1(b2, b3, r);
-                if (g == 0) {
-                    hd[2 * i] = s0;
-                    hd[2 * i + 1] = s1;
-                }
-            } else {
-                const uint32_t *hsrc = hs + 4 * i * L + 4 * g;
-                const uint4 c0 = ld4(hsrc), c1 = ld4(hsrc + L), c2 = ld4(hsrc + 2 * L), c3 = ld4(hsrc + 3 * L);
-                h0 = fold4(c0, c1, r);
-                h1 = fold4(c2, c3, r);
-                st4(hd + 2 * i * L + 4 * g, h0);
-                st4(hd + (2 * i + 1) * L + 4 * g, h1);
-            }
-        } else {
-            f0 = ld4(fs + 2 * i * L + 4 * g);
-            f1 = ld4(fs + (2 * i + 1) * L + 4 * g);
-            if (PH2) {
-                s0 = hs[2 * i];
-                s1 = hs[2 * i + 1];
-            } else {
-                h0 = ld4(hs + 2 * i * L + 4 * g);
-                h1 = ld4(hs + (2 * i + 1) * L + 4 * g);
-            }
-        }
-        if (PH2) {
-            h0 = make_uint4(s0, s0, s0, s0);
-            h1 = make_uint4(s1, s1, s1, s1);
-        }
-        const uint4 e0 = mul4(f0, h0), e1 = mul4(f1, h1), e2 = mul4(add4(f0, f1), add4(h0, h1));
-        acc[0][0] += e0.x; acc[0][1] += e0.y; acc[0][2] += e0.z; acc[0][3] += e0.w;
-        acc[1][0] += e1.x; acc[1][1] += e1.y; acc[1][2] += e1.z; acc[1][3] += e1.w;
-        acc[2][0] += e2.x; acc[2][1] += e2.y; acc[2][2] += e2.z; acc[2][3] += e2.w;
-    }
-    block_reduce_store<L>(acc, p.partials + ((uint64_t)blockIdx.x * p.R + rep) * 3 * L);
+    for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
+        round_item<L, PH2, FUSED>(fs, fd, hs, hd, w / LG, g, r, acc);
+    group_reduce<L>(acc, red, p.partials + ((uint64_t)blockIdx.x * R + rep) * Q);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(p.ticket, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    for (uint32_t i = threadIdx.x; i < R * Q; i += kThreads) lsum[i] = 0;
+    __syncthreads();
+    const uint32_t nvals = gridDim.x * R * Q;
+    for (uint32_t f = threadIdx.x; f < nvals; f += kThreads)
+        atomicAdd(&lsum[f % (R * Q)], (unsigned long long)p.partials[f]);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < R * Q; i += kThreads) sums[i] = m31_reduce64(lsum[i]);
+    __syncthreads();
+    transcript_step<L>(sums, p.ts, rr);
+    if (threadIdx.x == 0) *p.ticket = 0;
 }
 
-// Dot product sum_z eq[rep][z] * vals[z] (lanes), for claimed_v.
+// Rounds k0 .. n-1 of one phase in a single workgroup of 768 threads: the tables are
+// small (at most kTailRows rows), so the launch and cross-block reduction cost of a grid
+// would dominate.  Threads [256 j, 256 j + 256) work on rep j.
+constexpr uint32_t kTailThreads = 768;  // 3 groups of 256: one per repetition
+constexpr uint32_t kTailRows = 2048;
+struct TailParams {
+    const uint32_t *f0;       // round-k0 source of f (the layer input when k0 <= 1)
+    uint64_t f0_rs;
+    const uint32_t *h0;       // round-k0 source of h
+    uint64_t h0_rs;
+    uint32_t *fA, *fB, *hA, *hB;
+    uint64_t fA_rs, fB_rs, hA_rs, hB_rs;
+    uint32_t k0, n;
+    TsArgs ts;
+};
+
+// Rounds p.k0 .. p.n - 1 of one phase, run by one workgroup (blockDim a multiple of 256,
+// at least 256 R): group j = threads [256 j, 256 j + 256) works on rep j.
+template <int L, bool PH2>
+__device__ void phase_rounds(const TailParams &p)
+{
+    constexpr uint32_t LG = L / 4, Q = 3 * L;
+    __shared__ uint32_t red[4][4][3 * L];
+    __shared__ uint32_t sums[3 * Q], rr[3], idle[3 * L];
+    const uint32_t R = p.ts.R;
+    const uint32_t grp = threadIdx.x >> 8, tg = threadIdx.x & 255, g = tg % LG;
+    if (threadIdx.x < R) rr[threadIdx.x] = p.k0 > 0 ? p.ts.chal[threadIdx.x * kChalStride + (PH2 ? 32u : 0u) + p.k0 - 1] : 0u;
+    __syncthreads();
+    for (uint32_t k = p.k0; k < p.n; k++) {
+        const uint64_t items = 1ull << (p.n - k - 1);
+        // table k lives in A for odd k, B for even k >= 2; table k-1 is the source
+        const uint32_t *fs = (k == p.k0) ? p.f0 : ((k - 1) & 1 ? p.fA : p.fB);
+        const uint64_t fs_rs = (k == p.k0) ? p.f0_rs : ((k - 1) & 1 ? p.fA_rs : p.fB_rs);
+        const uint32_t *hs = (k == p.k0) ? p.h0 : ((k - 1) & 1 ? p.hA : p.hB);
+        const uint64_t hs_rs = (k == p.k0) ? p.h0_rs : ((k - 1) & 1 ? p.hA_rs : p.hB_rs);
+        uint32_t *fd = (k & 1) ? p.fA : p.fB;
+        const uint64_t fd_rs = (k & 1) ? p.fA_rs : p.fB_rs;
+        uint32_t *hd = (k & 1) ? p.hA : p.hB;
+        const uint64_t hd_rs = (k & 1) ? p.hA_rs : p.hB_rs;
+        uint64_t acc[3][4] = {};
+        if (grp < R) {
+            const uint32_t rep = grp;
+            const uint32_t *f = fs + rep * fs_rs;
+            const uint32_t *h = hs + rep * hs_rs;
+            for (uint64_t w = tg; w < items * LG; w += 256) {
+                if (k == 0) round_item<L, PH2, false>(f, nullptr, h, nullptr, w / LG, g, 0u, acc);
+                else round_item<L, PH2, true>(f, fd + rep * fd_rs, h, hd + rep * hd_rs, w / LG, g, rr[rep], acc);
+            }
+        }
+        group_reduce<L>(acc, red[grp], grp < R ? sums + grp * Q : idle);
+        __syncthreads();
+        TsArgs a = p.ts;
+        a.k = k;
+        a.last = (k == p.n - 1);
+        a.fin = (k == 0) ? fs : fd;
+        a.fin_rs = (k == 0) ? fs_rs : fd_rs;
+        transcript_step<L>(sums, a, rr);
+    }
+}
+
+template <int L, bool PH2>
+__global__ void __launch_bounds__(kTailThreads) k_tail(TailParams p)
+{
+    phase_rounds<L, PH2>(p);
+}
+
+// Dot product sum_z eq[rep][z] * vals[z] (lanes): claimed_v = eval_multilinear(output, rz1)
+// (include/LinearGKR/gkr.hpp:51-55).  The last block adds the partials.
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_dot(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ eq,
                                                   uint64_t eq_rs, uint32_t M, uint32_t *__restrict__ partials,
-                                                  uint32_t R)
+                                                  uint32_t *ticket, uint32_t R, uint32_t *claims)
 {
-    constexpr uint32_t LG = L / 4;
+    constexpr uint32_t LG = L / 4, Q = 3 * L;
+    __shared__ uint32_t red[4][3 * L];
+    __shared__ unsigned long long lsum[3 * Q];
+    __shared__ bool is_last;
     const uint32_t rep = blockIdx.y;
     const uint32_t g = threadIdx.x % LG;
     const uint32_t *e = eq + rep * eq_rs;
-    uint64_t acc[3][4];
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[a][c] = 0;
+    uint64_t acc[3][4] = {};
     const uint64_t total = (uint64_t)M * LG;
     for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
         const uint64_t z = w / LG;
         const uint4 t = mul4s(ld4(vals + z * L + 4 * g), e[z]);
         acc[0][0] += t.x; acc[0][1] += t.y; acc[0][2] += t.z; acc[0][3] += t.w;
     }
-    block_reduce_store<L>(acc, partials + ((uint64_t)blockIdx.x * R + rep) * 3 * L);
+    group_reduce<L>(acc, red, partials + ((uint64_t)blockIdx.x * R + rep) * Q);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    for (uint32_t i = threadIdx.x; i < R * Q; i += kThreads) lsum[i] = 0;
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < gridDim.x * R * Q; f += kThreads)
+        atomicAdd(&lsum[f % (R * Q)], (unsigned long long)partials[f]);
+    __syncthreads();
+    if (threadIdx.x < R * L) {
+        const uint32_t j = threadIdx.x / L, c = threadIdx.x % L;
+        claims[threadIdx.x] = m31_reduce64(lsum[j * Q + c]);
+    }
+    if (threadIdx.x == 0) *ticket = 0;
 }
 
-// ------------------------------------------------------------------ transcript
-// Device transcript (include/fiat_shamir/transcript.hpp:87-175) over the proof bytes
-// that follow the commitment.  state: [0] = len (bytes), [1] = hash_start, [2..9] digest.
-struct DevTranscript {
-    uint32_t len, hash_start;
-    uint32_t digest[8];
-    uint32_t *tail;
-    int kind;
+// ------------------------------------------------------------------ batched proving
+// A batch of independent proofs runs as ONE launch: workgroup b proves job order[b]
+// from the forward pass to the last transcript byte, with no synchronization between
+// workgroups (each proof's transcript is its own serial chain, so proofs only share the
+// GPU).  Jobs are ordered largest first so the long proofs start early.  Every step reuses
+// the block-level building blocks of the single-proof path.
+constexpr uint32_t kBatchThreads = 768;  // 3 groups of 256: one per repetition
+constexpr uint32_t kBatchMaxVars = 20;   // eq tables split 10 + 10 bits in LDS
 
-    __device__ void load(const uint32_t *st, uint32_t *t, int k)
-    {
-        len = st[0];
-        hash_start = st[1];
-        for (int i = 0; i < 8; i++) digest[i] = st[2 + i];
-        tail = t;
-        kind = k;
-    }
-    __device__ void store(uint32_t *st) const
-    {
-        st[0] = len;
-        st[1] = hash_start;
-        for (int i = 0; i < 8; i++) st[2 + i] = digest[i];
-    }
-    __device__ void append_words(const uint32_t *w, uint32_t n)
-    {
-        for (uint32_t i = 0; i < n; i++) tail[len / 4 + i] = w[i];
-        len += 4 * n;
-    }
-    // challenge_f: hash the bytes appended since the last challenge (or the digest),
-    // then M31::from_bytes of the first four digest bytes.
-    __device__ uint32_t challenge()
-    {
-        uint32_t d[8];
-        if (len > hash_start) {
-            hash_words(kind, tail + hash_start / 4, (len - hash_start) / 4, d);
-            hash_start = len;
-        } else {
-            hash_words(kind, digest, 8, d);
-        }
-        for (int i = 0; i < 8; i++) digest[i] = d[i];
-        return m31_from_u32_full(digest[0]);
-    }
-};
-
-struct FinishParams {
-    const uint32_t *partials;
-    uint32_t nbx, R;
-    int mode;              // 0 = sumcheck round, 1 = claims (sums only)
-    int phase;             // 1 or 2
-    int last;              // last round of the phase
-    int layer_end;         // last round of phase 2: append vy claims, draw alpha and beta
-    uint32_t k;            // round index in the phase
-    uint32_t *chal;        // this layer's challenge slots, rep stride kChalStride
-    uint32_t *vx;          // R * L
-    uint32_t *vy;          // R * L
-    const uint32_t *fin;   // f table of this round (2 rows per rep at the last round)
-    uint64_t fin_rs;
-    uint32_t *tstate;
-    uint32_t *tail;
-    uint32_t *ab_out;      // alpha, beta drawn after this layer
-    uint32_t *claims;      // mode 1 output, R * L
-    int hash_kind;
+struct BatchJob {
+    uint32_t D;
+    const LayerDev *layers;  // device array of D layers
+    uint32_t *values;
+    uint32_t *eqz, *eqx, *hg0, *hgA, *hgB, *fA, *fB, *s0, *sA, *sB;
+    uint64_t eqz_rs, eqx_rs, hg0_rs, hgA_rs, hgB_rs, fA_rs, fB_rs, s0_rs, sA_rs, sB_rs;
+    uint32_t *chal;          // (D + 1) * R * kChalStride
+    uint32_t *ab;            // 2 (D + 1)
+    uint32_t *vxy;           // 2 R L
+    uint32_t *tstate;        // 16
+    uint32_t *tail;          // proof bytes after the commitment
+    uint32_t *claims;        // R L
+    const uint32_t *seed;    // rz1 in slot layout [R * kChalStride], tstate [10], grind bytes [8]
 };
 
 template <int L>
-__global__ void __launch_bounds__(kFinishThreads) k_finish(FinishParams p)
+__device__ void block_forward(const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
+                              const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t M)
 {
-    constexpr uint32_t Q = 3 * L;
-    __shared__ uint32_t part[kFinishThreads];
-    __shared__ uint32_t sums[3 * Q];  // R <= 3
-    const uint32_t items = p.R * Q;
-    const uint32_t nparts = kFinishThreads / items;
-    const uint32_t t = threadIdx.x;
-    if (t < nparts * items) {
-        const uint32_t item = t % items, pt = t / items;
-        uint64_t s = 0;
-        for (uint32_t b = pt; b < p.nbx; b += nparts) s += p.partials[(uint64_t)b * items + item];
-        part[t] = m31_reduce64(s);
+    constexpr uint32_t LG = L / 4;
+    for (uint64_t w = threadIdx.x; w < (uint64_t)M * LG; w += blockDim.x) {
+        const uint32_t o = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
+        uint4 acc = make_uint4(0, 0, 0, 0);
+        const uint32_t e1 = ptr[o + 1];
+        for (uint32_t e = ptr[o]; e < e1; e++) {
+            const uint32_t a = ent[3 * (uint64_t)e], b = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+            uint4 va = ld4(vin + (uint64_t)a * L + 4 * g);
+            if (b != kAddMark) va = mul4(va, ld4(vin + (uint64_t)b * L + 4 * g));
+            acc = add4(acc, mul4s(va, c));
+        }
+        st4(vout + (uint64_t)o * L + 4 * g, acc);
     }
-    __syncthreads();
-    if (t < items) {
-        uint32_t s = 0;
-        for (uint32_t pt = 0; pt < nparts; pt++) s = m31_add(s, part[pt * items + t]);
-        sums[t] = s;
-    }
-    __syncthreads();
-    if (p.mode == 1) {
-        // claimed_v[rep] = first L sums of the rep
-        if (t < p.R * L) p.claims[t] = sums[(t / L) * Q + (t % L)];
-        return;
-    }
-    if (t != 0) return;
-    DevTranscript tr;
-    tr.load(p.tstate, p.tail, p.hash_kind);
-    uint32_t msg[3 * L];
-    for (uint32_t j = 0; j < p.R; j++) {
-        const uint32_t *sj = sums + j * Q;
-        for (uint32_t c = 0; c < L; c++) {
-            uint32_t q0 = sj[c], q1 = sj[L + c], q2 = sj[2 * L + c];
-            if (p.phase == 2) {
-                const uint32_t vx = p.vx[j * L + c];
-                q0 = m31_mul(q0, vx);
-                q1 = m31_mul(q1, vx);
-                q2 = m31_mul(q2, vx);
+}
+
+// out[j][z] = f1 eq(r1_j, z) + f2 eq(r2_j, z) for z < 2^m (m <= 20), rep by rep, from
+// 2^10-entry low / high tables in LDS.  ab == nullptr: f1 = 1 and no second term.
+__device__ void block_eq(uint32_t *__restrict__ out, uint64_t out_rs, const uint32_t *__restrict__ chal,
+                         uint32_t r1_off, uint32_t r2_off, const uint32_t *__restrict__ ab, uint32_t m, uint32_t R)
+{
+    __shared__ uint32_t lo1[1024], lo2[1024], hi1[1024], hi2[1024];
+    const bool two = ab != nullptr;
+    const uint32_t h = m < 10 ? m : 10, hb = m - h;
+    for (uint32_t j = 0; j < R; j++) {
+        const uint32_t *r1 = chal + j * kChalStride + r1_off, *r2 = chal + j * kChalStride + r2_off;
+        const uint32_t f1 = two ? ab[0] : 1u, f2 = two ? ab[1] : 0u;
+        for (uint32_t t = threadIdx.x; t < (1u << h); t += blockDim.x) {
+            uint32_t a = f1, b = f2;
+            for (uint32_t i = 0; i < h; i++) {
+                const uint32_t bit = (t >> i) & 1u;
+                a = m31_mul(a, bit ? r1[i] : m31_sub(1u, r1[i]));
+                if (two) b = m31_mul(b, bit ? r2[i] : m31_sub(1u, r2[i]));
             }
-            // p2 = p1 * 6 + p0 * 3 - p2 * 2 (sumcheck_helper.hpp:100)
-            msg[c] = q0;
-            msg[L + c] = q1;
-            msg[2 * L + c] = m31_sub(m31_add(m31_mul(q1, 6u), m31_mul(q0, 3u)), m31_mul(q2, 2u));
+            lo1[t] = a;
+            lo2[t] = b;
         }
-        tr.append_words(msg, 3 * L);
-        const uint32_t r = tr.challenge();
-        p.chal[j * kChalStride + (p.phase == 2 ? 32u : 0u) + p.k] = r;
-        if (p.last) {
-            const uint32_t *f = p.fin + j * p.fin_rs;
-            uint32_t *dst = (p.phase == 1 ? p.vx : p.vy) + j * L;
-            for (uint32_t c = 0; c < L; c++) dst[c] = fold1(f[c], f[L + c], r);
-            if (p.phase == 1) tr.append_words(dst, L);  // vx_claim right after rep j's last x round
+        for (uint32_t t = threadIdx.x; t < (1u << hb); t += blockDim.x) {
+            uint32_t a = 1u, b = 1u;
+            for (uint32_t i = 0; i < hb; i++) {
+                const uint32_t bit = (t >> i) & 1u;
+                a = m31_mul(a, bit ? r1[h + i] : m31_sub(1u, r1[h + i]));
+                if (two) b = m31_mul(b, bit ? r2[h + i] : m31_sub(1u, r2[h + i]));
+            }
+            hi1[t] = a;
+            hi2[t] = b;
+        }
+        __syncthreads();
+        const uint32_t mask = (1u << h) - 1u;
+        uint32_t *o = out + j * out_rs;
+        for (uint32_t z = threadIdx.x; z < (1u << m); z += blockDim.x) {
+            uint32_t v = m31_mul(lo1[z & mask], hi1[z >> h]);
+            if (two) v = m31_add(v, m31_mul(lo2[z & mask], hi2[z >> h]));
+            o[z] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// claimed_v[j] = sum_z eq_j[z] * out[z], group j on rep j
+template <int L>
+__device__ void block_claims(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ eq, uint64_t eq_rs,
+                             uint32_t M, uint32_t R, uint32_t *__restrict__ claims)
+{
+    constexpr uint32_t LG = L / 4, Q = 3 * L;
+    __shared__ uint32_t red[4][4][3 * L];
+    __shared__ uint32_t sums[3 * Q], idle[3 * L];
+    const uint32_t grp = threadIdx.x >> 8, tg = threadIdx.x & 255, g = tg % LG;
+    uint64_t acc[3][4] = {};
+    if (grp < R) {
+        const uint32_t *e = eq + grp * eq_rs;
+        for (uint64_t w = tg; w < (uint64_t)M * LG; w += 256) {
+            const uint64_t z = w / LG;
+            const uint4 t = mul4s(ld4(vals + z * L + 4 * g), e[z]);
+            acc[0][0] += t.x; acc[0][1] += t.y; acc[0][2] += t.z; acc[0][3] += t.w;
         }
     }
-    if (p.layer_end) {
-        for (uint32_t j = 0; j < p.R; j++) tr.append_words(p.vy + j * L, L);
-        p.ab_out[0] = tr.challenge();
-        p.ab_out[1] = tr.challenge();
-    }
-    tr.store(p.tstate);
+    group_reduce<L>(acc, red[grp], grp < R ? sums + grp * Q : idle);
+    __syncthreads();
+    if (threadIdx.x < R * L) claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
+    __syncthreads();
 }
 
 template <int L>
-int forward_impl(const DevCircuit &dc, hipStream_t st)
+__device__ void block_gx(uint32_t *__restrict__ hg, uint64_t hg_rs, const uint32_t *__restrict__ v,
+                         const uint32_t *__restrict__ eqz, uint64_t eqz_rs, const uint32_t *__restrict__ ptr,
+                         const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
+{
+    constexpr uint32_t LG = L / 4;
+    for (uint64_t w = threadIdx.x; w < (uint64_t)N * LG; w += blockDim.x) {
+        const uint32_t x = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
+        uint4 acc[3] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+        const uint32_t e1 = ptr[x + 1];
+        for (uint32_t e = ptr[x]; e < e1; e++) {
+            const uint32_t b = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+            uint32_t ce[3];
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++) ce[j] = j < R ? m31_mul(c, eqz[j * eqz_rs + z]) : 0u;
+            if (b != kAddMark) {
+                const uint4 vb = ld4(v + (uint64_t)b * L + 4 * g);
+#pragma unroll
+                for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], mul4s(vb, ce[j]));
+            } else {
+#pragma unroll
+                for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], make_uint4(ce[j], ce[j], ce[j], ce[j]));
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 3; j++)
+            if (j < R) st4(hg + j * hg_rs + (uint64_t)x * L + 4 * g, acc[j]);
+    }
+}
+
+__device__ void block_hy(uint32_t *__restrict__ s, uint64_t s_rs, const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
+                         const uint32_t *__restrict__ eqx, uint64_t eqx_rs, const uint32_t *__restrict__ ptr,
+                         const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
+{
+    for (uint32_t y = threadIdx.x; y < N; y += blockDim.x) {
+        uint32_t acc[3] = {0, 0, 0};
+        const uint32_t e1 = ptr[y + 1];
+        for (uint32_t e = ptr[y]; e < e1; e++) {
+            const uint32_t x = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++)
+                if (j < R) acc[j] = m31_add(acc[j], m31_mul(m31_mul(eqz[j * eqz_rs + z], eqx[j * eqx_rs + x]), c));
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 3; j++)
+            if (j < R) s[j * s_rs + y] = acc[j];
+    }
+}
+
+template <int L>
+__global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *__restrict__ jobs,
+                                                               const uint32_t *__restrict__ order, uint32_t R,
+                                                               int hash_kind)
+{
+    const BatchJob J = jobs[order[blockIdx.x]];
+    const uint32_t D = J.D;
+    auto chal_slot = [&](uint32_t l) { return J.chal + (uint64_t)l * R * kChalStride; };
+    auto ab_slot = [&](uint32_t l) { return J.ab + 2ull * l; };
+    // transcript seed from the host prefix: rz1 -> slot D, alpha = 1, beta = 0, state, grind bytes
+    for (uint32_t t = threadIdx.x; t < R * kChalStride; t += blockDim.x) chal_slot(D)[t] = J.seed[t];
+    if (threadIdx.x < 10) J.tstate[threadIdx.x] = J.seed[R * kChalStride + threadIdx.x];
+    if (threadIdx.x < 8) J.tail[threadIdx.x] = J.seed[R * kChalStride + 10 + threadIdx.x];
+    if (threadIdx.x == 0) {
+        ab_slot(D)[0] = 1u;
+        ab_slot(D)[1] = 0u;
+    }
+    // forward pass (Circuit::evaluate)
+    for (uint32_t l = 0; l < D; l++) {
+        const LayerDev &ly = J.layers[l];
+        block_forward<L>(J.values + ly.v_off, J.values + ly.out_off, ly.fw_ptr, ly.fw_ent, 1u << ly.m);
+        __syncthreads();
+    }
+    for (int l = (int)D - 1; l >= 0; l--) {
+        const LayerDev &ly = J.layers[l];
+        const uint32_t n = ly.n, m = ly.m;
+        const uint32_t *V = J.values + ly.v_off;
+        uint32_t *chal_l = chal_slot(l);
+        block_eq(J.eqz, J.eqz_rs, chal_slot(l + 1), 0u, 32u, ab_slot(l + 1), m, R);
+        if (l == (int)D - 1) block_claims<L>(J.values + ly.out_off, J.eqz, J.eqz_rs, 1u << m, R, J.claims);
+        block_gx<L>(J.hg0, J.hg0_rs, V, J.eqz, J.eqz_rs, ly.gx_ptr, ly.gx_ent, 1u << n, R);
+        __syncthreads();
+        TsArgs ts{};
+        ts.R = R; ts.chal = chal_l; ts.vx = J.vxy; ts.vy = J.vxy + R * L;
+        ts.tstate = J.tstate; ts.tail = J.tail; ts.ab_out = ab_slot(l); ts.hash_kind = hash_kind;
+        TailParams tp{};
+        tp.f0 = V; tp.f0_rs = 0; tp.fA = J.fA; tp.fB = J.fB; tp.fA_rs = J.fA_rs; tp.fB_rs = J.fB_rs;
+        tp.k0 = 0; tp.n = n;
+        tp.h0 = J.hg0; tp.h0_rs = J.hg0_rs; tp.hA = J.hgA; tp.hB = J.hgB; tp.hA_rs = J.hgA_rs; tp.hB_rs = J.hgB_rs;
+        tp.ts = ts;
+        tp.ts.phase = 1;
+        phase_rounds<L, false>(tp);
+        block_eq(J.eqx, J.eqx_rs, chal_l, 0u, 0u, nullptr, n, R);
+        block_hy(J.s0, J.s0_rs, J.eqz, J.eqz_rs, J.eqx, J.eqx_rs, ly.hy_ptr, ly.hy_ent, 1u << n, R);
+        __syncthreads();
+        tp.h0 = J.s0; tp.h0_rs = J.s0_rs; tp.hA = J.sA; tp.hB = J.sB; tp.hA_rs = J.sA_rs; tp.hB_rs = J.sB_rs;
+        tp.ts.phase = 2;
+        phase_rounds<L, true>(tp);
+    }
+}
+
+// Records a HIP event pair around one launch when profiling is on.
+template <typename F>
+inline void timed(Profiler *pf, int cls, hipStream_t st, F &&launch)
+{
+    if (pf) pf->begin(cls, st);
+    launch();
+    if (pf) pf->end(st);
+}
+
+template <int L>
+int forward_impl(const DevCircuit &dc, hipStream_t st, Profiler *pf)
 {
     for (uint32_t l = 0; l < dc.D; l++) {
         const LayerDev &ly = dc.layers[l];
         const uint64_t M = 1ull << ly.m;
         const uint64_t work = M * (L / 4);
         const uint32_t blocks = (uint32_t)std::min<uint64_t>((work + kThreads - 1) / kThreads, 8192);
-        hipLaunchKernelGGL(k_forward<L>, dim3(blocks), dim3(kThreads), 0, st, dc.values + ly.v_off,
-                           dc.values + ly.out_off, ly.fw_ptr, ly.fw_ent, (uint32_t)M);
+        timed(pf, kKForward, st, [&] {
+            hipLaunchKernelGGL(k_forward<L>, dim3(blocks), dim3(kThreads), 0, st, dc.values + ly.v_off,
+                               dc.values + ly.out_off, ly.fw_ptr, ly.fw_ent, (uint32_t)M);
+        });
     }
     GKRX_HIP_CHECK(hipGetLastError());
     return GKRX_OK;
 }
+
+// rows reserved per rep in a ping-pong table: at least 4 so that every rep base of a
+// scalar table stays 16-byte aligned
+inline uint64_t scratch_rows(uint64_t rows) { return std::max<uint64_t>(4, (rows + 3) & ~3ull); }
 
 uint32_t grid_for(uint64_t work)
 {
@@ -485,15 +851,15 @@ uint32_t grid_for(uint64_t work)
 }
 
 template <int L>
-int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, hipStream_t st)
+int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, hipStream_t st, Profiler *pf)
 {
     constexpr uint32_t LG = L / 4;
     const uint32_t D = dc.D;
     const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
     const uint64_t eqz_rs = maxM, eqx_rs = maxN;
-    const uint64_t hg0_rs = maxN * L, hgA_rs = (maxN / 2) * L, hgB_rs = std::max<uint64_t>(maxN / 4, 1) * L;
-    const uint64_t fA_rs = (maxN / 2) * L, fB_rs = std::max<uint64_t>(maxN / 4, 1) * L;
-    const uint64_t s0_rs = maxN, sA_rs = maxN / 2, sB_rs = std::max<uint64_t>(maxN / 4, 1);
+    const uint64_t hg0_rs = maxN * L, hgA_rs = scratch_rows(maxN / 2) * L, hgB_rs = scratch_rows(maxN / 4) * L;
+    const uint64_t fA_rs = scratch_rows(maxN / 2) * L, fB_rs = scratch_rows(maxN / 4) * L;
+    const uint64_t s0_rs = scratch_rows(maxN), sA_rs = scratch_rows(maxN / 2), sB_rs = scratch_rows(maxN / 4);
     auto chal_slot = [&](uint32_t l) { return s.chal + (uint64_t)l * R * kChalStride; };
     auto ab_slot = [&](uint32_t l) { return s.ab + 2ull * l; };
 
@@ -507,85 +873,102 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, h
         // eqZ = alpha * eq(rz1, .) + beta * eq(rz2, .), rz1 = rx and rz2 = ry of the layer above
         {
             dim3 grid((uint32_t)((M + kEqChunk * 256 - 1) / (kEqChunk * 256)), R);
-            hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqz, eqz_rs, chal_prev, 0u, 32u,
-                               (const uint32_t *)ab_slot(l + 1), m);
+            timed(pf, kKEq, st, [&] {
+                hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqz, eqz_rs, chal_prev, 0u, 32u,
+                                   (const uint32_t *)ab_slot(l + 1), m);
+            });
         }
         if (l == (int)D - 1) {
-            // claimed_v[j] = eval_multilinear(output, rz1[j]) = sum_z eq(rz1[j], z) out[z]
             const uint32_t nbx = grid_for(M * LG);
-            hipLaunchKernelGGL(k_dot<L>, dim3(nbx, R), dim3(kThreads), 0, st, dc.values + ly.out_off, s.eqz, eqz_rs,
-                               (uint32_t)M, s.partials, R);
-            FinishParams fp{};
-            fp.partials = s.partials; fp.nbx = nbx; fp.R = R; fp.mode = 1; fp.claims = s.claims;
-            hipLaunchKernelGGL(k_finish<L>, dim3(1), dim3(kFinishThreads), 0, st, fp);
+            timed(pf, kKDot, st, [&] {
+                hipLaunchKernelGGL(k_dot<L>, dim3(nbx, R), dim3(kThreads), 0, st, dc.values + ly.out_off, s.eqz,
+                                   eqz_rs, (uint32_t)M, s.partials, s.ticket, R, s.claims);
+            });
         }
         {
-            dim3 grid(grid_for(N * LG) * 4, R);
-            hipLaunchKernelGGL(k_gx<L>, grid, dim3(kThreads), 0, st, s.hg0, hg0_rs, V, s.eqz, eqz_rs, ly.gx_ptr,
-                               ly.gx_ent, (uint32_t)N);
+            dim3 grid(grid_for(N * LG) * 4);
+            timed(pf, kKGx, st, [&] {
+                hipLaunchKernelGGL(k_gx<L>, grid, dim3(kThreads), 0, st, s.hg0, hg0_rs, V, s.eqz, eqz_rs, ly.gx_ptr,
+                                   ly.gx_ent, (uint32_t)N, R);
+            });
         }
         for (int phase = 1; phase <= 2; phase++) {
             if (phase == 2) {
                 dim3 grid((uint32_t)((N + kEqChunk * 256 - 1) / (kEqChunk * 256)), R);
-                hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqx, eqx_rs, (const uint32_t *)chal_l, 0u,
-                                   0u, (const uint32_t *)nullptr, n);
-                dim3 g2(grid_for(N) * 4, R);
-                hipLaunchKernelGGL(k_hy, g2, dim3(kThreads), 0, st, s.s0, s0_rs, s.eqz, eqz_rs, s.eqx, eqx_rs,
-                                   ly.hy_ptr, ly.hy_ent, (uint32_t)N);
+                timed(pf, kKEq, st, [&] {
+                    hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqx, eqx_rs, (const uint32_t *)chal_l,
+                                       0u, 0u, (const uint32_t *)nullptr, n);
+                });
+                dim3 g2(grid_for(N) * 4);
+                timed(pf, kKHy, st, [&] {
+                    hipLaunchKernelGGL(k_hy, g2, dim3(kThreads), 0, st, s.s0, s0_rs, s.eqz, eqz_rs, s.eqx, eqx_rs,
+                                       ly.hy_ptr, ly.hy_ent, (uint32_t)N, R);
+                });
             }
-            for (uint32_t k = 0; k < n; k++) {
+            uint32_t *hA = phase == 1 ? s.hgA : s.sA, *hB = phase == 1 ? s.hgB : s.sB;
+            const uint32_t *h0 = phase == 1 ? s.hg0 : s.s0;
+            const uint64_t hA_rs = phase == 1 ? hgA_rs : sA_rs, hB_rs = phase == 1 ? hgB_rs : sB_rs;
+            const uint64_t h0_rs = phase == 1 ? hg0_rs : s0_rs;
+            TsArgs ts{};
+            ts.R = R; ts.phase = phase; ts.chal = chal_l; ts.vx = s.vxy; ts.vy = s.vxy + R * L;
+            ts.tstate = s.tstate; ts.tail = s.tail; ts.ab_out = ab_slot(l); ts.hash_kind = hash_kind;
+            // rounds with more than kTailRows rows run on the whole GPU, the rest in k_tail
+            uint32_t k0 = 0;
+            while (k0 < n && (N >> k0) > kTailRows) k0++;
+            for (uint32_t k = 0; k < k0; k++) {
                 RoundParams rp{};
                 rp.items = (uint32_t)(N >> (k + 1));
-                rp.R = R;
                 rp.partials = s.partials;
-                // f source / destination (ping-pong A: N/2 rows, B: N/4 rows)
-                const uint32_t *fcur;
-                uint64_t fcur_rs;
+                rp.ticket = s.ticket;
+                rp.ts = ts;
+                rp.ts.k = k;
+                rp.ts.last = 0;  // the last round is always in k_tail
                 if (k == 0) {
                     rp.fsrc = V; rp.fsrc_rs = 0;
-                    fcur = V; fcur_rs = 0;
+                    rp.hsrc = h0; rp.hsrc_rs = h0_rs;
                 } else {
                     rp.fsrc = (k == 1) ? V : ((k - 1) & 1 ? s.fA : s.fB);
                     rp.fsrc_rs = (k == 1) ? 0 : ((k - 1) & 1 ? fA_rs : fB_rs);
                     rp.fdst = (k & 1) ? s.fA : s.fB;
                     rp.fdst_rs = (k & 1) ? fA_rs : fB_rs;
-                    fcur = rp.fdst; fcur_rs = rp.fdst_rs;
+                    rp.hsrc = (k == 1) ? h0 : ((k - 1) & 1 ? hA : hB);
+                    rp.hsrc_rs = (k == 1) ? h0_rs : ((k - 1) & 1 ? hA_rs : hB_rs);
+                    rp.hdst = (k & 1) ? hA : hB;
+                    rp.hdst_rs = (k & 1) ? hA_rs : hB_rs;
                     rp.rprev = chal_l + (phase == 2 ? 32 : 0) + (k - 1);
-                }
-                if (phase == 1) {
-                    if (k == 0) { rp.hsrc = s.hg0; rp.hsrc_rs = hg0_rs; }
-                    else {
-                        rp.hsrc = (k == 1) ? s.hg0 : ((k - 1) & 1 ? s.hgA : s.hgB);
-                        rp.hsrc_rs = (k == 1) ? hg0_rs : ((k - 1) & 1 ? hgA_rs : hgB_rs);
-                        rp.hdst = (k & 1) ? s.hgA : s.hgB;
-                        rp.hdst_rs = (k & 1) ? hgA_rs : hgB_rs;
-                    }
-                } else {
-                    if (k == 0) { rp.hsrc = s.s0; rp.hsrc_rs = s0_rs; }
-                    else {
-                        rp.hsrc = (k == 1) ? s.s0 : ((k - 1) & 1 ? s.sA : s.sB);
-                        rp.hsrc_rs = (k == 1) ? s0_rs : ((k - 1) & 1 ? sA_rs : sB_rs);
-                        rp.hdst = (k & 1) ? s.sA : s.sB;
-                        rp.hdst_rs = (k & 1) ? sA_rs : sB_rs;
-                    }
                 }
                 const uint32_t nbx = grid_for((uint64_t)rp.items * LG);
                 dim3 grid(nbx, R);
-                if (phase == 1) {
-                    if (k == 0) hipLaunchKernelGGL((k_round<L, false, false>), grid, dim3(kThreads), 0, st, rp);
-                    else hipLaunchKernelGGL((k_round<L, false, true>), grid, dim3(kThreads), 0, st, rp);
-                } else {
-                    if (k == 0) hipLaunchKernelGGL((k_round<L, true, false>), grid, dim3(kThreads), 0, st, rp);
-                    else hipLaunchKernelGGL((k_round<L, true, true>), grid, dim3(kThreads), 0, st, rp);
-                }
-                FinishParams fp{};
-                fp.partials = s.partials; fp.nbx = nbx; fp.R = R; fp.mode = 0;
-                fp.phase = phase; fp.last = (k == n - 1); fp.layer_end = (phase == 2 && k == n - 1);
-                fp.k = k; fp.chal = chal_l; fp.vx = s.vxy; fp.vy = s.vxy + R * L;
-                fp.fin = fcur; fp.fin_rs = fcur_rs;
-                fp.tstate = s.tstate; fp.tail = s.tail; fp.ab_out = ab_slot(l); fp.hash_kind = hash_kind;
-                hipLaunchKernelGGL(k_finish<L>, dim3(1), dim3(kFinishThreads), 0, st, fp);
+                const int cls = phase == 1 ? (k == 0 ? kKRoundPh1First : kKRoundPh1)
+                                           : (k == 0 ? kKRoundPh2First : kKRoundPh2);
+                timed(pf, cls, st, [&] {
+                    if (phase == 1) {
+                        if (k == 0) hipLaunchKernelGGL((k_round<L, false, false>), grid, dim3(kThreads), 0, st, rp);
+                        else hipLaunchKernelGGL((k_round<L, false, true>), grid, dim3(kThreads), 0, st, rp);
+                    } else {
+                        if (k == 0) hipLaunchKernelGGL((k_round<L, true, false>), grid, dim3(kThreads), 0, st, rp);
+                        else hipLaunchKernelGGL((k_round<L, true, true>), grid, dim3(kThreads), 0, st, rp);
+                    }
+                });
             }
+            TailParams tp{};
+            if (k0 == 0) {
+                tp.f0 = V; tp.f0_rs = 0; tp.h0 = h0; tp.h0_rs = h0_rs;
+            } else if (k0 == 1) {
+                tp.f0 = V; tp.f0_rs = 0; tp.h0 = h0; tp.h0_rs = h0_rs;
+            } else {
+                tp.f0 = (k0 - 1) & 1 ? s.fA : s.fB;
+                tp.f0_rs = (k0 - 1) & 1 ? fA_rs : fB_rs;
+                tp.h0 = (k0 - 1) & 1 ? hA : hB;
+                tp.h0_rs = (k0 - 1) & 1 ? hA_rs : hB_rs;
+            }
+            tp.fA = s.fA; tp.fB = s.fB; tp.hA = hA; tp.hB = hB;
+            tp.fA_rs = fA_rs; tp.fB_rs = fB_rs; tp.hA_rs = hA_rs; tp.hB_rs = hB_rs;
+            tp.k0 = k0; tp.n = n; tp.ts = ts;
+            timed(pf, phase == 1 ? kKTailPh1 : kKTailPh2, st, [&] {
+                if (phase == 1) hipLaunchKernelGGL((k_tail<L, false>), dim3(1), dim3(kTailThreads), 0, st, tp);
+                else hipLaunchKernelGGL((k_tail<L, true>), dim3(1), dim3(kTailThreads), 0, st, tp);
+            });
         }
     }
     GKRX_HIP_CHECK(hipGetLastError());
@@ -595,32 +978,42 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, h
 }  // namespace
 
 // ------------------------------------------------------------------ host side
-int dev_forward(const DevCircuit &dc, hipStream_t st)
+int dev_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf)
 {
-    if (dc.L == 8) return forward_impl<8>(dc, st);
-    if (dc.L == 16) return forward_impl<16>(dc, st);
+    if (dc.L == 8) return forward_impl<8>(dc, st, pf);
+    if (dc.L == 16) return forward_impl<16>(dc, st, pf);
     return GKRX_ERR_CONFIG;
 }
 
 int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, const uint32_t *rz1_host,
-              const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st)
+              const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,
+              Profiler *pf)
 {
     if (R > 3) return GKRX_ERR_CONFIG;
     const uint32_t D = dc.D;
     const uint32_t mlast = dc.layers[D - 1].m;
-    // seed slot D: rx = rz1, ry = 0 (rz2), alpha = 1, beta = 0
-    std::vector<uint32_t> seed((size_t)R * kChalStride, 0u);
+    // seed slot D: rx = rz1, ry = 0 (rz2), alpha = 1, beta = 0; staged in pinned memory
+    // because the copies complete after this function returns
+    const uint64_t seed_u32 = (uint64_t)R * kChalStride;
+    const uint64_t need = seed_u32 + 2 + 10 + (tail_prefix_len + 3) / 4;
+    if (need > s.stage_u32) return GKRX_ERR_ARG;
+    GKRX_HIP_CHECK(hipStreamSynchronize(st));  // the previous prove may still read the stage
+    uint32_t *seed = s.stage, *ab_seed = seed + seed_u32, *ts = ab_seed + 2, *pre = ts + 10;
+    memset(seed, 0, seed_u32 * 4);
     for (uint32_t j = 0; j < R; j++)
         for (uint32_t i = 0; i < mlast; i++) seed[(size_t)j * kChalStride + i] = rz1_host[(size_t)j * mlast + i];
-    GKRX_HIP_CHECK(hipMemcpyAsync(s.chal + (uint64_t)D * R * kChalStride, seed.data(), seed.size() * 4,
+    ab_seed[0] = 1u;
+    ab_seed[1] = 0u;
+    memcpy(ts, tstate_host, 40);
+    if (tail_prefix_len) memcpy(pre, tail_prefix, tail_prefix_len);
+    GKRX_HIP_CHECK(hipMemcpyAsync(s.chal + (uint64_t)D * R * kChalStride, seed, seed_u32 * 4,
                                   hipMemcpyHostToDevice, st));
-    const uint32_t ab_seed[2] = {1u, 0u};
     GKRX_HIP_CHECK(hipMemcpyAsync(s.ab + 2ull * D, ab_seed, 8, hipMemcpyHostToDevice, st));
-    GKRX_HIP_CHECK(hipMemcpyAsync(s.tstate, tstate_host, 10 * 4, hipMemcpyHostToDevice, st));
+    GKRX_HIP_CHECK(hipMemcpyAsync(s.tstate, ts, 40, hipMemcpyHostToDevice, st));
     if (tail_prefix_len)
-        GKRX_HIP_CHECK(hipMemcpyAsync(s.tail, tail_prefix, tail_prefix_len, hipMemcpyHostToDevice, st));
-    if (dc.L == 8) return prove_impl<8>(dc, s, R, hash_kind, st);
-    if (dc.L == 16) return prove_impl<16>(dc, s, R, hash_kind, st);
+        GKRX_HIP_CHECK(hipMemcpyAsync(s.tail, pre, tail_prefix_len, hipMemcpyHostToDevice, st));
+    if (dc.L == 8) return prove_impl<8>(dc, s, R, hash_kind, st, pf);
+    if (dc.L == 16) return prove_impl<16>(dc, s, R, hash_kind, st, pf);
     return GKRX_ERR_CONFIG;
 }
 
@@ -686,6 +1079,7 @@ int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &
     uint64_t pool = 0;
     for (auto &v : host) pool += (v.size() + 3) & ~3ull;  // 16-byte aligned pieces
     GKRX_HIP_CHECK(hipMalloc(&dc.csr_pool, std::max<uint64_t>(pool, 4) * 4));
+    GKRX_HIP_CHECK(hipMalloc(&dc.d_layers, sizeof(LayerDev) * dc.D));
     GKRX_HIP_CHECK(hipMalloc(&dc.values, dc.values_u32 * 4));
     uint64_t at = 0;
     for (uint32_t l = 0; l < dc.D; l++) {
@@ -698,6 +1092,7 @@ int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &
             at += (v.size() + 3) & ~3ull;
         }
     }
+    GKRX_HIP_CHECK(hipMemcpy(dc.d_layers, dc.layers.data(), sizeof(LayerDev) * dc.D, hipMemcpyHostToDevice));
     return GKRX_OK;
 }
 
@@ -705,8 +1100,10 @@ void dev_free_circuit(DevCircuit &dc)
 {
     if (dc.csr_pool) (void)hipFree(dc.csr_pool);
     if (dc.values) (void)hipFree(dc.values);
+    if (dc.d_layers) (void)hipFree(dc.d_layers);
     dc.csr_pool = nullptr;
     dc.values = nullptr;
+    dc.d_layers = nullptr;
 }
 
 int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t tail_bytes)
@@ -717,14 +1114,14 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
     dev_free_scratch(s);
     const uint32_t L = dc.L;
     const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
-    const uint64_t N2 = std::max<uint64_t>(maxN / 2, 1), N4 = std::max<uint64_t>(maxN / 4, 1);
+    const uint64_t N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4), N1 = scratch_rows(maxN);
     struct Piece { uint32_t **p; uint64_t u32; };
     Piece pieces[] = {
         {&s.eqz, R * maxM}, {&s.eqx, R * maxN}, {&s.hg0, R * maxN * L}, {&s.hgA, R * N2 * L},
-        {&s.hgB, R * N4 * L}, {&s.fA, R * N2 * L}, {&s.fB, R * N4 * L}, {&s.s0, R * maxN},
+        {&s.hgB, R * N4 * L}, {&s.fA, R * N2 * L}, {&s.fB, R * N4 * L}, {&s.s0, R * N1},
         {&s.sA, R * N2}, {&s.sB, R * N4}, {&s.partials, (uint64_t)R * kMaxBlocksPerRep * 3 * L},
         {&s.chal, (uint64_t)(dc.D + 1) * R * kChalStride}, {&s.ab, 2ull * (dc.D + 1)}, {&s.claims, R * L},
-        {&s.vxy, 2ull * R * L}, {&s.tstate, 16}, {&s.tail, (tail_bytes + 3) / 4 + 64}};
+        {&s.vxy, 2ull * R * L}, {&s.tstate, 16}, {&s.ticket, 16}, {&s.tail, (tail_bytes + 3) / 4 + 64}};
     uint64_t total = 0;
     for (auto &pc : pieces) total += (pc.u32 + 63) & ~63ull;
     GKRX_HIP_CHECK(hipMalloc(&s.pool, total * 4));
@@ -733,6 +1130,9 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
         *pc.p = reinterpret_cast<uint32_t *>(s.pool) + at;
         at += (pc.u32 + 63) & ~63ull;
     }
+    GKRX_HIP_CHECK(hipMemset(s.ticket, 0, 16 * 4));
+    s.stage_u32 = 1024;
+    GKRX_HIP_CHECK(hipHostMalloc(&s.stage, s.stage_u32 * 4, hipHostMallocDefault));
     s.bytes = total * 4;
     s.max_n = dc.max_n;
     s.max_m = dc.max_m;
@@ -745,7 +1145,131 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
 void dev_free_scratch(DevScratch &s)
 {
     if (s.pool) (void)hipFree(s.pool);
+    if (s.stage) (void)hipHostFree(s.stage);
     s = DevScratch{};
+}
+
+// ------------------------------------------------------------------ batched prove (host)
+int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_kind, BatchScratch &bs, hipStream_t st,
+                    Profiler *pf)
+{
+    const size_t nj = items.size();
+    if (nj == 0) return GKRX_OK;
+    if (R > 3) return GKRX_ERR_CONFIG;
+    const uint32_t L = items[0].dc->L;
+    // sizes of every job's scratch, one pool
+    std::vector<uint64_t> job_off(nj + 1, 0), tail_off(nj + 1, 0);
+    const uint64_t seed_u32 = (uint64_t)R * kChalStride + 10 + 8;
+    for (size_t i = 0; i < nj; i++) {
+        const DevCircuit &dc = *items[i].dc;
+        if (dc.L != L) return GKRX_ERR_ARG;
+        const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
+        const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
+        uint64_t u = 0;
+        auto add = [&](uint64_t n) { u += (n + 63) & ~63ull; };
+        add(R * maxM); add(R * maxN); add(R * maxN * L); add(R * N2 * L); add(R * N4 * L); add(R * N2 * L);
+        add(R * N4 * L); add(R * N1); add(R * N2); add(R * N4); add((uint64_t)(dc.D + 1) * R * kChalStride);
+        add(2ull * (dc.D + 1)); add(2ull * R * L); add(16);
+        job_off[i + 1] = job_off[i] + u;
+        tail_off[i + 1] = tail_off[i] + ((items[i].tail_bytes + 3) / 4 + 63) / 64 * 64;
+    }
+    const uint64_t need = job_off[nj] + tail_off[nj] + nj * (uint64_t)R * L + nj * seed_u32 + 64;
+    const uint64_t need_jobs = nj * sizeof(BatchJob) + nj * 4 + 64;
+    if (need > bs.pool_u32 || need_jobs > bs.meta_bytes) {
+        if (bs.pool) (void)hipFree(bs.pool);
+        if (bs.meta) (void)hipFree(bs.meta);
+        if (bs.host_meta) (void)hipHostFree(bs.host_meta);
+        bs = BatchScratch{};
+        GKRX_HIP_CHECK(hipMalloc(&bs.pool, need * 4));
+        GKRX_HIP_CHECK(hipMalloc(&bs.meta, need_jobs));
+        GKRX_HIP_CHECK(hipHostMalloc(&bs.host_meta, need_jobs + nj * seed_u32 * 4, hipHostMallocDefault));
+        bs.pool_u32 = need;
+        bs.meta_bytes = need_jobs;
+    }
+    GKRX_HIP_CHECK(hipStreamSynchronize(st));  // the previous batch may still read the staging
+    uint32_t *tails = bs.pool + job_off[nj];
+    uint32_t *claims = tails + tail_off[nj];
+    uint32_t *seeds = claims + nj * (uint64_t)R * L;
+    BatchJob *hjobs = reinterpret_cast<BatchJob *>(bs.host_meta);
+    uint32_t *horder = reinterpret_cast<uint32_t *>(bs.host_meta + nj * sizeof(BatchJob));
+    uint32_t *hseed = reinterpret_cast<uint32_t *>(bs.host_meta + need_jobs);
+    std::vector<std::pair<uint64_t, uint32_t>> work(nj);
+    for (size_t i = 0; i < nj; i++) {
+        const DevCircuit &dc = *items[i].dc;
+        const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
+        const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
+        uint32_t *at = bs.pool + job_off[i];
+        auto take = [&](uint64_t n) { uint32_t *p = at; at += (n + 63) & ~63ull; return p; };
+        BatchJob &J = hjobs[i];
+        J = BatchJob{};
+        J.D = dc.D;
+        J.layers = dc.d_layers;
+        J.values = dc.values;
+        J.eqz_rs = maxM; J.eqz = take(R * maxM);
+        J.eqx_rs = maxN; J.eqx = take(R * maxN);
+        J.hg0_rs = maxN * L; J.hg0 = take(R * maxN * L);
+        J.hgA_rs = N2 * L; J.hgA = take(R * N2 * L);
+        J.hgB_rs = N4 * L; J.hgB = take(R * N4 * L);
+        J.fA_rs = N2 * L; J.fA = take(R * N2 * L);
+        J.fB_rs = N4 * L; J.fB = take(R * N4 * L);
+        J.s0_rs = N1; J.s0 = take(R * N1);
+        J.sA_rs = N2; J.sA = take(R * N2);
+        J.sB_rs = N4; J.sB = take(R * N4);
+        J.chal = take((uint64_t)(dc.D + 1) * R * kChalStride);
+        J.ab = take(2ull * (dc.D + 1));
+        J.vxy = take(2ull * R * L);
+        J.tstate = take(16);
+        J.tail = tails + tail_off[i];
+        J.claims = claims + i * (uint64_t)R * L;
+        J.seed = seeds + i * seed_u32;
+        // seed record: rz1 in chal-slot layout, transcript state, grind bytes
+        uint32_t *sd = hseed + i * seed_u32;
+        memset(sd, 0, seed_u32 * 4);
+        const uint32_t mlast = dc.layers[dc.D - 1].m;
+        for (uint32_t j = 0; j < R; j++)
+            for (uint32_t v = 0; v < mlast; v++) sd[j * kChalStride + v] = items[i].rz1[(size_t)j * mlast + v];
+        memcpy(sd + R * kChalStride, items[i].tstate, 40);
+        memcpy(sd + R * kChalStride + 10, items[i].grind, 32);
+        uint64_t w = 0;
+        for (uint32_t l = 0; l < dc.D; l++) w += (1ull << dc.layers[l].n);
+        work[i] = {w, (uint32_t)i};
+    }
+    std::stable_sort(work.begin(), work.end(), [](auto &a, auto &b) { return a.first > b.first; });
+    for (size_t i = 0; i < nj; i++) horder[i] = work[i].second;
+    GKRX_HIP_CHECK(hipMemcpyAsync(bs.meta, bs.host_meta, need_jobs, hipMemcpyHostToDevice, st));
+    GKRX_HIP_CHECK(hipMemcpyAsync(seeds, hseed, nj * seed_u32 * 4, hipMemcpyHostToDevice, st));
+    const BatchJob *djobs = reinterpret_cast<const BatchJob *>(bs.meta);
+    const uint32_t *dorder = reinterpret_cast<const uint32_t *>(bs.meta + nj * sizeof(BatchJob));
+    timed(pf, kKBatch, st, [&] {
+        if (L == 8)
+            hipLaunchKernelGGL(k_prove_batch<8>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder, R,
+                               hash_kind);
+        else
+            hipLaunchKernelGGL(k_prove_batch<16>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder, R,
+                               hash_kind);
+    });
+    GKRX_HIP_CHECK(hipGetLastError());
+    for (size_t i = 0; i < nj; i++) {
+        GKRX_HIP_CHECK(hipMemcpyAsync(items[i].tail_out, tails + tail_off[i], items[i].tail_bytes,
+                                      hipMemcpyDeviceToHost, st));
+    }
+    GKRX_HIP_CHECK(hipMemcpyAsync(bs.host_claims(nj * R * L), claims, nj * (uint64_t)R * L * 4,
+                                  hipMemcpyDeviceToHost, st));
+    return GKRX_OK;
+}
+
+int dev_batch_supported(const DevCircuit &dc)
+{
+    return dc.max_n <= kBatchMaxVars && dc.max_m <= kBatchMaxVars;
+}
+
+void dev_free_batch(BatchScratch &bs)
+{
+    if (bs.pool) (void)hipFree(bs.pool);
+    if (bs.meta) (void)hipFree(bs.meta);
+    if (bs.host_meta) (void)hipHostFree(bs.host_meta);
+    if (bs.claims_host) (void)hipHostFree(bs.claims_host);
+    bs = BatchScratch{};
 }
 
 }  // namespace gkrx

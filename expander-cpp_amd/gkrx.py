@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -69,6 +70,11 @@ def lib():
                                    C.POINTER(C.c_uint32), _u32p, C.POINTER(C.c_uint32)]
     L.gkrx_synth_witness_m31.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _u32p]
     L.gkrx_hash.argtypes = [C.c_int, C.c_char_p, C.c_size_t, C.c_char_p]
+    L.gkrx_prove_batch.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
+                                   C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]
+    L.gkrx_ctx_profile.argtypes = [vp, C.c_int]
+    L.gkrx_ctx_kernel_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -112,6 +118,20 @@ def write_text(layers, mul_path: str, add_path: str) -> None:
             fa.write(" ".join([str(len(add))] + [str(int(x)) for x in np.asarray(add).ravel()]) + "\n")
 
 
+def kernel_stats(ctx: "Context") -> dict:
+    """{kernel class: {"count": launches, "ms": total ms}} recorded while profiling was on."""
+    out = {}
+    i = 0
+    while True:
+        name, cnt, ms = C.c_char_p(), C.c_uint64(), C.c_double()
+        if lib().gkrx_ctx_kernel_stat(ctx._h, i, C.byref(name), C.byref(cnt), C.byref(ms)) != 0:
+            break
+        if cnt.value:
+            out[name.value.decode()] = {"count": int(cnt.value), "ms": float(ms.value)}
+        i += 1
+    return out
+
+
 def host_hash(fs_hash: int, data: bytes) -> bytes:
     out = C.create_string_buffer(32)
     _check(lib().gkrx_hash(fs_hash, data, len(data), out), "gkrx_hash")
@@ -146,13 +166,18 @@ class Context:
 
     def __init__(self, config: Config, device: int = 0):
         self.config = config
+        self._h = None
+        self._circuits = weakref.WeakSet()
         h = C.c_void_p()
         _check(lib().gkrx_ctx_create(C.byref(config._c), device, C.byref(h)), "gkrx_ctx_create")
         self._h = h
 
     def close(self):
+        """Destroys the context after every circuit created on it."""
         if self._h:
-            lib().gkrx_ctx_destroy(self._h)
+            for c in list(self._circuits):
+                c.close()
+            _check(lib().gkrx_ctx_destroy(self._h), "gkrx_ctx_destroy")
             self._h = None
 
     def __del__(self):
@@ -168,6 +193,7 @@ class Circuit:
     def __init__(self, ctx: Context, handle):
         self.ctx = ctx
         self._h = handle
+        ctx._circuits.add(self)
         n = C.c_uint32()
         _check(lib().gkrx_circuit_shape(handle, C.byref(n), None, None), "shape")
         self.nb_layers = n.value
@@ -246,6 +272,21 @@ class Prover:
         n = C.c_size_t()
         _check(lib().gkrx_prove(self.ctx._h, circuit._h, buf, buf.size, C.byref(n), claimed), "gkrx_prove")
         return claimed.reshape(R, -1), buf[:n.value]
+
+    def prove_batch(self, circuits, outs=None):
+        """All circuits in one batched launch; returns [(claimed[R, L], proof)] in order.
+        Each proof is byte-identical to prove() of the same circuit."""
+        n = len(circuits)
+        R, L = self.ctx.config.get_num_repetitions(), self.ctx.config.lanes
+        bufs = outs if outs is not None else [np.empty(self.proof_size(c), dtype=np.uint8) for c in circuits]
+        claims = [np.empty(R * L, dtype=np.uint32) for _ in circuits]
+        hs = (C.c_void_p * n)(*[c._h.value for c in circuits])
+        ps = (C.c_void_p * n)(*[b.ctypes.data for b in bufs])
+        caps = (C.c_size_t * n)(*[b.size for b in bufs])
+        lens = (C.c_size_t * n)()
+        cs = (C.c_void_p * n)(*[c.ctypes.data for c in claims])
+        _check(lib().gkrx_prove_batch(self.ctx._h, n, hs, ps, caps, lens, cs), "gkrx_prove_batch")
+        return [(claims[i].reshape(R, L), bufs[i][:lens[i]]) for i in range(n)]
 
     def last_timing(self):
         ms = (C.c_double * 3)()

@@ -59,6 +59,7 @@ int gkrx_config_num_repetitions(const gkrx_config *cfg);
 /* Prover::Prover(const Config&) (include/LinearGKR/LinearGKR.hpp:42-47) plus the device
  * resources the prover needs: one context per GPU.  device = HIP device ordinal. */
 int gkrx_ctx_create(const gkrx_config *cfg, int device, gkrx_ctx **out);
+/* Fails with GKRX_ERR_STATE while circuits created on the context are still alive. */
 int gkrx_ctx_destroy(gkrx_ctx *ctx);
 
 /* Circuit::load_extracted_gates (include/circuit/circuit.hpp:152-215): the text
@@ -91,6 +92,14 @@ size_t gkrx_proof_size(const gkrx_ctx *ctx, const gkrx_circuit *c);
  * of the tuple prove returns. */
 int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap, size_t *proof_len,
                uint32_t *claimed_v);
+/* n independent proofs at once (no reference counterpart: the reference proves one
+ * proof per thread, benchmarks/bench_keccak.cpp:32-42).  Proof i is byte-identical to
+ * gkrx_prove on circuits[i].  The transcript prefixes run on a pool of host threads, then
+ * one launch proves every circuit, one workgroup per proof (circuits must be at most
+ * 2^20 wide per layer).  proofs[i] holds proof_caps[i] >= gkrx_proof_size bytes;
+ * claimed_v[i] receives R * L u32. */
+int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, uint8_t *const *proofs,
+                     const size_t *proof_caps, size_t *proof_lens, uint32_t *const *claimed_v);
 /* Verifier::verify (LinearGKR.hpp:104-149), on the host.  Returns 1 = accepted,
  * 0 = rejected, negative = error. */
 int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v, const uint8_t *proof,
@@ -99,6 +108,13 @@ int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v,
 /* Millisecond split of the last gkrx_prove on this context: [0] host transcript prefix
  * (commitment hash + grinding + output challenges), [1] device work, [2] total. */
 int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3);
+
+/* Per-kernel-class timing with HIP events on the context's stream (benchmark support; no
+ * reference counterpart -- the reference's Timing class, sumcheck_helper.hpp:18-46,
+ * prints wall-clock phases).  enable = 1 resets the counters and starts recording;
+ * gkrx_ctx_kernel_stat reads class idx (0 .. classes-1; GKRX_ERR_ARG past the end). */
+int gkrx_ctx_profile(gkrx_ctx *ctx, int enable);
+int gkrx_ctx_kernel_stat(gkrx_ctx *ctx, int idx, const char **name, uint64_t *launches, double *ms);
 
 /* Thread-local message of the last failed call. */
 const char *gkrx_last_error(void);

@@ -311,6 +311,21 @@ static uint32_t tr_challenge(transcript *t)
     return oracle_m31_from_bytes(t->digest);
 }
 
+void *oracle_transcript_new(int hash_kind)
+{
+    transcript *t = (transcript *)malloc(sizeof(transcript));
+    tr_init(t, hash_kind, 1024);
+    return t;
+}
+void oracle_transcript_append(void *t, const uint8_t *bytes, size_t len) { tr_append((transcript *)t, bytes, len); }
+uint32_t oracle_transcript_challenge(void *t) { return tr_challenge((transcript *)t); }
+void oracle_transcript_free(void *t)
+{
+    if (!t) return;
+    free(((transcript *)t)->bytes);
+    free(t);
+}
+
 /* grind (include/LinearGKR/LinearGKR.hpp:13-32): 256/31 = 8 challenges serialized at
  * 4-byte strides (ceil(31, 8) = 4) into 32 bytes, hashed 2^10 times in place, appended. */
 static void grind(transcript *t)

@@ -55,6 +55,11 @@ static void sha256_block(uint32_t st[8], const uint8_t blk[64])
     st[4] += e; st[5] += f; st[6] += g; st[7] += h;
 }
 
+void oracle_sha256_blocks(uint32_t state[8], const uint8_t *data, size_t nblocks)
+{
+    for (size_t i = 0; i < nblocks; i++) sha256_block(state, data + 64 * i);
+}
+
 void oracle_sha256(const uint8_t *in, size_t len, uint8_t out[32])
 {
     uint32_t st[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,

@@ -45,10 +45,19 @@ uint32_t oracle_m31_from_bytes(const uint8_t *in); /* full reduction */
 
 /* --- hashes ------------------------------------------------------------------- */
 void oracle_sha256(const uint8_t *in, size_t len, uint8_t out[32]);
+/* Raw compression of nblocks 64-byte blocks into state (no padding), the unit the
+ * reference's SHA-256 self-test pins (lib/btc_sha256/crypto/sha256.cpp:489-575). */
+void oracle_sha256_blocks(uint32_t state[8], const uint8_t *data, size_t nblocks);
 /* Keccak sponge with rate 136 and a caller-chosen domain pad byte:
  * pad 0x01 = Keccak-256, pad 0x06 = SHA3-256. */
 void oracle_keccak256_pad(const uint8_t *in, size_t len, uint8_t pad, uint8_t out[32]);
 void oracle_keccak256(const uint8_t *in, size_t len, uint8_t out[32]);
+
+/* --- transcript (include/fiat_shamir/transcript.hpp:87-175) ------------------ */
+void *oracle_transcript_new(int hash_kind);
+void oracle_transcript_append(void *t, const uint8_t *bytes, size_t len);
+uint32_t oracle_transcript_challenge(void *t);
+void oracle_transcript_free(void *t);
 
 /* --- circuits ----------------------------------------------------------------- */
 typedef struct {

@@ -1,0 +1,81 @@
+"""Batched proving: one launch proves many circuits of uneven widths and depths; every
+proof is byte-identical to the same circuit proved alone (and, for small circuits, to the
+oracle), and the results do not depend on what else is in the batch."""
+import os
+
+import numpy as np
+import pytest
+
+import gkrx
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed(seed, count, kmin, kmax, dmin, dmax):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        k = int(rng.integers(kmin, kmax + 1))
+        d = int(rng.integers(dmin, dmax + 1))
+        specs.append((seed * 1000 + i, d, k))
+    return specs
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gkrx.Context(gkrx.Config(fs_hash=gkrx.FS_KECCAK256, nb_parallel=16))
+    yield c
+    c.close()
+
+
+def _circuits(ctx, specs):
+    out = []
+    for s, d, k in specs:
+        circ = gkrx.Circuit.from_layers(ctx, gkrx.synth_circuit(s, d, k, 700, True))
+        circ.set_witness(gkrx.synth_witness(s, 1 << k, 16))
+        out.append(circ)
+    return out
+
+
+def test_batch_equals_single_proofs(ctx):
+    print("[gpu] batch of 24 uneven proofs vs one at a time", flush=True)
+    specs = _mixed(3, 24, 2, 13, 1, 6)
+    circs = _circuits(ctx, specs)
+    prover = gkrx.Prover(ctx)
+    batch = prover.prove_batch(circs)
+    for (claimed, proof), circ in zip(batch, circs):
+        c1, p1 = prover.prove(circ)
+        assert np.array_equal(claimed, c1)
+        assert bytes(proof) == bytes(p1)
+        assert gkrx.Verifier(ctx).verify(circ, claimed, proof)
+    for c in circs:
+        c.close()
+
+
+def test_batch_matches_oracle_small(ctx, tmp_path):
+    print("[gpu] batch vs oracle", flush=True)
+    specs = _mixed(4, 6, 3, 8, 2, 4)
+    circs = _circuits(ctx, specs)
+    batch = gkrx.Prover(ctx).prove_batch(circs)
+    for (claimed, proof), (s, d, k) in zip(batch, specs):
+        mp, ap = str(tmp_path / f"m{s}.txt"), str(tmp_path / f"a{s}.txt")
+        gkrx.write_text(gkrx.synth_circuit(s, d, k, 700, True), mp, ap)
+        oc, op = ol.OracleCircuit(mp, ap).prove(16, ol.HASH_KECCAK256, gkrx.synth_witness(s, 1 << k, 16))
+        assert np.array_equal(claimed, oc) and bytes(proof) == op
+    for c in circs:
+        c.close()
+
+
+def test_result_independent_of_batch_mix(ctx):
+    """The same circuit proved alone in a batch of one, and inside a larger mix, gives the
+    same bytes."""
+    print("[gpu] batch mix independence", flush=True)
+    specs = _mixed(5, 10, 4, 12, 2, 8)
+    circs = _circuits(ctx, specs)
+    prover = gkrx.Prover(ctx)
+    mix = prover.prove_batch(circs)
+    alone = prover.prove_batch([circs[7]])
+    assert bytes(mix[7][1]) == bytes(alone[0][1]) and np.array_equal(mix[7][0], alone[0][0])
+    for c in circs:
+        c.close()

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 CASES = [
     (8, 10, 500, False, 8, gkrx.FS_SHA256, 0),      # the reference's CPU-runnable shape
     (8, 10, 500, False, 8, gkrx.FS_KECCAK256, 0),
-    (3, 1, 500, False, 8, gkrx.FS_SHA256, 3),       # smallest: one variable per layer
+    (3, 2, 500, False, 8, gkrx.FS_SHA256, 3),       # smallest valid width (max index 3)
     (2, 2, 500, False, 16, gkrx.FS_SHA256, 4),
     (4, 6, 700, True, 16, gkrx.FS_KECCAK256, 5),
     (3, 12, 700, True, 16, gkrx.FS_SHA256, 6),
@@ -82,6 +82,16 @@ def test_proof_matches_oracle(case, tmp_path, contexts):
     bad_claim[0, 0] = (int(bad_claim[0, 0]) + 1) % gkrx.P31
     assert not v.verify(circ, bad_claim, proof)
     circ.close()
+
+
+def test_width_two_layer_is_rejected(contexts):
+    """A layer whose largest index is 1 gets 0 variables from the reference's
+    _compute_nb_vars (next_pow_of_2(1) = 1, include/circuit/circuit.hpp:145-146), which
+    its prover cannot handle; the library refuses such circuits instead."""
+    ctx = _ctx(contexts, 8, gkrx.FS_SHA256)
+    layers = gkrx.synth_circuit(3, 2, 1, 500, False)
+    with pytest.raises(gkrx.GkrxError):
+        gkrx.Circuit.from_layers(ctx, layers)
 
 
 def test_repeat_prove_is_deterministic(tmp_path, contexts):
