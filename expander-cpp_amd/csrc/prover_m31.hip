@@ -527,11 +527,13 @@ __device__ void phase_rounds(const TailParams &p)
     __syncthreads();
     for (uint32_t k = p.k0; k < p.n; k++) {
         const uint64_t items = 1ull << (p.n - k - 1);
-        // table k lives in A for odd k, B for even k >= 2; table k-1 is the source
-        const uint32_t *fs = (k == p.k0) ? p.f0 : ((k - 1) & 1 ? p.fA : p.fB);
-        const uint64_t fs_rs = (k == p.k0) ? p.f0_rs : ((k - 1) & 1 ? p.fA_rs : p.fB_rs);
-        const uint32_t *hs = (k == p.k0) ? p.h0 : ((k - 1) & 1 ? p.hA : p.hB);
-        const uint64_t hs_rs = (k == p.k0) ? p.h0_rs : ((k - 1) & 1 ? p.hA_rs : p.hB_rs);
+        // table k lives in A for odd k, B for even k >= 2; table k-1 is the source.  Round
+        // k0 reads f0 / h0, and so does round 1 (table 0 is the layer input, hg0 or s0).
+        const bool first = (k == p.k0) || (k == 1);
+        const uint32_t *fs = first ? p.f0 : ((k - 1) & 1 ? p.fA : p.fB);
+        const uint64_t fs_rs = first ? p.f0_rs : ((k - 1) & 1 ? p.fA_rs : p.fB_rs);
+        const uint32_t *hs = first ? p.h0 : ((k - 1) & 1 ? p.hA : p.hB);
+        const uint64_t hs_rs = first ? p.h0_rs : ((k - 1) & 1 ? p.hA_rs : p.hB_rs);
         uint32_t *fd = (k & 1) ? p.fA : p.fB;
         const uint64_t fd_rs = (k & 1) ? p.fA_rs : p.fB_rs;
         uint32_t *hd = (k & 1) ? p.hA : p.hB;
