@@ -207,11 +207,12 @@ struct KeccakWaveLane {
     int word;         // absorb: rate word x + 5y (< 17) or -1
 };
 
+// rho offsets r[x + 5y]
+__constant__ static const uint8_t kKeccakRho[25] = {0,  1,  62, 28, 27, 36, 44, 6,  55, 20, 3,  10, 43,
+                                                    25, 39, 41, 45, 15, 21, 8,  18, 2,  61, 56, 14};
+
 __device__ __forceinline__ KeccakWaveLane keccak_wave_lane()
 {
-    // rho offsets r[x + 5y]
-    const uint8_t kRho[25] = {0, 1, 62, 28, 27, 36, 44, 6, 55, 20, 3, 10, 43, 25, 39,
-                              41, 45, 15, 21, 8, 18, 2, 61, 56, 14};
     KeccakWaveLane k;
     const uint32_t lane = __lane_id();
     const uint32_t y = lane >> 3, j = lane & 7, x = (j + 4) % 5;
@@ -220,7 +221,7 @@ __device__ __forceinline__ KeccakWaveLane keccak_wave_lane()
     // pi: destination (x', y') reads source (3 (y' - 3 x') mod 5, x')
     const uint32_t xs = (3u * ((y + 15u - 3u * x) % 5u)) % 5u, ys = x;
     k.src4 = ok ? 4u * (8u * ys + xs + 1u) : 4u * lane;
-    const uint32_t r = ok ? kRho[x + 5 * y] : 0u;
+    const uint32_t r = ok ? kKeccakRho[x + 5 * y] : 0u;
     k.rsh = (32u - (r & 31u)) & 31u;
     k.rswap = (r >= 32u) != ((r & 31u) == 0u);
     k.copy67 = j >= 6;
@@ -246,7 +247,17 @@ __device__ __forceinline__ uint32_t xor_rows(uint32_t v)
 
 __device__ __forceinline__ void keccak_f1600_wave(uint32_t &lo, uint32_t &hi, const KeccakWaveLane &k)
 {
-#pragma unroll 1
+    // round constants as literals: the 24 rounds are unrolled, so iota is one masked XOR per
+    // half on the lane of word (0, 0) with no constant-memory load
+    constexpr uint64_t kRC[24] = {
+        0x0000000000000001ULL, 0x0000000000008082ULL, 0x800000000000808aULL, 0x8000000080008000ULL,
+        0x000000000000808bULL, 0x0000000080000001ULL, 0x8000000080008081ULL, 0x8000000000008009ULL,
+        0x000000000000008aULL, 0x0000000000000088ULL, 0x0000000080008009ULL, 0x000000008000000aULL,
+        0x000000008000808bULL, 0x800000000000008bULL, 0x8000000000008089ULL, 0x8000000000008003ULL,
+        0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
+        0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
+    const uint32_t m00 = k.is00 ? 0xffffffffu : 0u;
+#pragma unroll
     for (int round = 0; round < 24; round++) {
         // theta
         const uint32_t clo = xor_rows(lo ^ dpp_ror8(lo)), chi = xor_rows(hi ^ dpp_ror8(hi));
@@ -260,16 +271,11 @@ __device__ __forceinline__ void keccak_f1600_wave(uint32_t &lo, uint32_t &hi, co
         // pi
         lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)k.src4, (int)rlo) & k.valid;
         hi = (uint32_t)__builtin_amdgcn_ds_bpermute((int)k.src4, (int)rhi) & k.valid;
-        // chi
+        // chi, then iota
         const uint32_t b1lo = dpp_from_next(lo), b1hi = dpp_from_next(hi);
         const uint32_t b2lo = dpp_from_next2(lo), b2hi = dpp_from_next2(hi);
-        lo ^= ~b1lo & b2lo;
-        hi ^= ~b1hi & b2hi;
-        // iota
-        if (k.is00) {
-            lo ^= (uint32_t)kKeccakRC[round];
-            hi ^= (uint32_t)(kKeccakRC[round] >> 32);
-        }
+        lo ^= (~b1lo & b2lo) ^ (m00 & (uint32_t)kRC[round]);
+        hi ^= (~b1hi & b2hi) ^ (m00 & (uint32_t)(kRC[round] >> 32));
         // the copies of columns 0 and 1 in slots 6 and 7 come from slots 1 and 2
         const uint32_t clo6 = dpp_from_prev5(lo), chi6 = dpp_from_prev5(hi);
         if (k.copy67) {

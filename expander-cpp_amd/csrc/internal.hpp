@@ -100,13 +100,12 @@ struct DevScratch {
     uint32_t *s0 = nullptr;       // R * maxN        (phase-2 scalar h table)
     uint32_t *sA = nullptr;       // R * maxN/2
     uint32_t *sB = nullptr;       // R * maxN/4
-    uint32_t *partials = nullptr; // R * maxBlocks * 3L
+    uint32_t *partials = nullptr; // R * 3L * maxBlocks (value-major block partials)
     uint32_t *chal = nullptr;     // D * R * 2 * 32 (rx then ry per layer and rep) + rz1 seed
     uint32_t *ab = nullptr;       // D * 2 (alpha, beta after each layer)
     uint32_t *claims = nullptr;   // R * L
     uint32_t *vxy = nullptr;      // R * L (vx) + R * L (vy)
     uint32_t *tstate = nullptr;   // transcript state: len, hash_start, digest[8]
-    uint32_t *ticket = nullptr;   // last-block counter of the multi-block reductions (zero at rest)
     uint32_t *tail = nullptr;     // transcript bytes after the commitment
     uint64_t tail_cap = 0;        // bytes
     uint32_t max_n = 0, max_m = 0, L = 0, D = 0;

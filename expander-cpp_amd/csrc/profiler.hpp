@@ -22,6 +22,7 @@ enum KernelClass : int {
     kKTailPh1,
     kKTailPh2,
     kKBatch,
+    kKRoundFin,
     kKClassCount
 };
 
@@ -30,7 +31,7 @@ inline const char *kernel_class_name(int c)
     static const char *names[kKClassCount] = {"k_forward",      "k_eq",      "k_gx",           "k_hy",
                                               "k_dot",          "k_round_ph1_first", "k_round_ph1",
                                               "k_round_ph2_first", "k_round_ph2", "k_tail_ph1",
-                                              "k_tail_ph2",     "k_prove_batch"};
+                                              "k_tail_ph2",     "k_prove_batch", "k_round_fin"};
     return (c >= 0 && c < kKClassCount) ? names[c] : "";
 }
 

@@ -197,11 +197,12 @@ __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint6
 
 // ------------------------------------------------------------------ reductions
 // Reduces the 3 x 4 lane sums (u64) of the 256 threads [t0, t0 + 256) that share
-// t % LG.  red is this group's LDS scratch [4][3L]; the result goes to out[q],
+// t % LG.  red is this group's LDS scratch [4][3L]; the result goes to out[q * ostride],
 // q = pk * L + 4 * g + c, written by threads t0 .. t0 + 3L - 1.  Every thread of the block
 // must call it (it synchronizes the block).
 template <int L>
-__device__ __forceinline__ void group_reduce(uint64_t (&acc)[3][4], uint32_t (*red)[3 * L], uint32_t *out)
+__device__ __forceinline__ void group_reduce(uint64_t (&acc)[3][4], uint32_t (*red)[3 * L], uint32_t *out,
+                                             uint32_t ostride = 1)
 {
     constexpr uint32_t LG = L / 4;
     uint32_t v[3][4];
@@ -229,7 +230,7 @@ __device__ __forceinline__ void group_reduce(uint64_t (&acc)[3][4], uint32_t (*r
         uint32_t s = 0;
 #pragma unroll
         for (int w = 0; w < 4; w++) s = m31_add(s, red[w][tg]);
-        out[tg] = s;
+        out[tg * ostride] = s;
     }
 }
 
@@ -334,36 +335,47 @@ __device__ __forceinline__ uint32_t msg_word(const uint32_t *sj, const uint32_t 
 
 // Block-wide transcript step for round k of one phase, all reps.  sums: LDS [R][3L].
 // r_out: LDS [R] receives the challenges.  Every hash is computed by a whole wavefront:
-// wave j hashes rep j when the reps are independent.  Every thread of the block must call
-// it; the block has at least 64 R threads.
+// wave j hashes rep j when the reps are independent.  The words a hash reads are staged
+// in LDS as they are produced (they also go to the proof tail), so a hash does not wait
+// on a global-memory round trip.  At most L words can be pending before a round (the last
+// x claim); hb keeps L lead words in front of every message.  Every thread of the block
+// must call it; the block has at least 64 R threads.
 template <int L>
 __device__ void transcript_step(const uint32_t *sums, const TsArgs &a, uint32_t *r_out)
 {
-    constexpr uint32_t Q = 3 * L;
+    constexpr uint32_t Q = 3 * L, S = L + Q;
     __shared__ uint32_t ts[10];
     __shared__ uint32_t dig[8];
+    __shared__ uint32_t hb[3 * S];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     if (t < 10) ts[t] = a.tstate[t];
     __syncthreads();
+    const uint32_t base = ts[0], hs0 = ts[1];
+    const uint32_t P = (base - hs0) / 4;  // unhashed words in front of this round's messages
+    const bool staged = P <= L;
+    if (t < P && staged) hb[L - P + t] = a.tail[hs0 / 4 + t];
     if (!(a.phase == 1 && a.last)) {
-        const uint32_t base = ts[0], hs0 = ts[1];
-        for (uint32_t w = t; w < a.R * Q; w += blockDim.x)
-            a.tail[base / 4 + w] = msg_word<L>(sums + (w / Q) * Q, a.vx + (w / Q) * L, a.phase, w % Q);
+        for (uint32_t w = t; w < a.R * Q; w += blockDim.x) {
+            const uint32_t j = w / Q, q = w % Q;
+            const uint32_t v = msg_word<L>(sums + j * Q, a.vx + j * L, a.phase, q);
+            a.tail[base / 4 + w] = v;
+            hb[j * S + L + q] = v;
+        }
         __syncthreads();
         if (wave < a.R) {
             const uint32_t j = wave;
-            const uint32_t start = j == 0 ? hs0 : base + j * 4 * Q;
-            const uint32_t end = base + (j + 1) * 4 * Q;
+            const uint32_t lead = j == 0 ? P : 0;
+            const uint32_t *m = staged ? hb + j * S + L - lead : a.tail + (j == 0 ? hs0 / 4 : base / 4 + j * Q);
             uint32_t d[8];
-            hash_words_wave(a.hash_kind, a.tail + start / 4, (end - start) / 4, d);
+            hash_words_wave(a.hash_kind, m, lead + Q, d);
             if (lane == 0) {
                 const uint32_t r = challenge_from(d);
                 r_out[j] = r;
                 a.chal[j * kChalStride + (a.phase == 2 ? 32u : 0u) + a.k] = r;
                 if (j == a.R - 1) {
                     for (int i = 0; i < 8; i++) ts[2 + i] = d[i];
-                    ts[0] = end;
-                    ts[1] = end;
+                    ts[0] = base + a.R * 4 * Q;
+                    ts[1] = ts[0];
                 }
             }
         }
@@ -378,11 +390,12 @@ __device__ void transcript_step(const uint32_t *sums, const TsArgs &a, uint32_t 
                 const uint32_t v = fold1(f[c], f[L + c], r_out[j]);
                 a.vy[t] = v;
                 a.tail[len / 4 + t] = v;
+                hb[t] = v;
             }
             __syncthreads();
             if (wave == 0) {
                 uint32_t d[8];
-                hash_words_wave(a.hash_kind, a.tail + len / 4, a.R * L, d);
+                hash_words_wave(a.hash_kind, hb, a.R * L, d);
                 if (lane == 0) {
                     a.ab_out[0] = challenge_from(d);
                     for (int i = 0; i < 8; i++) dig[i] = d[i];
@@ -402,15 +415,20 @@ __device__ void transcript_step(const uint32_t *sums, const TsArgs &a, uint32_t 
         }
     } else {
         // last x round, in order: rep j hashes (vx_{j-1} ||) its message, then appends vx_j
-        // (sumcheck.hpp:90-95)
-        uint32_t len = ts[0], hs = ts[1];
+        // (sumcheck.hpp:90-95).  hb[0, L) holds the lead words, hb[L, L + Q) the message.
+        uint32_t len = base, hs = hs0;
         for (uint32_t j = 0; j < a.R; j++) {
-            for (uint32_t q = t; q < Q; q += blockDim.x)
-                a.tail[len / 4 + q] = msg_word<L>(sums + j * Q, a.vx + j * L, 1, q);
+            const uint32_t lead = (len - hs) / 4;
+            for (uint32_t q = t; q < Q; q += blockDim.x) {
+                const uint32_t v = msg_word<L>(sums + j * Q, a.vx + j * L, 1, q);
+                a.tail[len / 4 + q] = v;
+                hb[L + q] = v;
+            }
             __syncthreads();
             if (wave == 0) {
+                const uint32_t *m = (lead <= L) ? hb + L - lead : a.tail + hs / 4;
                 uint32_t d[8];
-                hash_words_wave(a.hash_kind, a.tail + hs / 4, (len + 4 * Q - hs) / 4, d);
+                hash_words_wave(a.hash_kind, m, lead + Q, d);
                 if (lane == 0) {
                     const uint32_t r = challenge_from(d);
                     r_out[j] = r;
@@ -426,6 +444,7 @@ __device__ void transcript_step(const uint32_t *sums, const TsArgs &a, uint32_t 
                 const uint32_t v = fold1(f[t], f[L + t], r_out[j]);
                 a.vx[j * L + t] = v;
                 a.tail[len / 4 + t] = v;
+                hb[t] = v;
             }
             __syncthreads();
             len += 4 * L;
@@ -450,23 +469,20 @@ struct RoundParams {
     uint32_t *hdst;
     uint64_t hdst_rs;
     const uint32_t *rprev; // challenge of round k-1 for rep 0; rep stride kChalStride
-    uint32_t *partials;    // [blockIdx.x][rep][3L]
-    uint32_t *ticket;      // zero between launches
+    uint32_t *partials;    // value-major: [rep][3L][gridDim.x]
     uint32_t items;        // pairs in round k = N_k / 2
-    TsArgs ts;
 };
 
-// Round k of one sumcheck phase for every repetition (blockIdx.y = rep), on many blocks.
-// The block that finishes last (ticket) adds the block partials and runs the transcript.
+// Round k of one sumcheck phase for every repetition (blockIdx.y = rep), on many blocks:
+// the fold of round k-1 and the three sums of round k.  Each block leaves its partial
+// sums; k_fin adds them and runs the transcript.  The launch boundary between the two
+// orders the partials for k_fin, so no device-scope fence is needed here (on a multi-XCD
+// part a release fence writes back the whole L2, once per block).
 template <int L, bool PH2, bool FUSED>
 __global__ void __launch_bounds__(kThreads) k_round(RoundParams p)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
     __shared__ uint32_t red[4][3 * L];
-    __shared__ unsigned long long lsum[3 * Q];
-    __shared__ uint32_t sums[3 * Q], rr[3];
-    __shared__ bool is_last;
-    const uint32_t R = p.ts.R;
     const uint32_t rep = blockIdx.y;
     const uint32_t g = threadIdx.x % LG;
     const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
@@ -478,23 +494,55 @@ __global__ void __launch_bounds__(kThreads) k_round(RoundParams p)
     const uint64_t total = (uint64_t)p.items * LG;
     for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
         round_item<L, PH2, FUSED>(fs, fd, hs, hd, w / LG, g, r, acc);
-    group_reduce<L>(acc, red, p.partials + ((uint64_t)blockIdx.x * R + rep) * Q);
-    __threadfence();
+    group_reduce<L>(acc, red, p.partials + (uint64_t)rep * Q * gridDim.x + blockIdx.x, gridDim.x);
+}
+
+// Adds the value-major block partials [R][3L][nb] of k_round (or k_dot) into sums[R][3L]
+// (LDS): wave w takes values w, w + 16, ... (at most kFinVals of them); lanes stride over
+// the nb <= kMaxBlocksPerRep blocks.  Every load is issued before the first add, so the
+// reduction costs one memory latency rather than one per value.  blockDim is kFinThreads.
+constexpr uint32_t kFinThreads = 1024;
+template <int L>
+__device__ __forceinline__ void reduce_partials(const uint32_t *__restrict__ partials, uint32_t nb, uint32_t nvals,
+                                                uint32_t *sums)
+{
+    constexpr uint32_t W = kFinThreads / 64;
+    constexpr uint32_t kFinVals = (3 * 3 * L + W - 1) / W;  // R <= 3
+    constexpr uint32_t kBlk = kMaxBlocksPerRep / 64;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t x[kFinVals][kBlk];
+#pragma unroll
+    for (uint32_t i = 0; i < kFinVals; i++) {
+        const uint32_t v = wave + i * W;
+#pragma unroll
+        for (uint32_t c = 0; c < kBlk; c++) {
+            const uint32_t b = lane + 64 * c;
+            x[i][c] = (v < nvals && b < nb) ? partials[(uint64_t)v * nb + b] : 0u;
+        }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kFinVals; i++) {
+        uint64_t a = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kBlk; c++) a += x[i][c];
+        uint32_t y = m31_reduce64(a);
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) y = m31_add(y, __shfl_xor(y, off, 64));
+        const uint32_t v = wave + i * W;
+        if (lane == 0 && v < nvals) sums[v] = y;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(p.ticket, 1u) == gridDim.x * gridDim.y - 1;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    for (uint32_t i = threadIdx.x; i < R * Q; i += kThreads) lsum[i] = 0;
-    __syncthreads();
-    const uint32_t nvals = gridDim.x * R * Q;
-    for (uint32_t f = threadIdx.x; f < nvals; f += kThreads)
-        atomicAdd(&lsum[f % (R * Q)], (unsigned long long)p.partials[f]);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < R * Q; i += kThreads) sums[i] = m31_reduce64(lsum[i]);
-    __syncthreads();
-    transcript_step<L>(sums, p.ts, rr);
-    if (threadIdx.x == 0) *p.ticket = 0;
+}
+
+// The end of a grid round: adds the partials of k_round and runs the transcript step
+// (one workgroup of kFinThreads).
+template <int L>
+__global__ void __launch_bounds__(kFinThreads) k_fin(const uint32_t *__restrict__ partials, uint32_t nb, TsArgs ts)
+{
+    constexpr uint32_t Q = 3 * L;
+    __shared__ uint32_t sums[3 * Q], rr[3];
+    reduce_partials<L>(partials, nb, ts.R * Q, sums);
+    transcript_step<L>(sums, ts, rr);
 }
 
 // Rounds k0 .. n-1 of one phase in a single workgroup of 768 threads: the tables are
@@ -565,17 +613,110 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(TailParams p)
     phase_rounds<L, PH2>(p);
 }
 
+// Rounds k0 .. kend - 1 (all before the last round of the phase) with one workgroup of
+// kRepThreads per repetition (blockIdx.x = rep).  Before the last x round the R challenge
+// chains are independent: rep j's message of round k sits at base_k + 4 Q j, where
+// base_k = len0 + (k - k0) 4 Q R, and its challenge hashes from there to the end of the
+// message (rep 0 of round k0 also takes the unhashed bytes from hs0 on).  The host tracks
+// len0 and hs0 (the proof layout is fixed by the circuit shape), so this kernel reads no
+// transcript state; the workgroup of the last rep writes the final state.
+constexpr uint32_t kRepThreads = 1024;
+struct TailRepParams {
+    TailParams t;
+    uint32_t kend;
+    uint32_t len0, hs0;  // transcript length and hash start (bytes of the tail) at round k0
+};
+
+template <int L, bool PH2>
+__global__ void __launch_bounds__(kRepThreads) k_tail_rep(TailRepParams q)
+{
+    constexpr uint32_t LG = L / 4, Q = 3 * L, W = kRepThreads / 64;
+    const TailParams &p = q.t;
+    __shared__ uint32_t red[W][3 * L];
+    __shared__ uint32_t sums[Q], rr, hb[L + Q];
+    const uint32_t R = p.ts.R, rep = blockIdx.x;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63, g = t % LG;
+    uint32_t *chal = p.ts.chal + rep * kChalStride + (PH2 ? 32u : 0u);
+    if (t == 0) rr = p.k0 > 0 ? chal[p.k0 - 1] : 0u;
+    __syncthreads();
+    uint32_t d[8] = {};
+    for (uint32_t k = p.k0; k < q.kend; k++) {
+        const uint64_t items = 1ull << (p.n - k - 1);
+        const bool first = (k == p.k0) || (k == 1);
+        const uint32_t *fs = (first ? p.f0 : ((k - 1) & 1 ? p.fA : p.fB)) + rep * (first ? p.f0_rs : ((k - 1) & 1 ? p.fA_rs : p.fB_rs));
+        const uint32_t *hs = (first ? p.h0 : ((k - 1) & 1 ? p.hA : p.hB)) + rep * (first ? p.h0_rs : ((k - 1) & 1 ? p.hA_rs : p.hB_rs));
+        uint32_t *fd = ((k & 1) ? p.fA : p.fB) + rep * ((k & 1) ? p.fA_rs : p.fB_rs);
+        uint32_t *hd = ((k & 1) ? p.hA : p.hB) + rep * ((k & 1) ? p.hA_rs : p.hB_rs);
+        const uint32_t r = rr;
+        uint64_t acc[3][4] = {};
+        for (uint64_t w = t; w < items * LG; w += kRepThreads) {
+            if (k == 0) round_item<L, PH2, false>(fs, nullptr, hs, nullptr, w / LG, g, 0u, acc);
+            else round_item<L, PH2, true>(fs, fd, hs, hd, w / LG, g, r, acc);
+        }
+        // block reduce: lanes sharing t % LG within the wave, then the W waves
+        uint32_t v[3][4];
+#pragma unroll
+        for (int pk = 0; pk < 3; pk++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) v[pk][c] = m31_reduce64(acc[pk][c]);
+#pragma unroll
+        for (uint32_t off = LG; off < 64; off <<= 1)
+#pragma unroll
+            for (int pk = 0; pk < 3; pk++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) v[pk][c] = m31_add(v[pk][c], __shfl_xor(v[pk][c], off, 64));
+        if (lane < LG) {
+#pragma unroll
+            for (int pk = 0; pk < 3; pk++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) red[wave][pk * L + 4 * lane + c] = v[pk][c];
+        }
+        __syncthreads();
+        if (t < Q) {
+            uint32_t x = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < W; w++) x = m31_add(x, red[w][t]);
+            sums[t] = x;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // message words to the proof tail and to hb (behind `lead` pending words)
+            const uint32_t base = q.len0 + (k - p.k0) * 4 * Q * R + 4 * Q * rep;
+            const uint32_t start = (rep == 0 && k == p.k0) ? q.hs0 : base;
+            const uint32_t lead = (base - start) / 4;
+            for (uint32_t w = lane; w < Q; w += 64) {
+                const uint32_t v = msg_word<L>(sums, p.ts.vx + rep * L, PH2 ? 2 : 1, w);
+                p.ts.tail[base / 4 + w] = v;
+                hb[L + w] = v;
+            }
+            if (lane < lead && lead <= L) hb[L - lead + lane] = p.ts.tail[start / 4 + lane];
+            __threadfence_block();
+            hash_words_wave(p.ts.hash_kind, lead <= L ? hb + L - lead : p.ts.tail + start / 4, lead + Q, d);
+            if (lane == 0) {
+                const uint32_t c = challenge_from(d);
+                chal[k] = c;
+                rr = c;
+            }
+        }
+        __syncthreads();
+    }
+    if (rep == R - 1 && t == 0) {
+        const uint32_t len = q.len0 + (q.kend - p.k0) * 4 * Q * R;
+        p.ts.tstate[0] = len;
+        p.ts.tstate[1] = len;
+        for (int i = 0; i < 8; i++) p.ts.tstate[2 + i] = d[i];
+    }
+}
+
 // Dot product sum_z eq[rep][z] * vals[z] (lanes): claimed_v = eval_multilinear(output, rz1)
-// (include/LinearGKR/gkr.hpp:51-55).  The last block adds the partials.
+// (include/LinearGKR/gkr.hpp:51-55).  Each block leaves value-major partials (the p0 slot
+// of the round layout; the other slots are zero) that k_dot_fin adds.
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_dot(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ eq,
-                                                  uint64_t eq_rs, uint32_t M, uint32_t *__restrict__ partials,
-                                                  uint32_t *ticket, uint32_t R, uint32_t *claims)
+                                                  uint64_t eq_rs, uint32_t M, uint32_t *__restrict__ partials)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
     __shared__ uint32_t red[4][3 * L];
-    __shared__ unsigned long long lsum[3 * Q];
-    __shared__ bool is_last;
     const uint32_t rep = blockIdx.y;
     const uint32_t g = threadIdx.x % LG;
     const uint32_t *e = eq + rep * eq_rs;
@@ -586,23 +727,17 @@ __global__ void __launch_bounds__(kThreads) k_dot(const uint32_t *__restrict__ v
         const uint4 t = mul4s(ld4(vals + z * L + 4 * g), e[z]);
         acc[0][0] += t.x; acc[0][1] += t.y; acc[0][2] += t.z; acc[0][3] += t.w;
     }
-    group_reduce<L>(acc, red, partials + ((uint64_t)blockIdx.x * R + rep) * Q);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1u) == gridDim.x * gridDim.y - 1;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    for (uint32_t i = threadIdx.x; i < R * Q; i += kThreads) lsum[i] = 0;
-    __syncthreads();
-    for (uint32_t f = threadIdx.x; f < gridDim.x * R * Q; f += kThreads)
-        atomicAdd(&lsum[f % (R * Q)], (unsigned long long)partials[f]);
-    __syncthreads();
-    if (threadIdx.x < R * L) {
-        const uint32_t j = threadIdx.x / L, c = threadIdx.x % L;
-        claims[threadIdx.x] = m31_reduce64(lsum[j * Q + c]);
-    }
-    if (threadIdx.x == 0) *ticket = 0;
+    group_reduce<L>(acc, red, partials + (uint64_t)rep * Q * gridDim.x + blockIdx.x, gridDim.x);
+}
+
+template <int L>
+__global__ void __launch_bounds__(kFinThreads) k_dot_fin(const uint32_t *__restrict__ partials, uint32_t nb, uint32_t R,
+                                                         uint32_t *__restrict__ claims)
+{
+    constexpr uint32_t Q = 3 * L;
+    __shared__ uint32_t sums[3 * Q];
+    reduce_partials<L>(partials, nb, R * Q, sums);
+    if (threadIdx.x < R * L) claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
 }
 
 // ------------------------------------------------------------------ batched proving
@@ -831,7 +966,7 @@ int forward_impl(const DevCircuit &dc, hipStream_t st, Profiler *pf)
         const LayerDev &ly = dc.layers[l];
         const uint64_t M = 1ull << ly.m;
         const uint64_t work = M * (L / 4);
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>((work + kThreads - 1) / kThreads, 8192);
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((work + kThreads - 1) / kThreads, 65536);
         timed(pf, kKForward, st, [&] {
             hipLaunchKernelGGL(k_forward<L>, dim3(blocks), dim3(kThreads), 0, st, dc.values + ly.v_off,
                                dc.values + ly.out_off, ly.fw_ptr, ly.fw_ent, (uint32_t)M);
@@ -845,6 +980,12 @@ int forward_impl(const DevCircuit &dc, hipStream_t st, Profiler *pf)
 // scalar table stays 16-byte aligned
 inline uint64_t scratch_rows(uint64_t rows) { return std::max<uint64_t>(4, (rows + 3) & ~3ull); }
 
+// blocks of kThreads covering `work` threads, at most cap (grid-stride loops take the rest)
+uint32_t grid_cover(uint64_t work, uint64_t cap)
+{
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + kThreads - 1) / kThreads, cap));
+}
+
 uint32_t grid_for(uint64_t work)
 {
     uint64_t b = (work + kThreads - 1) / kThreads;
@@ -852,10 +993,13 @@ uint32_t grid_for(uint64_t work)
     return (uint32_t)std::min<uint64_t>(b, kMaxBlocksPerRep);
 }
 
+// ts_len, ts_hs: transcript length and hash start (bytes of the tail) when the device
+// transcript starts; the host follows them through the fixed proof layout.
 template <int L>
-int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, hipStream_t st, Profiler *pf)
+int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, uint64_t ts_len, uint64_t ts_hs,
+               hipStream_t st, Profiler *pf)
 {
-    constexpr uint32_t LG = L / 4;
+    constexpr uint32_t LG = L / 4, Q = 3 * L;
     const uint32_t D = dc.D;
     const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
     const uint64_t eqz_rs = maxM, eqx_rs = maxN;
@@ -884,11 +1028,14 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, h
             const uint32_t nbx = grid_for(M * LG);
             timed(pf, kKDot, st, [&] {
                 hipLaunchKernelGGL(k_dot<L>, dim3(nbx, R), dim3(kThreads), 0, st, dc.values + ly.out_off, s.eqz,
-                                   eqz_rs, (uint32_t)M, s.partials, s.ticket, R, s.claims);
+                                   eqz_rs, (uint32_t)M, s.partials);
+                hipLaunchKernelGGL(k_dot_fin<L>, dim3(1), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials, nbx,
+                                   R, s.claims);
             });
         }
         {
-            dim3 grid(grid_for(N * LG) * 4);
+            // one item per thread: the CSR walk is a chain of dependent gathers, hidden by occupancy
+            dim3 grid(grid_cover(N * LG, 65536));
             timed(pf, kKGx, st, [&] {
                 hipLaunchKernelGGL(k_gx<L>, grid, dim3(kThreads), 0, st, s.hg0, hg0_rs, V, s.eqz, eqz_rs, ly.gx_ptr,
                                    ly.gx_ent, (uint32_t)N, R);
@@ -901,7 +1048,7 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, h
                     hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqx, eqx_rs, (const uint32_t *)chal_l,
                                        0u, 0u, (const uint32_t *)nullptr, n);
                 });
-                dim3 g2(grid_for(N) * 4);
+                dim3 g2(grid_cover(N, 65536));
                 timed(pf, kKHy, st, [&] {
                     hipLaunchKernelGGL(k_hy, g2, dim3(kThreads), 0, st, s.s0, s0_rs, s.eqz, eqz_rs, s.eqx, eqx_rs,
                                        ly.hy_ptr, ly.hy_ent, (uint32_t)N, R);
@@ -921,10 +1068,9 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, h
                 RoundParams rp{};
                 rp.items = (uint32_t)(N >> (k + 1));
                 rp.partials = s.partials;
-                rp.ticket = s.ticket;
-                rp.ts = ts;
-                rp.ts.k = k;
-                rp.ts.last = 0;  // the last round is always in k_tail
+                TsArgs tk = ts;
+                tk.k = k;
+                tk.last = 0;  // the last round is always in k_tail
                 if (k == 0) {
                     rp.fsrc = V; rp.fsrc_rs = 0;
                     rp.hsrc = h0; rp.hsrc_rs = h0_rs;
@@ -952,25 +1098,60 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, h
                         else hipLaunchKernelGGL((k_round<L, true, true>), grid, dim3(kThreads), 0, st, rp);
                     }
                 });
+                timed(pf, kKRoundFin, st, [&] {
+                    hipLaunchKernelGGL(k_fin<L>, dim3(1), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials, nbx,
+                                       tk);
+                });
+                ts_len += (uint64_t)R * 4 * Q;
+                ts_hs = ts_len;
             }
+            // round kk's sources: table kk - 1 (table 0 = the layer input V and h0)
             TailParams tp{};
-            if (k0 == 0) {
-                tp.f0 = V; tp.f0_rs = 0; tp.h0 = h0; tp.h0_rs = h0_rs;
-            } else if (k0 == 1) {
-                tp.f0 = V; tp.f0_rs = 0; tp.h0 = h0; tp.h0_rs = h0_rs;
-            } else {
-                tp.f0 = (k0 - 1) & 1 ? s.fA : s.fB;
-                tp.f0_rs = (k0 - 1) & 1 ? fA_rs : fB_rs;
-                tp.h0 = (k0 - 1) & 1 ? hA : hB;
-                tp.h0_rs = (k0 - 1) & 1 ? hA_rs : hB_rs;
-            }
             tp.fA = s.fA; tp.fB = s.fB; tp.hA = hA; tp.hB = hB;
             tp.fA_rs = fA_rs; tp.fB_rs = fB_rs; tp.hA_rs = hA_rs; tp.hB_rs = hB_rs;
-            tp.k0 = k0; tp.n = n; tp.ts = ts;
+            tp.n = n; tp.ts = ts;
+            auto set_src = [&](TailParams &t, uint32_t kk) {
+                t.k0 = kk;
+                if (kk <= 1) {
+                    t.f0 = V; t.f0_rs = 0; t.h0 = h0; t.h0_rs = h0_rs;
+                } else {
+                    t.f0 = (kk - 1) & 1 ? s.fA : s.fB;
+                    t.f0_rs = (kk - 1) & 1 ? fA_rs : fB_rs;
+                    t.h0 = (kk - 1) & 1 ? hA : hB;
+                    t.h0_rs = (kk - 1) & 1 ? hA_rs : hB_rs;
+                }
+            };
+            const uint32_t klast = n - 1;
+            if (k0 < klast) {
+                TailRepParams q{};
+                set_src(q.t, k0);
+                q.t.fA = s.fA; q.t.fB = s.fB; q.t.hA = hA; q.t.hB = hB;
+                q.t.fA_rs = fA_rs; q.t.fB_rs = fB_rs; q.t.hA_rs = hA_rs; q.t.hB_rs = hB_rs;
+                q.t.n = n; q.t.ts = ts;
+                q.kend = klast;
+                q.len0 = (uint32_t)ts_len;
+                q.hs0 = (uint32_t)ts_hs;
+                timed(pf, phase == 1 ? kKTailPh1 : kKTailPh2, st, [&] {
+                    if (phase == 1) hipLaunchKernelGGL((k_tail_rep<L, false>), dim3(R), dim3(kRepThreads), 0, st, q);
+                    else hipLaunchKernelGGL((k_tail_rep<L, true>), dim3(R), dim3(kRepThreads), 0, st, q);
+                });
+                ts_len += (uint64_t)(klast - k0) * R * 4 * Q;
+                ts_hs = ts_len;
+            }
+            // the last round of the phase: ordered x-claim hashing (phase 1) or the layer end
+            // (phase 2), one workgroup for all reps
+            set_src(tp, klast);
             timed(pf, phase == 1 ? kKTailPh1 : kKTailPh2, st, [&] {
                 if (phase == 1) hipLaunchKernelGGL((k_tail<L, false>), dim3(1), dim3(kTailThreads), 0, st, tp);
                 else hipLaunchKernelGGL((k_tail<L, true>), dim3(1), dim3(kTailThreads), 0, st, tp);
             });
+            if (phase == 1) {
+                ts_len += (uint64_t)R * (4 * Q + 4 * L);
+                ts_hs = ts_len - 4 * L;
+            } else {
+                ts_len += (uint64_t)R * (4 * Q + 4 * L);
+                ts_hs = ts_len;
+            }
         }
     }
     GKRX_HIP_CHECK(hipGetLastError());
@@ -1014,8 +1195,8 @@ int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, co
     GKRX_HIP_CHECK(hipMemcpyAsync(s.tstate, ts, 40, hipMemcpyHostToDevice, st));
     if (tail_prefix_len)
         GKRX_HIP_CHECK(hipMemcpyAsync(s.tail, pre, tail_prefix_len, hipMemcpyHostToDevice, st));
-    if (dc.L == 8) return prove_impl<8>(dc, s, R, hash_kind, st, pf);
-    if (dc.L == 16) return prove_impl<16>(dc, s, R, hash_kind, st, pf);
+    if (dc.L == 8) return prove_impl<8>(dc, s, R, hash_kind, tstate_host[0], tstate_host[1], st, pf);
+    if (dc.L == 16) return prove_impl<16>(dc, s, R, hash_kind, tstate_host[0], tstate_host[1], st, pf);
     return GKRX_ERR_CONFIG;
 }
 
@@ -1123,7 +1304,7 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
         {&s.hgB, R * N4 * L}, {&s.fA, R * N2 * L}, {&s.fB, R * N4 * L}, {&s.s0, R * N1},
         {&s.sA, R * N2}, {&s.sB, R * N4}, {&s.partials, (uint64_t)R * kMaxBlocksPerRep * 3 * L},
         {&s.chal, (uint64_t)(dc.D + 1) * R * kChalStride}, {&s.ab, 2ull * (dc.D + 1)}, {&s.claims, R * L},
-        {&s.vxy, 2ull * R * L}, {&s.tstate, 16}, {&s.ticket, 16}, {&s.tail, (tail_bytes + 3) / 4 + 64}};
+        {&s.vxy, 2ull * R * L}, {&s.tstate, 16}, {&s.tail, (tail_bytes + 3) / 4 + 64}};
     uint64_t total = 0;
     for (auto &pc : pieces) total += (pc.u32 + 63) & ~63ull;
     GKRX_HIP_CHECK(hipMalloc(&s.pool, total * 4));
@@ -1132,7 +1313,6 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
         *pc.p = reinterpret_cast<uint32_t *>(s.pool) + at;
         at += (pc.u32 + 63) & ~63ull;
     }
-    GKRX_HIP_CHECK(hipMemset(s.ticket, 0, 16 * 4));
     s.stage_u32 = 1024;
     GKRX_HIP_CHECK(hipHostMalloc(&s.stage, s.stage_u32 * 4, hipHostMallocDefault));
     s.bytes = total * 4;
