@@ -86,6 +86,71 @@ def cpu_baseline(layers_all, L, fs, seed, n_layers):
                       f"one proof on one core, evaluate excluded as in the reference harness"}
 
 
+def batch_specs(count, seed=3):
+    """The uneven batch: proof i has its own seed (3 + i), width 2^k with k uniform in
+    [10, 18] and depth uniform in [4, 24] (drawn from a generator seeded with 3)."""
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(10, 19, size=count)
+    ds = rng.integers(4, 25, size=count)
+    return [(seed + i, int(ds[i]), int(ks[i])) for i in range(count)]
+
+
+def batch_leg(gkrx, fs, count, steps, warmup):
+    """Batched proving (one launch for the whole batch) against the plain path (gkrx_prove
+    once per proof) on the same uneven batch of M31x16 proofs."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    L = 16
+    specs = batch_specs(count)
+    ctx = gkrx.Context(gkrx.Config(fs_hash=fs, nb_parallel=L), device=0)
+    t0 = time.perf_counter()
+
+    def make(spec):
+        s, d, k = spec
+        layers = gkrx.synth_circuit(s, d, k, 700, True)
+        c = gkrx.Circuit.from_layers(ctx, layers)
+        c.set_witness(gkrx.synth_witness(s, 1 << k, L))
+        return c, sum(len(m) + len(a) for m, a in layers)
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        made = list(ex.map(make, specs))
+    circs = [c for c, _ in made]
+    gates = sum(g for _, g in made)
+    log(f"batch of {count} proofs, {gates} gates, built in {time.perf_counter() - t0:.1f}s")
+    prover = gkrx.Prover(ctx)
+    outs = [np.empty(prover.proof_size(c), dtype=np.uint8) for c in circs]
+    for _ in range(warmup):
+        prover.prove_batch(circs, outs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = prover.prove_batch(circs, outs)
+    batched_s = (time.perf_counter() - t0) / steps
+    # the plain path on the same proofs, one gkrx_prove per proof; its proofs must equal
+    # the batched ones
+    plain_lat = []
+    same = True
+    buf = np.empty(max(o.size for o in outs), dtype=np.uint8)
+    t0 = time.perf_counter()
+    for i, c in enumerate(circs):
+        t1 = time.perf_counter()
+        c.evaluate()
+        cl, pr = prover.prove(c, out=buf)
+        plain_lat.append((time.perf_counter() - t1) * 1e3)
+        same &= bool(np.array_equal(cl, res[i][0])) and pr.size == res[i][1].size and bool(np.array_equal(pr, res[i][1]))
+    plain_s = time.perf_counter() - t0
+    ok = all(gkrx.Verifier(ctx).verify(circs[i], res[i][0], res[i][1]) for i in range(0, count, max(1, count // 8)))
+    ctx.close()
+    log(f"batched {batched_s * 1e3:.1f} ms, plain {plain_s * 1e3:.1f} ms, identical={same}, verified={ok}")
+    return {"workload": f"{count} independent M31x16 proofs, width 2^k with k uniform in [10, 18], depth uniform "
+                        f"in [4, 24], 70% add / 30% mul, locality wiring, seed 3 + proof id",
+            "proofs": count, "gates": gates,
+            "batched_gates_per_s": round(gates / batched_s, 1), "batched_ms": round(batched_s * 1e3, 3),
+            "plain_gates_per_s": round(gates / plain_s, 1), "plain_ms": round(plain_s * 1e3, 3),
+            "ratio": round(plain_s / batched_s, 3),
+            "plain_latency_ms": {"p50": round(percentile(plain_lat, 50), 3), "p99": round(percentile(plain_lat, 99), 3)},
+            "identical_to_plain": same, "verified_sample": ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +161,8 @@ def main():
     ap.add_argument("--hash", choices=["keccak256", "sha256"], default="keccak256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-layers", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=512, help="proofs in the uneven batch leg (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (proofs shard by proof id "
@@ -150,6 +217,10 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_round<16,phase1,fused>", "avg_launch_us": round(avg_ms * 1e3, 2),
                 "launches": st["count"]}
+    ctx.close()
+    batch = None
+    if args.batch > 0:
+        batch = batch_leg(gkrx, fs, args.batch, args.batch_steps, 1)
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -171,6 +242,7 @@ def main():
         "breakdown_ms": {k2: round(float(np.mean([p[k2] for p in parts])), 3) for k2 in parts[0]},
         "kernels_ms_per_proof": {k2: round(v["ms"] / args.steps, 3) for k2, v in sorted(stats.items())},
         "verified": bool(ok),
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

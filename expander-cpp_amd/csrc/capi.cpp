@@ -12,6 +12,7 @@
 #include <thread>
 #include <vector>
 
+#include "bn254.hpp"
 #include "gkrx.h"
 #include "host_hash.hpp"
 #include "internal.hpp"
@@ -21,9 +22,12 @@ struct gkrx_ctx {
     gkrx_config cfg;
     int device = 0;
     uint32_t R = 3;
-    uint32_t L = 8;
+    uint32_t L = 8;       // lanes of the field element
+    uint32_t W = 8;       // u32 words per element: L for M31, 8 for BN254
+    bool bn254 = false;
     hipStream_t stream = nullptr;
     gkrx::DevScratch scratch;
+    gkrx::BnScratch bn;
     gkrx::ProveTimings last;
     gkrx::Profiler prof;
     gkrx::BatchScratch batch;
@@ -86,10 +90,13 @@ extern "C" int gkrx_config_num_repetitions(const gkrx_config *cfg)
 extern "C" int gkrx_ctx_create(const gkrx_config *cfg, int device, gkrx_ctx **out)
 {
     if (!cfg || !out) return GKRX_ERR_ARG;
-    // Prover::Prover asserts M31 / Raw (LinearGKR.hpp:44-46); both transcript hashes run here.
-    if (cfg->field_type != GKRX_FIELD_M31 || cfg->pc_type != GKRX_PC_RAW ||
+    // Prover::Prover asserts M31 / Raw (LinearGKR.hpp:44-46); both transcript hashes run
+    // here, and so does the BN254 scalar field (one lane) that the Config enum names.
+    const bool bn = cfg->field_type == GKRX_FIELD_BN254;
+    if ((cfg->field_type != GKRX_FIELD_M31 && !bn) || cfg->pc_type != GKRX_PC_RAW ||
         (cfg->fs_hash != GKRX_FS_SHA256 && cfg->fs_hash != GKRX_FS_KECCAK256) ||
-        (cfg->nb_parallel != 8 && cfg->nb_parallel != 16) || cfg->grinding_bits < 0 || cfg->grinding_bits > 24) {
+        (!bn && cfg->nb_parallel != 8 && cfg->nb_parallel != 16) || cfg->grinding_bits < 0 ||
+        cfg->grinding_bits > 24) {
         set_error("unsupported config");
         return GKRX_ERR_CONFIG;
     }
@@ -109,7 +116,9 @@ extern "C" int gkrx_ctx_create(const gkrx_config *cfg, int device, gkrx_ctx **ou
     c->cfg = *cfg;
     c->device = device;
     c->R = (uint32_t)R;
-    c->L = (uint32_t)cfg->nb_parallel;
+    c->bn254 = bn;
+    c->L = bn ? 1u : (uint32_t)cfg->nb_parallel;
+    c->W = bn ? 8u : c->L;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         set_error("hipStreamCreate failed");
@@ -130,6 +139,7 @@ extern "C" int gkrx_ctx_destroy(gkrx_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     dev_free_scratch(ctx->scratch);
     dev_free_batch(ctx->batch);
+    dev_bn_free_scratch(ctx->bn);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GKRX_OK;
@@ -154,7 +164,9 @@ static bool canon_coef(uint32_t raw, uint32_t &out)
     return true;
 }
 
-static int finish_circuit(gkrx_ctx *ctx, gkrx_circuit *c)
+// Shape (_compute_nb_vars, circuit.hpp:124-149) and validation of the gate lists; M31
+// coefficients are canonicalized.  No device work.
+static int validate_circuit(bool bn254, gkrx_circuit *c)
 {
     const uint32_t D = c->D;
     c->n.assign(D, 0);
@@ -184,23 +196,32 @@ static int finish_circuit(gkrx_ctx *ctx, gkrx_circuit *c)
             set_error("layer " + std::to_string(l) + ": output variables differ from the next layer's inputs");
             return GKRX_ERR_CIRCUIT;
         }
+        // BN254 keeps the coefficient as read (every u32 is below r)
         auto &mu = c->mul[l];
         for (size_t g = 0; g < mu.size() / 4; g++) {
-            if ((mu[4 * g] >> n) || (mu[4 * g + 1] >> n) || (mu[4 * g + 2] >> m) || !canon_coef(mu[4 * g + 3], mu[4 * g + 3])) {
+            if ((mu[4 * g] >> n) || (mu[4 * g + 1] >> n) || (mu[4 * g + 2] >> m) ||
+                (!bn254 && !canon_coef(mu[4 * g + 3], mu[4 * g + 3]))) {
                 set_error("layer " + std::to_string(l) + ": mul gate index or coefficient out of range");
                 return GKRX_ERR_CIRCUIT;
             }
         }
         auto &ad = c->add[l];
         for (size_t g = 0; g < ad.size() / 3; g++) {
-            if ((ad[3 * g] >> n) || (ad[3 * g + 1] >> m) || !canon_coef(ad[3 * g + 2], ad[3 * g + 2])) {
+            if ((ad[3 * g] >> n) || (ad[3 * g + 1] >> m) || (!bn254 && !canon_coef(ad[3 * g + 2], ad[3 * g + 2]))) {
                 set_error("layer " + std::to_string(l) + ": add gate index or coefficient out of range");
                 return GKRX_ERR_CIRCUIT;
             }
         }
     }
+    return GKRX_OK;
+}
+
+static int finish_circuit(gkrx_ctx *ctx, gkrx_circuit *c)
+{
+    int rc = validate_circuit(ctx->bn254, c);
+    if (rc) return rc;
     GKRX_HIP_CHECK(hipSetDevice(ctx->device));
-    int rc = dev_upload_circuit(c->dev, ctx->L, c->n, c->m, c->mul, c->add);
+    rc = dev_upload_circuit(c->dev, ctx->W, c->n, c->m, c->mul, c->add);
     if (rc) return rc;
     c->ctx = ctx;
     ctx->live_circuits++;
@@ -359,18 +380,32 @@ extern "C" int gkrx_circuit_set_witness(gkrx_circuit *c, const uint32_t *lanes, 
     if (!c || !lanes) return GKRX_ERR_ARG;
     const size_t need = ((size_t)1 << c->n[0]) * c->dev.L;
     if (n_u32 != need) {
-        set_error("witness must hold 2^nb_input_vars(0) elements of L lanes");
+        set_error("witness must hold 2^nb_input_vars(0) field elements");
         return GKRX_ERR_ARG;
     }
-    for (size_t i = 0; i < n_u32; i++)
-        if (lanes[i] >= 0x7fffffffu) {
-            set_error("witness values must be canonical M31 (< 2^31 - 1)");
-            return GKRX_ERR_ARG;
-        }
+    const bool bn = c->ctx->bn254;
+    if (bn) {
+        const uint32_t r[8] = GKRX_BN_R;
+        for (size_t i = 0; i < n_u32; i += 8)
+            if (fr_geq(lanes + i, r)) {
+                set_error("witness values must be canonical BN254 scalars (< r)");
+                return GKRX_ERR_ARG;
+            }
+    } else {
+        for (size_t i = 0; i < n_u32; i++)
+            if (lanes[i] >= 0x7fffffffu) {
+                set_error("witness values must be canonical M31 (< 2^31 - 1)");
+                return GKRX_ERR_ARG;
+            }
+    }
     std::lock_guard<std::mutex> lk(c->ctx->mu);
     GKRX_HIP_CHECK(hipSetDevice(c->ctx->device));
     c->witness.assign(lanes, lanes + n_u32);
     GKRX_HIP_CHECK(hipMemcpyAsync(c->dev.values, c->witness.data(), n_u32 * 4, hipMemcpyHostToDevice, c->ctx->stream));
+    if (bn) {
+        int rc = dev_bn_to_mont(c->dev.values, n_u32 / 8, c->ctx->stream);
+        if (rc) return rc;
+    }
     GKRX_HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
     c->has_witness = true;
     c->evaluated = false;
@@ -384,7 +419,8 @@ static int evaluate_locked(gkrx_circuit *c)
         return GKRX_ERR_STATE;
     }
     gkrx_ctx *ctx = c->ctx;
-    int rc = dev_forward(c->dev, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr);
+    Profiler *pf = ctx->prof.on ? &ctx->prof : nullptr;
+    int rc = ctx->bn254 ? dev_bn_forward(c->dev, ctx->stream, pf) : dev_forward(c->dev, ctx->stream, pf);
     if (rc) return rc;
     c->evaluated = true;
     return GKRX_OK;
@@ -413,6 +449,15 @@ extern "C" int gkrx_circuit_get_layer_values(gkrx_circuit *c, uint32_t layer, ui
     GKRX_HIP_CHECK(hipSetDevice(c->ctx->device));
     GKRX_HIP_CHECK(hipMemcpyAsync(out, c->dev.values + off, need * 4, hipMemcpyDeviceToHost, c->ctx->stream));
     GKRX_HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+    if (c->ctx->bn254) {
+        // the device keeps Montgomery form; callers get canonical values
+        for (size_t i = 0; i < need; i += 8) {
+            Fr v;
+            memcpy(v.w, out + i, 32);
+            v = fr_from_mont(v);
+            memcpy(out + i, v.w, 32);
+        }
+    }
     return GKRX_OK;
 }
 
@@ -438,7 +483,7 @@ extern "C" int gkrx_circuit_destroy(gkrx_circuit *c)
 
 static size_t tail_bytes(const gkrx_ctx *ctx, const gkrx_circuit *c)
 {
-    const size_t S = (size_t)ctx->L * 4;
+    const size_t S = (size_t)ctx->W * 4;
     size_t t = 32;
     for (uint32_t l = 0; l < c->D; l++) t += ((size_t)6 * c->n[l] * ctx->R + 2 * ctx->R) * S;
     return t;
@@ -447,7 +492,48 @@ static size_t tail_bytes(const gkrx_ctx *ctx, const gkrx_circuit *c)
 extern "C" size_t gkrx_proof_size(const gkrx_ctx *ctx, const gkrx_circuit *c)
 {
     if (!ctx || !c) return 0;
-    return ((size_t)1 << c->n[0]) * ctx->L * 4 + tail_bytes(ctx, c);
+    return ((size_t)1 << c->n[0]) * ctx->W * 4 + tail_bytes(ctx, c);
+}
+
+// gkrx_prove over BN254 (ctx->mu held, device set)
+static int prove_bn254_locked(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, uint32_t *claimed_v)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t tb = tail_bytes(ctx, c);
+    int rc = dev_bn_ensure_scratch(ctx->bn, c->dev, tb);
+    if (rc) return rc;
+    if (!c->evaluated) {
+        rc = evaluate_locked(c);  // asynchronous: overlaps the host transcript prefix below
+        if (rc) return rc;
+    }
+    const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
+    const size_t commit_len = ((size_t)1 << c->n[0]) * 32;
+    const uint32_t mlast = c->m[c->D - 1];
+    std::vector<uint32_t> rz1((size_t)8 * mlast);
+    uint32_t tstate[10];
+    uint8_t grind[32];
+    host_transcript_prefix_bn254(kind, reinterpret_cast<const uint8_t *>(c->witness.data()), commit_len, mlast,
+                                 (uint32_t)ctx->cfg.grinding_bits, grind, rz1.data(), tstate);
+    const auto t1 = std::chrono::steady_clock::now();
+    Profiler *pf = ctx->prof.on ? &ctx->prof : nullptr;
+    rc = dev_bn_prove(c->dev, ctx->bn, kind, rz1.data(), tstate, grind, 32, ctx->stream, pf);
+    if (rc) return rc;
+    uint32_t tfinal[10];
+    memcpy(proof, c->witness.data(), commit_len);  // RAW commitment bytes (raw.hpp:19-27)
+    GKRX_HIP_CHECK(hipMemcpyAsync(proof + commit_len, ctx->bn.tail, tb, hipMemcpyDeviceToHost, ctx->stream));
+    GKRX_HIP_CHECK(hipMemcpyAsync(claimed_v, ctx->bn.claims, 32, hipMemcpyDeviceToHost, ctx->stream));
+    GKRX_HIP_CHECK(hipMemcpyAsync(tfinal, ctx->bn.tstate, sizeof tfinal, hipMemcpyDeviceToHost, ctx->stream));
+    GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto t2 = std::chrono::steady_clock::now();
+    if (ctx->prof.on) ctx->prof.collect();
+    if (tfinal[0] != tb) {
+        set_error("device transcript length mismatch: " + std::to_string(tfinal[0]) + " != " + std::to_string(tb));
+        return GKRX_ERR_DEVICE;
+    }
+    ctx->last.host_prefix_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    return GKRX_OK;
 }
 
 extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap, size_t *proof_len,
@@ -460,6 +546,12 @@ extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t
         return GKRX_ERR_ARG;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->bn254) {
+        GKRX_HIP_CHECK(hipSetDevice(ctx->device));
+        int rc = prove_bn254_locked(ctx, c, proof, claimed_v);
+        if (rc == GKRX_OK) *proof_len = total;
+        return rc;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     GKRX_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t tb = tail_bytes(ctx, c);
@@ -517,12 +609,20 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
             set_error("batch item " + std::to_string(i) + ": proof buffer too small");
             return GKRX_ERR_ARG;
         }
-        if (!dev_batch_supported(c->dev)) {
+        if (!ctx->bn254 && !dev_batch_supported(c->dev)) {
             set_error("batch item " + std::to_string(i) + ": layers wider than 2^20 (prove it with gkrx_prove)");
             return GKRX_ERR_CIRCUIT;
         }
     }
     if (n == 0) return GKRX_OK;
+    if (ctx->bn254) {
+        // BN254 proofs are compute-bound multi-block launches; a batch runs them in turn
+        for (uint32_t i = 0; i < n; i++) {
+            int rc = gkrx_prove(ctx, circuits[i], proofs[i], proof_caps[i], &proof_lens[i], claimed_v[i]);
+            if (rc) return rc;
+        }
+        return GKRX_OK;
+    }
     std::lock_guard<std::mutex> lk(ctx->mu);
     const auto t0 = std::chrono::steady_clock::now();
     GKRX_HIP_CHECK(hipSetDevice(ctx->device));
@@ -589,8 +689,39 @@ extern "C" int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t 
 {
     if (!ctx || !c || !claimed_v || !proof) return GKRX_ERR_ARG;
     HostCircuitView v{c->D, c->n.data(), c->m.data(), c->mul, c->add};
-    return host_verify(v, ctx->L, ctx->R, ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0,
-                       (uint32_t)ctx->cfg.grinding_bits, claimed_v, proof, proof_len);
+    const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
+    if (ctx->bn254) return host_verify_bn254(v, kind, (uint32_t)ctx->cfg.grinding_bits, claimed_v, proof, proof_len);
+    return host_verify(v, ctx->L, ctx->R, kind, (uint32_t)ctx->cfg.grinding_bits, claimed_v, proof, proof_len);
+}
+
+extern "C" int gkrx_verify_arrays(const gkrx_config *cfg, uint32_t nb_layers, const uint32_t *n_mul,
+                                  const uint32_t *const *mul, const uint32_t *n_add, const uint32_t *const *add,
+                                  const uint32_t *claimed_v, const uint8_t *proof, size_t proof_len)
+{
+    if (!cfg || nb_layers == 0 || !n_mul || !mul || !n_add || !add || !claimed_v || !proof) return GKRX_ERR_ARG;
+    const bool bn = cfg->field_type == GKRX_FIELD_BN254;
+    if ((cfg->field_type != GKRX_FIELD_M31 && !bn) || (cfg->fs_hash != GKRX_FS_SHA256 && cfg->fs_hash != GKRX_FS_KECCAK256) ||
+        (!bn && cfg->nb_parallel != 8 && cfg->nb_parallel != 16) || cfg->grinding_bits < 0 || cfg->grinding_bits > 24) {
+        set_error("unsupported config");
+        return GKRX_ERR_CONFIG;
+    }
+    gkrx_circuit c;
+    c.D = nb_layers;
+    c.mul.resize(nb_layers);
+    c.add.resize(nb_layers);
+    for (uint32_t l = 0; l < nb_layers; l++) {
+        c.mul[l].assign(mul[l], mul[l] + (size_t)4 * n_mul[l]);
+        c.add[l].assign(add[l], add[l] + (size_t)3 * n_add[l]);
+    }
+    int rc = validate_circuit(bn, &c);
+    if (rc) return rc;
+    HostCircuitView v{c.D, c.n.data(), c.m.data(), c.mul, c.add};
+    const int kind = cfg->fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
+    if (bn) return host_verify_bn254(v, kind, (uint32_t)cfg->grinding_bits, claimed_v, proof, proof_len);
+    const int R = gkrx_config_num_repetitions(cfg);
+    if (R < 1 || R > 3) return GKRX_ERR_CONFIG;
+    return host_verify(v, (uint32_t)cfg->nb_parallel, (uint32_t)R, kind, (uint32_t)cfg->grinding_bits, claimed_v,
+                       proof, proof_len);
 }
 
 extern "C" int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3)

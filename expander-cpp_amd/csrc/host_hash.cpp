@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 
+#include "bn254.hpp"
 #include "host_hash.hpp"
 
 namespace gkrx {
@@ -250,6 +251,41 @@ void host_transcript_prefix(int kind, const uint8_t *commit, size_t commit_len, 
             rz1[(size_t)j * mlast + i] = digest_to_m31(digest);
         }
     // transcript state relative to the tail that starts at the grind bytes
+    tstate[0] = 32;
+    tstate[1] = pending ? 0 : 32;
+    memcpy(&tstate[2], digest, 32);
+}
+
+// the digest as a 256-bit little-endian integer reduced mod r
+static void digest_to_fr(const uint8_t d[32], uint32_t out[8])
+{
+    uint32_t w[8];
+    memcpy(w, d, 32);
+    const Fr v = fr_reduce256(w);
+    memcpy(out, v.w, 32);
+}
+
+void host_transcript_prefix_bn254(int kind, const uint8_t *commit, size_t commit_len, uint32_t mlast,
+                                  uint32_t grinding_bits, uint8_t grind_out[32], uint32_t *rz1, uint32_t tstate[10])
+{
+    uint8_t digest[32];
+    uint8_t hb[32];
+    host_hash(kind, commit, commit_len, digest);
+    uint32_t h[8];
+    digest_to_fr(digest, h);
+    memcpy(hb, h, 32);
+    for (uint32_t i = 0; i < (1u << grinding_bits); i++) host_hash(kind, hb, 32, hb);
+    memcpy(grind_out, hb, 32);
+    bool pending = true;
+    for (uint32_t i = 0; i < mlast; i++) {
+        if (pending) {
+            host_hash(kind, hb, 32, digest);
+            pending = false;
+        } else {
+            host_hash(kind, digest, 32, digest);
+        }
+        digest_to_fr(digest, rz1 + 8 * (size_t)i);
+    }
     tstate[0] = 32;
     tstate[1] = pending ? 0 : 32;
     memcpy(&tstate[2], digest, 32);

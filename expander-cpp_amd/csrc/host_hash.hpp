@@ -19,5 +19,10 @@ bool host_sha_extensions();
 // [0] bytes in the tail, [1] start of the unhashed suffix, [2..9] digest.
 void host_transcript_prefix(int kind, const uint8_t *commit, size_t commit_len, uint32_t R, uint32_t mlast,
                             uint32_t grinding_bits, uint8_t grind_out[32], uint32_t *rz1, uint32_t tstate[10]);
+// The same over the BN254 scalar field (one repetition): grinding draws 256 / 254 = 1
+// challenge, written as its 32 canonical bytes; rz1 receives mlast challenges as canonical
+// values, 8 LE u32 words each.
+void host_transcript_prefix_bn254(int kind, const uint8_t *commit, size_t commit_len, uint32_t mlast,
+                                  uint32_t grinding_bits, uint8_t grind_out[32], uint32_t *rz1, uint32_t tstate[10]);
 
 }  // namespace gkrx

@@ -114,6 +114,30 @@ struct DevScratch {
     uint64_t stage_u32 = 0;
 };
 
+// Device scratch of the BN254 prover (one lane, one repetition; every table entry is one
+// Fr of 8 u32 limbs in Montgomery form).
+struct BnScratch {
+    void *pool = nullptr;
+    uint32_t *eqz = nullptr;      // maxM Fr
+    uint32_t *eqx = nullptr;      // maxN Fr
+    uint32_t *h0 = nullptr;       // maxN Fr: phase-1 hg, then the phase-2 table s
+    uint32_t *hA = nullptr;       // maxN/2 Fr (tables of odd rounds)
+    uint32_t *hB = nullptr;       // maxN/4 Fr (tables of even rounds >= 2)
+    uint32_t *fA = nullptr;
+    uint32_t *fB = nullptr;
+    uint32_t *partials = nullptr; // [block][3] Fr
+    uint32_t *chal = nullptr;     // (D + 1) slots of 64 Fr: rx then ry
+    uint32_t *ab = nullptr;       // (D + 1) x (alpha, beta)
+    uint32_t *claims = nullptr;   // canonical claimed_v
+    uint32_t *vxy = nullptr;      // vx, vy
+    uint32_t *tstate = nullptr;   // transcript state: len, hash_start, digest[8]
+    uint32_t *tail = nullptr;     // transcript bytes after the commitment
+    uint64_t tail_cap = 0;
+    uint32_t max_n = 0, max_m = 0, D = 0;
+    uint32_t *stage = nullptr;    // pinned staging of the seeds
+    uint64_t stage_u32 = 0;
+};
+
 struct ProveTimings {
     double host_prefix_ms = 0, device_ms = 0, total_ms = 0;
 };
@@ -139,5 +163,16 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
 // Whether a circuit fits the batch kernel (every layer at most 2^20 wide).
 int dev_batch_supported(const DevCircuit &dc);
 void dev_free_batch(BatchScratch &bs);
+
+// BN254 entry points (prover_bn254.hip).  A BN254 DevCircuit is uploaded with L = 8: one
+// entry is 8 u32 words.
+int dev_bn_to_mont(uint32_t *values, uint64_t n, hipStream_t st);
+int dev_bn_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf);
+int dev_bn_ensure_scratch(BnScratch &s, const DevCircuit &dc, uint64_t tail_bytes);
+void dev_bn_free_scratch(BnScratch &s);
+// rz1_host: m_last canonical challenges (8 words each); claims land in s.claims.
+int dev_bn_prove(const DevCircuit &dc, BnScratch &s, int hash_kind, const uint32_t *rz1_host,
+                 const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,
+                 Profiler *pf);
 
 }  // namespace gkrx

@@ -91,6 +91,36 @@ extern "C" int gkrx_synth_layer(uint64_t seed, uint32_t layer, uint32_t log_widt
     return GKRX_OK;
 }
 
+// BN254 scalars: four 64-bit draws masked to 254 bits (little-endian limbs), rejected
+// while >= r; each element is 8 little-endian u32 words.
+extern "C" int gkrx_synth_witness_bn254(uint64_t seed, uint64_t n_elems, uint32_t *out)
+{
+    if (!out) return GKRX_ERR_ARG;
+    static const uint64_t kR[4] = {0x43e1f593f0000001ULL, 0x2833e84879b97091ULL, 0xb85045b68181585dULL,
+                                   0x30644e72e131a029ULL};
+    Xoshiro256ss rng(seed ^ kWitnessMix);
+    for (uint64_t i = 0; i < n_elems; i++) {
+        uint64_t v[4];
+        for (;;) {
+            for (int k = 0; k < 4; k++) v[k] = rng.next();
+            v[3] &= (1ULL << 62) - 1;
+            bool below = false;
+            for (int k = 3; k >= 0; k--) {
+                if (v[k] != kR[k]) {
+                    below = v[k] < kR[k];
+                    break;
+                }
+            }
+            if (below) break;
+        }
+        for (int k = 0; k < 4; k++) {
+            out[8 * i + 2 * k] = (uint32_t)v[k];
+            out[8 * i + 2 * k + 1] = (uint32_t)(v[k] >> 32);
+        }
+    }
+    return GKRX_OK;
+}
+
 extern "C" int gkrx_synth_witness_m31(uint64_t seed, uint64_t n_elems, uint32_t lanes, uint32_t *out)
 {
     if (!out || lanes == 0) return GKRX_ERR_ARG;

@@ -18,5 +18,9 @@ struct HostCircuitView {
 // 1 = accepted, 0 = rejected.
 int host_verify(const HostCircuitView &c, uint32_t L, uint32_t R, int kind, uint32_t grinding_bits,
                 const uint32_t *claimed, const uint8_t *proof, size_t proof_len);
+// The same over the BN254 scalar field (one lane, one repetition); claimed is the
+// canonical value as 8 little-endian u32 words.
+int host_verify_bn254(const HostCircuitView &c, int kind, uint32_t grinding_bits, const uint32_t *claimed,
+                      const uint8_t *proof, size_t proof_len);
 
 }  // namespace gkrx

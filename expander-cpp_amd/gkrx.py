@@ -65,10 +65,13 @@ def lib():
     L.gkrx_proof_size.restype = C.c_size_t
     L.gkrx_prove.argtypes = [vp, vp, _u8p, C.c_size_t, C.POINTER(C.c_size_t), _u32p]
     L.gkrx_verify.argtypes = [vp, vp, _u32p, _u8p, C.c_size_t]
+    L.gkrx_verify_arrays.argtypes = [C.POINTER(_Config), C.c_uint32, _u32p, C.POINTER(C.c_void_p), _u32p,
+                                     C.POINTER(C.c_void_p), _u32p, _u8p, C.c_size_t]
     L.gkrx_last_timing.argtypes = [vp, C.POINTER(C.c_double)]
     L.gkrx_synth_layer.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u32p,
                                    C.POINTER(C.c_uint32), _u32p, C.POINTER(C.c_uint32)]
     L.gkrx_synth_witness_m31.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _u32p]
+    L.gkrx_synth_witness_bn254.argtypes = [C.c_uint64, C.c_uint64, _u32p]
     L.gkrx_hash.argtypes = [C.c_int, C.c_char_p, C.c_size_t, C.c_char_p]
     L.gkrx_prove_batch.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]
@@ -106,6 +109,19 @@ def synth_witness(seed: int, n_elems: int, lanes: int) -> np.ndarray:
     out = np.empty(n_elems * lanes, dtype=np.uint32)
     _check(lib().gkrx_synth_witness_m31(seed, n_elems, lanes, out), "gkrx_synth_witness_m31")
     return out
+
+
+def synth_witness_bn254(seed: int, n_elems: int) -> np.ndarray:
+    """n_elems BN254 scalars as a flat uint32 array (8 LE words per element)."""
+    out = np.empty(n_elems * 8, dtype=np.uint32)
+    _check(lib().gkrx_synth_witness_bn254(seed, n_elems, out), "gkrx_synth_witness_bn254")
+    return out
+
+
+def bn254_to_ints(words: np.ndarray) -> list:
+    """Flat uint32 words (8 per element) -> Python integers."""
+    w = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1, 8)
+    return [int.from_bytes(row.tobytes(), "little") for row in w]
 
 
 def write_text(layers, mul_path: str, add_path: str) -> None:
@@ -155,7 +171,13 @@ class Config:
 
     @property
     def lanes(self) -> int:
-        return self._c.nb_parallel
+        """Data-parallel lanes of a field element (BN254: one)."""
+        return 1 if self._c.field_type == FIELD_BN254 else self._c.nb_parallel
+
+    @property
+    def words(self) -> int:
+        """u32 words of one field element: the lanes for M31, 8 for a BN254 scalar."""
+        return 8 if self._c.field_type == FIELD_BN254 else self._c.nb_parallel
 
     def get_num_repetitions(self) -> int:
         return _check(lib().gkrx_config_num_repetitions(C.byref(self._c)), "num_repetitions")
@@ -231,7 +253,7 @@ class Circuit:
         _check(lib().gkrx_circuit_evaluate(self._h), "gkrx_circuit_evaluate")
 
     def layer_values(self, layer: int) -> np.ndarray:
-        L = self.ctx.config.lanes
+        L = self.ctx.config.words
         vars_ = self.nb_input_vars[layer] if layer < self.nb_layers else self.nb_output_vars[-1]
         out = np.empty((1 << vars_) * L, dtype=np.uint32)
         _check(lib().gkrx_circuit_get_layer_values(self._h, layer, out, out.size), "get_layer_values")
@@ -268,7 +290,7 @@ class Prover:
         size = self.proof_size(circuit)
         buf = out if out is not None else np.empty(size, dtype=np.uint8)
         R = self.ctx.config.get_num_repetitions()
-        claimed = np.empty(R * self.ctx.config.lanes, dtype=np.uint32)
+        claimed = np.empty(R * self.ctx.config.words, dtype=np.uint32)
         n = C.c_size_t()
         _check(lib().gkrx_prove(self.ctx._h, circuit._h, buf, buf.size, C.byref(n), claimed), "gkrx_prove")
         return claimed.reshape(R, -1), buf[:n.value]
@@ -277,7 +299,7 @@ class Prover:
         """All circuits in one batched launch; returns [(claimed[R, L], proof)] in order.
         Each proof is byte-identical to prove() of the same circuit."""
         n = len(circuits)
-        R, L = self.ctx.config.get_num_repetitions(), self.ctx.config.lanes
+        R, L = self.ctx.config.get_num_repetitions(), self.ctx.config.words
         bufs = outs if outs is not None else [np.empty(self.proof_size(c), dtype=np.uint8) for c in circuits]
         claims = [np.empty(R * L, dtype=np.uint32) for _ in circuits]
         hs = (C.c_void_p * n)(*[c._h.value for c in circuits])
@@ -305,3 +327,18 @@ class Verifier:
         pb = np.frombuffer(bytes(proof), dtype=np.uint8) if not isinstance(proof, np.ndarray) else proof
         pb = np.ascontiguousarray(pb, dtype=np.uint8)
         return _check(lib().gkrx_verify(self.ctx._h, circuit._h, cv, pb, pb.size), "gkrx_verify") == 1
+
+
+def verify_layers(config: Config, layers, claimed_v, proof) -> bool:
+    """Verifier::verify on gate arrays, with no context and no GPU (gkrx_verify_arrays)."""
+    D = len(layers)
+    muls = [np.ascontiguousarray(m, dtype=np.uint32).ravel() for m, _ in layers]
+    adds = [np.ascontiguousarray(a, dtype=np.uint32).ravel() for _, a in layers]
+    nm = np.array([len(m) // 4 for m in muls], dtype=np.uint32)
+    na = np.array([len(a) // 3 for a in adds], dtype=np.uint32)
+    mp = (C.c_void_p * D)(*[m.ctypes.data for m in muls])
+    ap = (C.c_void_p * D)(*[a.ctypes.data for a in adds])
+    cv = np.ascontiguousarray(claimed_v, dtype=np.uint32).ravel()
+    pb = np.ascontiguousarray(np.frombuffer(bytes(proof), dtype=np.uint8))
+    return _check(lib().gkrx_verify_arrays(C.byref(config._c), D, nm, mp, na, ap, cv, pb, pb.size),
+                  "gkrx_verify_arrays") == 1

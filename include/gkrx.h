@@ -104,6 +104,12 @@ int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, u
  * 0 = rejected, negative = error. */
 int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v, const uint8_t *proof,
                 size_t proof_len);
+/* The same Verifier::verify with no context and no device: the circuit comes as gate
+ * arrays (as in gkrx_circuit_from_arrays) and the check runs on the calling host thread,
+ * as the reference verifier does.  Returns 1 accepted, 0 rejected, < 0 on error. */
+int gkrx_verify_arrays(const gkrx_config *cfg, uint32_t nb_layers, const uint32_t *n_mul, const uint32_t *const *mul,
+                       const uint32_t *n_add, const uint32_t *const *add, const uint32_t *claimed_v,
+                       const uint8_t *proof, size_t proof_len);
 
 /* Millisecond split of the last gkrx_prove on this context: [0] host transcript prefix
  * (commitment hash + grinding + output challenges), [1] device work, [2] total. */
@@ -129,6 +135,10 @@ int gkrx_synth_layer(uint64_t seed, uint32_t layer, uint32_t log_width, uint32_t
                      uint32_t *mul_out, uint32_t *n_mul, uint32_t *add_out, uint32_t *n_add);
 /* n_elems x lanes M31 values, uniform in [0, p) by rejection. */
 int gkrx_synth_witness_m31(uint64_t seed, uint64_t n_elems, uint32_t lanes, uint32_t *out);
+/* n_elems BN254 scalars (8 LE u32 words each), uniform in [0, r): four 64-bit draws
+ * masked to 254 bits, rejected while >= r.  No reference counterpart (the reference has
+ * no BN254 field). */
+int gkrx_synth_witness_bn254(uint64_t seed, uint64_t n_elems, uint32_t *out);
 
 /* Transcript hashes of the library (host implementations), for tests. */
 int gkrx_hash(int fs_hash, const uint8_t *in, size_t len, uint8_t out[32]);

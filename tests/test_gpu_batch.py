@@ -67,6 +67,21 @@ def test_batch_matches_oracle_small(ctx, tmp_path):
         c.close()
 
 
+def test_batch_wide_layers_equal_single(ctx):
+    """Widths up to 2^18 (the largest of the uneven-batch workload) in one batch."""
+    print("[gpu] batch of wide proofs (2^16 .. 2^18) vs one at a time", flush=True)
+    specs = [(901, 2, 16), (902, 3, 17), (903, 2, 18), (904, 1, 10)]
+    circs = _circuits(ctx, specs)
+    prover = gkrx.Prover(ctx)
+    batch = prover.prove_batch(circs)
+    for (claimed, proof), circ in zip(batch, circs):
+        c1, p1 = prover.prove(circ)
+        assert np.array_equal(claimed, c1)
+        assert bytes(proof) == bytes(p1)
+    for c in circs:
+        c.close()
+
+
 def test_result_independent_of_batch_mix(ctx):
     """The same circuit proved alone in a batch of one, and inside a larger mix, gives the
     same bytes."""
