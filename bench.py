@@ -151,6 +151,43 @@ def batch_leg(gkrx, fs, count, steps, warmup):
             "identical_to_plain": same, "verified_sample": ok}
 
 
+def bn254_leg(gkrx, fs, depth, log_width, steps, warmup):
+    """One BN254 scalar-field proof (one lane, one repetition): depth layers of 2^log_width
+    gates, 70% add / 30% mul, locality wiring, witness uniform in [0, r), seed 2."""
+    seed = 2
+    layers = gkrx.synth_circuit(seed, depth, log_width, 700, True)
+    gates = sum(len(m) + len(a) for m, a in layers)
+    ctx = gkrx.Context(gkrx.Config(fs_hash=fs, field_type=gkrx.FIELD_BN254), device=0)
+    circ = gkrx.Circuit.from_layers(ctx, layers)
+    circ.set_witness(gkrx.synth_witness_bn254(seed, 1 << log_width))
+    prover = gkrx.Prover(ctx)
+    buf = np.empty(prover.proof_size(circ), dtype=np.uint8)
+    for _ in range(warmup):
+        circ.evaluate()
+        prover.prove(circ, out=buf)
+    gkrx.lib().gkrx_ctx_profile(ctx._h, 1)
+    lat, parts = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        circ.evaluate()
+        claimed, proof = prover.prove(circ, out=buf)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        parts.append(prover.last_timing())
+    stats = gkrx.kernel_stats(ctx)
+    gkrx.lib().gkrx_ctx_profile(ctx._h, 0)
+    ok = gkrx.Verifier(ctx).verify(circ, claimed, proof)
+    ctx.close()
+    ms = float(np.mean(lat))
+    log(f"bn254 {depth} x 2^{log_width}: {ms:.1f} ms per proof, verified={ok}")
+    return {"workload": f"one BN254 scalar-field proof, {depth} layers x 2^{log_width} gates, 70% add / 30% mul, "
+                        f"locality wiring, witness uniform in [0, r), seed 2", "gates": gates,
+            "gates_per_s": round(gates / (ms * 1e-3), 1), "ms_per_proof": round(ms, 3),
+            "proof_bytes": int(proof.size),
+            "breakdown_ms": {k2: round(float(np.mean([p[k2] for p in parts])), 3) for k2 in parts[0]},
+            "kernels_ms_per_proof": {k2: round(v["ms"] / steps, 3) for k2, v in sorted(stats.items())},
+            "verified": bool(ok)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +200,8 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=2)
     ap.add_argument("--batch", type=int, default=512, help="proofs in the uneven batch leg (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--bn254-layers", type=int, default=12, help="layers of the BN254 leg (0 = skip)")
+    ap.add_argument("--bn254-log-width", type=int, default=18)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (proofs shard by proof id "
@@ -221,6 +260,9 @@ def main():
     batch = None
     if args.batch > 0:
         batch = batch_leg(gkrx, fs, args.batch, args.batch_steps, 1)
+    bn254 = None
+    if args.bn254_layers > 0:
+        bn254 = bn254_leg(gkrx, fs, args.bn254_layers, args.bn254_log_width, 2, 1)
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -243,6 +285,7 @@ def main():
         "kernels_ms_per_proof": {k2: round(v["ms"] / args.steps, 3) for k2, v in sorted(stats.items())},
         "verified": bool(ok),
         "batch": batch,
+        "bn254": bn254,
     }
     print(json.dumps(line), flush=True)
 

@@ -16,35 +16,8 @@
 #include "gkrx.h"
 #include "host_hash.hpp"
 #include "internal.hpp"
+#include "handles.hpp"
 #include "verifier.hpp"
-
-struct gkrx_ctx {
-    gkrx_config cfg;
-    int device = 0;
-    uint32_t R = 3;
-    uint32_t L = 8;       // lanes of the field element
-    uint32_t W = 8;       // u32 words per element: L for M31, 8 for BN254
-    bool bn254 = false;
-    hipStream_t stream = nullptr;
-    gkrx::DevScratch scratch;
-    gkrx::BnScratch bn;
-    gkrx::ProveTimings last;
-    gkrx::Profiler prof;
-    gkrx::BatchScratch batch;
-    std::mutex mu;
-    std::atomic<int> live_circuits{0};  // circuits bound to this context
-};
-
-struct gkrx_circuit {
-    gkrx_ctx *ctx = nullptr;
-    uint32_t D = 0;
-    std::vector<uint32_t> n, m;
-    std::vector<std::vector<uint32_t>> mul, add;  // canonical coefficients
-    gkrx::DevCircuit dev;
-    std::vector<uint32_t> witness;
-    bool has_witness = false;
-    bool evaluated = false;
-};
 
 namespace gkrx {
 
@@ -481,14 +454,6 @@ extern "C" int gkrx_circuit_destroy(gkrx_circuit *c)
     return GKRX_OK;
 }
 
-static size_t tail_bytes(const gkrx_ctx *ctx, const gkrx_circuit *c)
-{
-    const size_t S = (size_t)ctx->W * 4;
-    size_t t = 32;
-    for (uint32_t l = 0; l < c->D; l++) t += ((size_t)6 * c->n[l] * ctx->R + 2 * ctx->R) * S;
-    return t;
-}
-
 extern "C" size_t gkrx_proof_size(const gkrx_ctx *ctx, const gkrx_circuit *c)
 {
     if (!ctx || !c) return 0;
@@ -669,6 +634,7 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
     int rc = dev_prove_batch(items, ctx->R, kind, ctx->batch, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr);
     if (rc) return rc;
     GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    dev_batch_collect(items, ctx->batch);
     const uint32_t *cl = ctx->batch.claims_host;
     if (!cl) return GKRX_ERR_NOMEM;
     for (uint32_t i = 0; i < n; i++) {

@@ -48,6 +48,7 @@ struct DevCircuit {
     uint32_t *csr_pool = nullptr;    // one allocation holding every CSR
     uint32_t *values = nullptr;      // all layer values, D + 1 tables of L lanes
     uint64_t values_u32 = 0;
+    uint64_t csr_u32 = 0;            // size of csr_pool
     uint32_t max_n = 0, max_m = 0;
     LayerDev *d_layers = nullptr;    // device copy of `layers` (read by the batch kernel)
 };
@@ -72,6 +73,11 @@ struct BatchScratch {
     uint8_t *host_meta = nullptr;   // pinned staging of the job table and seeds
     uint32_t *claims_host = nullptr;
     uint64_t claims_cap = 0;
+    // pinned staging of the proof tails: the device copies land here (a copy into pageable
+    // memory would block the launching thread), dev_batch_collect hands them out
+    uint8_t *tails_host = nullptr;
+    uint64_t tails_cap = 0;
+    std::vector<uint64_t> tail_stage_off;
     // pinned host buffer for n claim words (nullptr when allocation fails)
     uint32_t *host_claims(uint64_t n)
     {
@@ -156,10 +162,16 @@ int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, co
               const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,
               Profiler *pf);
 // Proves every item in one launch (one workgroup per proof).  Enqueues the copies of the
-// tails to items[i].tail_out and of the claims (item-major, R * L each) to
-// bs.host_claims(); the caller synchronizes the stream.
+// tails and of the claims (item-major, R * L each, to bs.host_claims()) into pinned
+// staging; once the stream has reached them, dev_batch_collect copies each tail to
+// items[i].tail_out.
 int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_kind, BatchScratch &bs, hipStream_t st,
                     Profiler *pf);
+void dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &bs);
+// Device bytes the batch prover takes for one proof of dc (its tables and transcript).
+uint64_t dev_batch_job_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes);
+// Device bytes of a circuit's resident values and wiring.
+inline uint64_t dev_circuit_bytes(const DevCircuit &dc) { return 4 * (dc.values_u32 + dc.csr_u32); }
 // Whether a circuit fits the batch kernel (every layer at most 2^20 wide).
 int dev_batch_supported(const DevCircuit &dc);
 void dev_free_batch(BatchScratch &bs);
@@ -170,6 +182,7 @@ int dev_bn_to_mont(uint32_t *values, uint64_t n, hipStream_t st);
 int dev_bn_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf);
 int dev_bn_ensure_scratch(BnScratch &s, const DevCircuit &dc, uint64_t tail_bytes);
 void dev_bn_free_scratch(BnScratch &s);
+uint64_t dev_bn_scratch_bytes(const DevCircuit &dc, uint64_t tail_bytes);
 // rz1_host: m_last canonical challenges (8 words each); claims land in s.claims.
 int dev_bn_prove(const DevCircuit &dc, BnScratch &s, int hash_kind, const uint32_t *rz1_host,
                  const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,

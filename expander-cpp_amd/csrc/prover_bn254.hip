@@ -487,6 +487,14 @@ void dev_bn_free_scratch(BnScratch &s)
     s = BnScratch{};
 }
 
+uint64_t dev_bn_scratch_bytes(const DevCircuit &dc, uint64_t tail_bytes)
+{
+    const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
+    const uint64_t N2 = std::max<uint64_t>(2, maxN / 2), N4 = std::max<uint64_t>(2, maxN / 4);
+    return 4 * (8 * maxM + 16 * maxN + 16 * N2 + 16 * N4 + 24ull * kBnMaxBlocks + 8ull * (dc.D + 1) * kBnChalStride +
+                16ull * (dc.D + 1) + 40 + tail_bytes / 4 + 64 + 14 * 64);
+}
+
 int dev_bn_ensure_scratch(BnScratch &s, const DevCircuit &dc, uint64_t tail_bytes)
 {
     if (s.pool && s.max_n >= dc.max_n && s.max_m >= dc.max_m && s.D >= dc.D && s.tail_cap >= tail_bytes)

@@ -1261,7 +1261,8 @@ int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &
     dc.values_u32 = off;
     uint64_t pool = 0;
     for (auto &v : host) pool += (v.size() + 3) & ~3ull;  // 16-byte aligned pieces
-    GKRX_HIP_CHECK(hipMalloc(&dc.csr_pool, std::max<uint64_t>(pool, 4) * 4));
+    dc.csr_u32 = std::max<uint64_t>(pool, 4);
+    GKRX_HIP_CHECK(hipMalloc(&dc.csr_pool, dc.csr_u32 * 4));
     GKRX_HIP_CHECK(hipMalloc(&dc.d_layers, sizeof(LayerDev) * dc.D));
     GKRX_HIP_CHECK(hipMalloc(&dc.values, dc.values_u32 * 4));
     uint64_t at = 0;
@@ -1357,18 +1358,29 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     }
     const uint64_t need = job_off[nj] + tail_off[nj] + nj * (uint64_t)R * L + nj * seed_u32 + 64;
     const uint64_t need_jobs = nj * sizeof(BatchJob) + nj * 4 + 64;
+    GKRX_HIP_CHECK(hipStreamSynchronize(st));  // the previous batch may still use the buffers
     if (need > bs.pool_u32 || need_jobs > bs.meta_bytes) {
         if (bs.pool) (void)hipFree(bs.pool);
         if (bs.meta) (void)hipFree(bs.meta);
         if (bs.host_meta) (void)hipHostFree(bs.host_meta);
-        bs = BatchScratch{};
+        bs.pool = nullptr;
+        bs.meta = nullptr;
+        bs.host_meta = nullptr;
+        bs.pool_u32 = bs.meta_bytes = 0;
         GKRX_HIP_CHECK(hipMalloc(&bs.pool, need * 4));
         GKRX_HIP_CHECK(hipMalloc(&bs.meta, need_jobs));
         GKRX_HIP_CHECK(hipHostMalloc(&bs.host_meta, need_jobs + nj * seed_u32 * 4, hipHostMallocDefault));
         bs.pool_u32 = need;
         bs.meta_bytes = need_jobs;
     }
-    GKRX_HIP_CHECK(hipStreamSynchronize(st));  // the previous batch may still read the staging
+    if (tail_off[nj] * 4 > bs.tails_cap) {
+        if (bs.tails_host) (void)hipHostFree(bs.tails_host);
+        bs.tails_host = nullptr;
+        bs.tails_cap = 0;
+        GKRX_HIP_CHECK(hipHostMalloc(&bs.tails_host, tail_off[nj] * 4, hipHostMallocDefault));
+        bs.tails_cap = tail_off[nj] * 4;
+    }
+    bs.tail_stage_off.assign(tail_off.begin(), tail_off.end());
     uint32_t *tails = bs.pool + job_off[nj];
     uint32_t *claims = tails + tail_off[nj];
     uint32_t *seeds = claims + nj * (uint64_t)R * L;
@@ -1431,13 +1443,32 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                                hash_kind);
     });
     GKRX_HIP_CHECK(hipGetLastError());
-    for (size_t i = 0; i < nj; i++) {
-        GKRX_HIP_CHECK(hipMemcpyAsync(items[i].tail_out, tails + tail_off[i], items[i].tail_bytes,
-                                      hipMemcpyDeviceToHost, st));
-    }
-    GKRX_HIP_CHECK(hipMemcpyAsync(bs.host_claims(nj * R * L), claims, nj * (uint64_t)R * L * 4,
-                                  hipMemcpyDeviceToHost, st));
+    GKRX_HIP_CHECK(hipMemcpyAsync(bs.tails_host, tails, tail_off[nj] * 4, hipMemcpyDeviceToHost, st));
+    uint32_t *hc = bs.host_claims(nj * R * L);
+    if (!hc) return GKRX_ERR_NOMEM;
+    GKRX_HIP_CHECK(hipMemcpyAsync(hc, claims, nj * (uint64_t)R * L * 4, hipMemcpyDeviceToHost, st));
     return GKRX_OK;
+}
+
+void dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &bs)
+{
+    for (size_t i = 0; i < items.size(); i++)
+        memcpy(items[i].tail_out, bs.tails_host + bs.tail_stage_off[i] * 4, items[i].tail_bytes);
+}
+
+uint64_t dev_batch_job_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes)
+{
+    const uint32_t L = dc.L;
+    const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
+    const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
+    uint64_t u = 0;
+    auto add = [&](uint64_t n) { u += (n + 63) & ~63ull; };
+    add(R * maxM); add(R * maxN); add(R * maxN * L); add(R * N2 * L); add(R * N4 * L); add(R * N2 * L);
+    add(R * N4 * L); add(R * N1); add(R * N2); add(R * N4); add((uint64_t)(dc.D + 1) * R * kChalStride);
+    add(2ull * (dc.D + 1)); add(2ull * R * L); add(16);
+    add((tail_bytes + 3) / 4);
+    add((uint64_t)R * L + (uint64_t)R * kChalStride + 18);
+    return 4 * u;
 }
 
 int dev_batch_supported(const DevCircuit &dc)
@@ -1451,6 +1482,7 @@ void dev_free_batch(BatchScratch &bs)
     if (bs.meta) (void)hipFree(bs.meta);
     if (bs.host_meta) (void)hipHostFree(bs.host_meta);
     if (bs.claims_host) (void)hipHostFree(bs.claims_host);
+    if (bs.tails_host) (void)hipHostFree(bs.tails_host);
     bs = BatchScratch{};
 }
 
