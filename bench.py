@@ -139,16 +139,122 @@ def batch_leg(gkrx, fs, count, steps, warmup):
         same &= bool(np.array_equal(cl, res[i][0])) and pr.size == res[i][1].size and bool(np.array_equal(pr, res[i][1]))
     plain_s = time.perf_counter() - t0
     ok = all(gkrx.Verifier(ctx).verify(circs[i], res[i][0], res[i][1]) for i in range(0, count, max(1, count // 8)))
-    ctx.close()
     log(f"batched {batched_s * 1e3:.1f} ms, plain {plain_s * 1e3:.1f} ms, identical={same}, verified={ok}")
-    return {"workload": f"{count} independent M31x16 proofs, width 2^k with k uniform in [10, 18], depth uniform "
-                        f"in [4, 24], 70% add / 30% mul, locality wiring, seed 3 + proof id",
-            "proofs": count, "gates": gates,
-            "batched_gates_per_s": round(gates / batched_s, 1), "batched_ms": round(batched_s * 1e3, 3),
-            "plain_gates_per_s": round(gates / plain_s, 1), "plain_ms": round(plain_s * 1e3, 3),
-            "ratio": round(plain_s / batched_s, 3),
-            "plain_latency_ms": {"p50": round(percentile(plain_lat, 50), 3), "p99": round(percentile(plain_lat, 99), 3)},
-            "identical_to_plain": same, "verified_sample": ok}
+    out = {"workload": f"{count} independent M31x16 proofs, width 2^k with k uniform in [10, 18], depth uniform "
+                       f"in [4, 24], 70% add / 30% mul, locality wiring, seed 3 + proof id",
+           "proofs": count, "gates": gates,
+           "batched_gates_per_s": round(gates / batched_s, 1), "batched_ms": round(batched_s * 1e3, 3),
+           "plain_gates_per_s": round(gates / plain_s, 1), "plain_ms": round(plain_s * 1e3, 3),
+           "ratio": round(plain_s / batched_s, 3),
+           "plain_latency_ms": {"p50": round(percentile(plain_lat, 50), 3), "p99": round(percentile(plain_lat, 99), 3)},
+           "identical_to_plain": same, "verified_sample": ok}
+    # the context and circuits stay resident for the serving leg (the caller closes ctx)
+    return out, (ctx, circs, [g for _, g in made], batched_s / count)
+
+
+def serve_leg(gkrx, fs, m31_state, seconds, budget_gb, slots):
+    """Serving under Poisson arrivals: 95% of the requests are M31x16 proofs drawn from the
+    uneven-batch circuits (witnesses resident in HBM), 5% BN254 proofs of 2^14 .. 2^16 gates
+    per layer and 8 .. 12 layers.  Two servers (one per field) share the GPU; arrivals come
+    at 80% of the saturated rate of that mix (batched M31 time per proof and the BN254
+    proof time measured alone).  A request's latency runs from its arrival to its proof on
+    the host.  Each completed proof is checked against the proof of the same circuit made
+    alone."""
+    import random
+    import threading
+
+    mctx, mcircs, mgates, m31_s = m31_state
+    bctx = gkrx.Context(gkrx.Config(fs_hash=fs, field_type=gkrx.FIELD_BN254), device=0)
+    rng = random.Random(5)
+    bcircs, bgates = [], []
+    for i in range(16):
+        k, d = rng.randint(14, 16), rng.randint(8, 12)
+        layers = gkrx.synth_circuit(20000 + i, d, k, 700, True)
+        c = gkrx.Circuit.from_layers(bctx, layers)
+        c.set_witness(gkrx.synth_witness_bn254(20000 + i, 1 << k))
+        bcircs.append(c)
+        bgates.append(sum(len(m) + len(a) for m, a in layers))
+    # reference proofs made alone, for the identity check
+    mprover, bprover = gkrx.Prover(mctx), gkrx.Prover(bctx)
+    check_m = {i: mprover.prove(mcircs[i]) for i in range(0, len(mcircs), 37)}
+    t0 = time.perf_counter()
+    check_b = {i: bprover.prove(c) for i, c in enumerate(bcircs)}
+    bn_s = (time.perf_counter() - t0) / len(bcircs)
+    rate = 0.8 / (0.95 * m31_s + 0.05 * bn_s)
+    log(f"serving: {rate:.0f} requests/s for {seconds}s (M31 batched {m31_s * 1e3:.2f} ms/proof, "
+        f"BN254 {bn_s * 1e3:.1f} ms/proof alone)")
+    budget = int(budget_gb * (1 << 30))
+    msrv, bsrv = gkrx.Server(mctx, budget, slots), gkrx.Server(bctx, budget // 8, 2)
+    free = {"m": list(range(len(mcircs))), "b": list(range(len(bcircs)))}
+    pending = []  # (server, ticket, kind, index, arrival)
+    lock = threading.Lock()
+    lat, done_gates, mismatches = [], [0], [0]
+    stop = threading.Event()
+
+    def reaper():
+        while not (stop.is_set() and not pending):
+            with lock:
+                items = list(pending)
+            progressed = False
+            for it in items:
+                srv, t, kind, i, arr, sub = it
+                r = srv.wait(t, timeout_s=0)
+                if r is None:
+                    continue
+                cl, pr, _ = r
+                now = time.perf_counter()
+                ref = (check_m if kind == "m" else check_b).get(i)
+                if ref is not None and (bytes(pr) != bytes(ref[1]) or not np.array_equal(cl, ref[0])):
+                    mismatches[0] += 1
+                with lock:
+                    pending.remove(it)
+                    free[kind].append(i)
+                    lat.append((now - arr) * 1e3)
+                    done_gates[0] += (mgates if kind == "m" else bgates)[i]
+                progressed = True
+            if not progressed:
+                time.sleep(0.0005)
+
+    th = threading.Thread(target=reaper)
+    th.start()
+    start = time.perf_counter()
+    t_next = start
+    arrivals = waited = 0
+    while t_next - start < seconds:
+        now = time.perf_counter()
+        if now < t_next:
+            time.sleep(min(t_next - now, 0.002))
+            continue
+        kind = "b" if rng.random() < 0.05 else "m"
+        while True:
+            with lock:
+                if free[kind]:
+                    i = free[kind].pop(rng.randrange(len(free[kind])))
+                    break
+            waited += 1
+            time.sleep(0.0002)
+        srv, circ = (msrv, mcircs[i]) if kind == "m" else (bsrv, bcircs[i])
+        t = srv.submit(circ)
+        with lock:
+            pending.append((srv, t, kind, i, t_next, time.perf_counter()))
+        arrivals += 1
+        t_next += rng.expovariate(rate)
+    stop.set()
+    th.join()
+    elapsed = time.perf_counter() - start
+    ms, bs = msrv.stats(), bsrv.stats()
+    msrv.close()
+    bsrv.close()
+    bctx.close()
+    log(f"served {arrivals} requests in {elapsed:.1f}s, p50 {percentile(lat, 50):.1f} ms, p99 {percentile(lat, 99):.1f} ms")
+    return {"workload": f"Poisson arrivals for {seconds}s at 80% of the saturated rate of the mix; 95% M31x16 proofs "
+                        f"of the uneven batch, 5% BN254 proofs (2^14..2^16 gates x 8..12 layers); HBM budget "
+                        f"{budget_gb} GB; {slots} M31 slots", "rate_per_s": round(rate, 1), "requests": arrivals,
+            "gates_per_s": round(done_gates[0] / elapsed, 1), "proofs_per_s": round(arrivals / elapsed, 1),
+            "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)},
+            "m31_batches": ms["batches"], "m31_largest_batch": ms["max_batch"],
+            "m31_peak_admitted_gb": round(ms["peak_bytes"] / (1 << 30), 2), "bn254_proofs": bs["completed"],
+            "arrivals_waiting_for_a_free_circuit": waited, "mismatches_vs_alone": mismatches[0]}
 
 
 def bn254_leg(gkrx, fs, depth, log_width, steps, warmup):
@@ -200,6 +306,9 @@ def main():
     ap.add_argument("--cpu-layers", type=int, default=2)
     ap.add_argument("--batch", type=int, default=512, help="proofs in the uneven batch leg (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving leg duration (0 = skip)")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--bn254-layers", type=int, default=12, help="layers of the BN254 leg (0 = skip)")
     ap.add_argument("--bn254-log-width", type=int, default=18)
     args = ap.parse_args()
@@ -257,9 +366,12 @@ def main():
                 "kernel": "k_round<16,phase1,fused>", "avg_launch_us": round(avg_ms * 1e3, 2),
                 "launches": st["count"]}
     ctx.close()
-    batch = None
+    batch = serve = None
     if args.batch > 0:
-        batch = batch_leg(gkrx, fs, args.batch, args.batch_steps, 1)
+        batch, m31_state = batch_leg(gkrx, fs, args.batch, args.batch_steps, 1)
+        if args.serve_seconds > 0:
+            serve = serve_leg(gkrx, fs, m31_state, args.serve_seconds, args.serve_budget_gb, args.serve_slots)
+        m31_state[0].close()
     bn254 = None
     if args.bn254_layers > 0:
         bn254 = bn254_leg(gkrx, fs, args.bn254_layers, args.bn254_log_width, 2, 1)
@@ -285,6 +397,7 @@ def main():
         "kernels_ms_per_proof": {k2: round(v["ms"] / args.steps, 3) for k2, v in sorted(stats.items())},
         "verified": bool(ok),
         "batch": batch,
+        "serving": serve,
         "bn254": bn254,
     }
     print(json.dumps(line), flush=True)

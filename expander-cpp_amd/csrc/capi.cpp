@@ -351,6 +351,10 @@ extern "C" int gkrx_circuit_write_text(const gkrx_circuit *c, const char *mul_pa
 extern "C" int gkrx_circuit_set_witness(gkrx_circuit *c, const uint32_t *lanes, size_t n_u32)
 {
     if (!c || !lanes) return GKRX_ERR_ARG;
+    if (c->in_server) {
+        set_error("circuit has a server request in flight");
+        return GKRX_ERR_STATE;
+    }
     const size_t need = ((size_t)1 << c->n[0]) * c->dev.L;
     if (n_u32 != need) {
         set_error("witness must hold 2^nb_input_vars(0) field elements");
@@ -402,6 +406,10 @@ static int evaluate_locked(gkrx_circuit *c)
 extern "C" int gkrx_circuit_evaluate(gkrx_circuit *c)
 {
     if (!c) return GKRX_ERR_ARG;
+    if (c->in_server) {
+        set_error("circuit has a server request in flight");
+        return GKRX_ERR_STATE;
+    }
     std::lock_guard<std::mutex> lk(c->ctx->mu);
     GKRX_HIP_CHECK(hipSetDevice(c->ctx->device));
     return evaluate_locked(c);
@@ -443,6 +451,10 @@ extern "C" int gkrx_circuit_get_output(gkrx_circuit *c, uint32_t *out, size_t n_
 extern "C" int gkrx_circuit_destroy(gkrx_circuit *c)
 {
     if (!c) return GKRX_ERR_ARG;
+    if (c->in_server) {
+        set_error("circuit has a server request in flight");
+        return GKRX_ERR_STATE;
+    }
     if (c->ctx) {
         std::lock_guard<std::mutex> lk(c->ctx->mu);
         (void)hipSetDevice(c->ctx->device);
@@ -505,6 +517,10 @@ extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t
                           uint32_t *claimed_v)
 {
     if (!ctx || !c || !proof || !claimed_v || c->ctx != ctx) return GKRX_ERR_ARG;
+    if (c->in_server) {
+        set_error("circuit has a server request in flight");
+        return GKRX_ERR_STATE;
+    }
     const size_t total = gkrx_proof_size(ctx, c);
     if (proof_cap < total) {
         set_error("proof buffer too small");
@@ -566,6 +582,10 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
     for (uint32_t i = 0; i < n; i++) {
         gkrx_circuit *c = circuits[i];
         if (!c || c->ctx != ctx || !proofs[i] || !claimed_v[i]) return GKRX_ERR_ARG;
+        if (c->in_server) {
+            set_error("batch item " + std::to_string(i) + " has a server request in flight");
+            return GKRX_ERR_STATE;
+        }
         if (!c->has_witness) {
             set_error("batch item " + std::to_string(i) + " has no witness");
             return GKRX_ERR_STATE;

@@ -75,6 +75,11 @@ def lib():
     L.gkrx_hash.argtypes = [C.c_int, C.c_char_p, C.c_size_t, C.c_char_p]
     L.gkrx_prove_batch.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]
+    L.gkrx_server_create.argtypes = [vp, C.c_uint64, C.c_uint32, C.POINTER(vp)]
+    L.gkrx_server_submit.argtypes = [vp, vp, _u8p, C.c_size_t, _u32p, C.POINTER(C.c_uint64)]
+    L.gkrx_server_wait.argtypes = [vp, C.c_uint64, C.c_int64, C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
+    L.gkrx_server_stats.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.gkrx_server_destroy.argtypes = [vp]
     L.gkrx_ctx_profile.argtypes = [vp, C.c_int]
     L.gkrx_ctx_kernel_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_double)]
@@ -190,6 +195,7 @@ class Context:
         self.config = config
         self._h = None
         self._circuits = weakref.WeakSet()
+        self._servers = weakref.WeakSet()
         h = C.c_void_p()
         _check(lib().gkrx_ctx_create(C.byref(config._c), device, C.byref(h)), "gkrx_ctx_create")
         self._h = h
@@ -197,6 +203,8 @@ class Context:
     def close(self):
         """Destroys the context after every circuit created on it."""
         if self._h:
+            for s in list(self._servers):
+                s.close()
             for c in list(self._circuits):
                 c.close()
             _check(lib().gkrx_ctx_destroy(self._h), "gkrx_ctx_destroy")
@@ -342,3 +350,56 @@ def verify_layers(config: Config, layers, claimed_v, proof) -> bool:
     pb = np.ascontiguousarray(np.frombuffer(bytes(proof), dtype=np.uint8))
     return _check(lib().gkrx_verify_arrays(C.byref(config._c), D, nm, mp, na, ap, cv, pb, pb.size),
                   "gkrx_verify_arrays") == 1
+
+
+class Server:
+    """Continuous-batching prove server on one context (gkrx_server_*): requests are
+    admitted against an HBM budget and launched in batches on free slots as they arrive.
+    Proofs are byte-identical to Prover.prove of the same circuit."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, slots: int = 4):
+        self.ctx = ctx
+        self._h = None
+        h = C.c_void_p()
+        _check(lib().gkrx_server_create(ctx._h, int(hbm_budget_bytes), slots, C.byref(h)), "gkrx_server_create")
+        self._h = h
+        self._pending = {}  # ticket -> (circuit, proof buffer, claims buffer)
+        ctx._servers.add(self)
+
+    def submit(self, circuit: Circuit) -> int:
+        size = lib().gkrx_proof_size(self.ctx._h, circuit._h)
+        buf = np.empty(size, dtype=np.uint8)
+        claims = np.empty(self.ctx.config.get_num_repetitions() * self.ctx.config.words, dtype=np.uint32)
+        t = C.c_uint64()
+        _check(lib().gkrx_server_submit(self._h, circuit._h, buf, buf.size, claims, C.byref(t)), "gkrx_server_submit")
+        self._pending[t.value] = (circuit, buf, claims)
+        return t.value
+
+    def wait(self, ticket: int, timeout_s: float | None = None):
+        """(claimed[R, words], proof, latency_ms), or None if still pending at the timeout."""
+        n, lat = C.c_size_t(), C.c_double()
+        us = -1 if timeout_s is None else int(timeout_s * 1e6)
+        rc = lib().gkrx_server_wait(self._h, ticket, us, C.byref(n), C.byref(lat))
+        if rc == 1:
+            return None
+        circuit, buf, claims = self._pending.pop(ticket)
+        _check(rc, "gkrx_server_wait")
+        return claims.reshape(self.ctx.config.get_num_repetitions(), -1), buf[:n.value], lat.value
+
+    def stats(self) -> dict:
+        out = (C.c_uint64 * 6)()
+        _check(lib().gkrx_server_stats(self._h, out), "gkrx_server_stats")
+        keys = ("submitted", "completed", "batches", "max_batch", "peak_bytes", "admitted_bytes")
+        return dict(zip(keys, [int(x) for x in out]))
+
+    def close(self):
+        if self._h:
+            _check(lib().gkrx_server_destroy(self._h), "gkrx_server_destroy")
+            self._h = None
+            self._pending.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -111,6 +111,30 @@ int gkrx_verify_arrays(const gkrx_config *cfg, uint32_t nb_layers, const uint32_
                        const uint32_t *n_add, const uint32_t *const *add, const uint32_t *claimed_v,
                        const uint8_t *proof, size_t proof_len);
 
+/* ---- Serving.  No reference counterpart (the reference proves one circuit per
+ * Prover::prove call): a continuous-batching server for callers that prove a stream of
+ * requests.  Requests queue in arrival order and are admitted while the HBM they need
+ * (the circuit's resident values and wiring plus the prover tables, computed from the
+ * circuit shape) fits hbm_budget_bytes; everything admitted is launched as one batch on
+ * the first free slot (a stream with its own scratch; `slots` <= 16), so requests that
+ * arrive while a batch runs start on the next free slot and overlap with it on the GPU.
+ * Each proof is byte-identical to gkrx_prove of the same circuit. */
+typedef struct gkrx_server gkrx_server;
+int gkrx_server_create(gkrx_ctx *ctx, uint64_t hbm_budget_bytes, uint32_t slots, gkrx_server **out);
+/* Queues one prove of c (witness set).  proof / claimed_v must stay valid, and c must not
+ * be proved, changed or destroyed, until the request completes. */
+int gkrx_server_submit(gkrx_server *s, gkrx_circuit *c, uint8_t *proof, size_t proof_cap, uint32_t *claimed_v,
+                       uint64_t *ticket);
+/* Waits up to timeout_us (< 0: without limit) for a request: GKRX_OK when it completed
+ * (proof_len and latency_ms = submit to completion are set; the ticket is released),
+ * 1 if it is still pending, < 0 if it failed. */
+int gkrx_server_wait(gkrx_server *s, uint64_t ticket, int64_t timeout_us, size_t *proof_len, double *latency_ms);
+/* out: [0] submitted, [1] completed, [2] batches launched, [3] largest batch,
+ * [4] peak admitted bytes, [5] admitted bytes now. */
+int gkrx_server_stats(gkrx_server *s, uint64_t out[6]);
+/* Finishes every queued request, then frees the server. */
+int gkrx_server_destroy(gkrx_server *s);
+
 /* Millisecond split of the last gkrx_prove on this context: [0] host transcript prefix
  * (commitment hash + grinding + output challenges), [1] device work, [2] total. */
 int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3);
