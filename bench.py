@@ -184,7 +184,8 @@ def serve_leg(gkrx, fs, m31_state, seconds, budget_gb, slots):
     log(f"serving: {rate:.0f} requests/s for {seconds}s (M31 batched {m31_s * 1e3:.2f} ms/proof, "
         f"BN254 {bn_s * 1e3:.1f} ms/proof alone)")
     budget = int(budget_gb * (1 << 30))
-    msrv, bsrv = gkrx.Server(mctx, budget, slots), gkrx.Server(bctx, budget // 8, 2)
+    # M31: `slots` batch streams + one stream for large proofs; BN254: one stream
+    msrv, bsrv = gkrx.Server(mctx, budget, slots), gkrx.Server(bctx, budget // 8, 1)
     free = {"m": list(range(len(mcircs))), "b": list(range(len(bcircs)))}
     pending = []  # (server, ticket, kind, index, arrival)
     lock = threading.Lock()
@@ -308,7 +309,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="serving leg duration (0 = skip)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--bn254-layers", type=int, default=12, help="layers of the BN254 leg (0 = skip)")
     ap.add_argument("--bn254-log-width", type=int, default=18)
     args = ap.parse_args()

@@ -1,16 +1,26 @@
 // Continuous-batching prove server (gkrx_server_*): the serving loop of the library.
 //
 // The reference proves one circuit per Prover::prove call and has no serving layer; this
-// is the build's own.  Requests (a circuit with its witness resident in HBM) queue in
-// arrival order.  A scheduler thread admits them while the HBM they need -- the circuit's
-// resident values and wiring plus the prover tables, both computed from the circuit shape
-// -- fits the budget, and launches everything admitted as one batch on the first free
-// slot (a stream with its own scratch).  Several slots run at once, so requests that
-// arrive while a batch is on the GPU start on the next free slot instead of waiting for
-// that batch to finish, and their work overlaps on the device.  A batch is one
-// k_prove_batch launch (one workgroup per proof, M31); BN254 requests run one per slot
-// through the multi-block BN254 path.  Each proof is the same bytes as gkrx_prove of the
-// same circuit: a proof's transcript depends only on its own circuit and witness.
+// is the build's own.  A request is a circuit whose witness is resident in HBM.
+//
+//   1. Host prefix.  A pool of host threads computes each request's transcript prefix
+//      (the commitment hash, grinding and output challenges: a serial hash chain per
+//      proof) as soon as it arrives, so that work overlaps everything on the GPU.
+//   2. Admission.  Requests whose prefix is ready are admitted in arrival order while the
+//      HBM they need -- the circuit's resident values and wiring plus the prover tables,
+//      both computed from the circuit shape -- fits the budget.
+//   3. Launch.  Slots are streams with their own scratch.  A batch slot launches every
+//      admitted small M31 request at once (k_prove_batch: one workgroup per proof); a
+//      single slot runs one large M31 proof (or one BN254 proof) with the multi-block
+//      path, which puts the whole GPU on it instead of one workgroup.  Whichever slot is
+//      free takes the next work, so requests that arrive while others run start without
+//      waiting for them, and their kernels overlap on the device.
+//   4. Completion is observed by polling each slot's event (a stream host-function
+//      callback is only delivered while some host thread is inside the runtime, and the
+//      server's threads sleep between events).
+//
+// Each proof is the same bytes as gkrx_prove of the same circuit: a proof's transcript
+// depends only on its own circuit and witness.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,15 +44,20 @@ using Clock = std::chrono::steady_clock;
 namespace {
 
 constexpr uint32_t kMaxBatch = 1024;
+// M31 requests with at least this many table rows over all layers (sum of 2^n) take a
+// single slot: in one workgroup they would run for hundreds of milliseconds.
+constexpr uint64_t kBigRows = 1ull << 19;
+constexpr auto kPollPeriod = std::chrono::microseconds(100);
 
 struct SrvReq {
     uint64_t id = 0;
     gkrx_circuit *c = nullptr;
     uint8_t *proof = nullptr;
     uint32_t *claims = nullptr;
-    uint64_t bytes = 0;  // admission footprint
+    uint64_t bytes = 0;   // admission footprint
+    bool single = false;  // runs on a single slot (large M31, or BN254)
     Clock::time_point t_submit;
-    int state = 0;       // 0 queued, 1 on the GPU, 2 done
+    int state = 0;        // 0 queued, 1 on the GPU, 2 done
     int rc = 0;
     size_t len = 0;
     double lat_ms = 0;
@@ -52,24 +67,19 @@ struct SrvReq {
 };
 
 struct Slot {
-    gkrx_server *srv = nullptr;
+    bool single = false;
     hipStream_t st = nullptr;
-    hipEvent_t ev = nullptr;  // recorded after a batch's last copy; the scheduler polls it
-    gkrx::BatchScratch bs;
-    gkrx::BnScratch bn;
-    uint8_t *bn_tail_host = nullptr;
-    uint64_t bn_tail_cap = 0;
-    uint32_t *bn_claims_host = nullptr;
+    hipEvent_t ev = nullptr;  // recorded after the slot's last copy
+    gkrx::BatchScratch bs;    // batch slot
+    gkrx::DevScratch ds;      // single slot, M31
+    gkrx::BnScratch bn;       // single slot, BN254
+    uint8_t *tail_host = nullptr;  // pinned staging of a single proof's tail
+    uint64_t tail_cap = 0;
+    uint32_t *small_host = nullptr;  // pinned: claims (up to 64 words) then tstate (16 words)
     std::vector<SrvReq *> reqs;
     std::vector<gkrx::BatchItem> items;
     bool busy = false;
 };
-
-// How often the scheduler polls the slot events while batches are in flight.  Completion
-// is observed by polling, not by a stream host-function callback: the runtime delivers such
-// callbacks only while some host thread is inside it, and a server whose threads all sleep
-// waiting for the callback never sees its batches finish.
-constexpr auto kPollPeriod = std::chrono::microseconds(100);
 
 }  // namespace
 
@@ -78,19 +88,24 @@ struct gkrx_server {
     uint64_t budget = 0, in_use = 0, peak = 0;
     std::vector<std::unique_ptr<Slot>> slots;
     std::mutex m;
-    std::condition_variable cv, done_cv;
-    std::deque<SrvReq *> queue;
+    std::condition_variable cv, done_cv, pcv;
+    std::deque<SrvReq *> pq;     // waiting for the host prefix
+    std::deque<SrvReq *> ready;  // prefix done, waiting for admission
+    uint32_t prefixing = 0;      // prefixes being computed
     std::unordered_map<uint64_t, SrvReq *> reqs;
     uint64_t next_id = 1;
     bool stop = false;
     uint64_t submitted = 0, completed = 0, batches = 0, max_batch = 0;
     std::thread sched;
+    std::vector<std::thread> workers;
 };
 
 using namespace gkrx;
 
 namespace {
 
+// HBM a request holds while admitted: its circuit plus the prover tables of one proof (the
+// single-slot M31 path keeps the same tables as one batch job)
 uint64_t footprint(const gkrx_ctx *ctx, const gkrx_circuit *c)
 {
     const uint64_t tb = tail_bytes(ctx, c);
@@ -98,63 +113,87 @@ uint64_t footprint(const gkrx_ctx *ctx, const gkrx_circuit *c)
            (ctx->bn254 ? dev_bn_scratch_bytes(c->dev, tb) : dev_batch_job_bytes(c->dev, ctx->R, tb));
 }
 
-void host_prefixes(gkrx_ctx *ctx, std::vector<SrvReq *> &rs)
+void compute_prefix(gkrx_ctx *ctx, SrvReq *r)
 {
     const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
-    std::atomic<size_t> next{0};
-    auto work = [&] {
-        for (size_t k; (k = next++) < rs.size();) {
-            SrvReq *r = rs[k];
-            gkrx_circuit *c = r->c;
-            const uint32_t mlast = c->m[c->D - 1];
-            const size_t cl = commit_bytes(ctx, c);
-            if (ctx->bn254) {
-                r->rz1.resize((size_t)8 * mlast);
-                host_transcript_prefix_bn254(kind, reinterpret_cast<const uint8_t *>(c->witness.data()), cl, mlast,
-                                             (uint32_t)ctx->cfg.grinding_bits, r->grind, r->rz1.data(), r->tstate);
-            } else {
-                r->rz1.resize((size_t)ctx->R * mlast);
-                host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()), cl, ctx->R, mlast,
-                                       (uint32_t)ctx->cfg.grinding_bits, r->grind, r->rz1.data(), r->tstate);
-            }
-            memcpy(r->proof, c->witness.data(), cl);  // RAW commitment bytes (raw.hpp:19-27)
-        }
-    };
-    const size_t nt = std::min<size_t>(rs.size(), 16);
-    std::vector<std::thread> pool;
-    for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
-    work();
-    for (auto &th : pool) th.join();
+    gkrx_circuit *c = r->c;
+    const uint32_t mlast = c->m[c->D - 1];
+    const size_t cl = commit_bytes(ctx, c);
+    if (ctx->bn254) {
+        r->rz1.resize((size_t)8 * mlast);
+        host_transcript_prefix_bn254(kind, reinterpret_cast<const uint8_t *>(c->witness.data()), cl, mlast,
+                                     (uint32_t)ctx->cfg.grinding_bits, r->grind, r->rz1.data(), r->tstate);
+    } else {
+        r->rz1.resize((size_t)ctx->R * mlast);
+        host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()), cl, ctx->R, mlast,
+                               (uint32_t)ctx->cfg.grinding_bits, r->grind, r->rz1.data(), r->tstate);
+    }
+    memcpy(r->proof, c->witness.data(), cl);  // RAW commitment bytes (raw.hpp:19-27)
 }
 
-// Enqueues the slot's batch; returns a status (nothing is enqueued on failure).
+void prefix_worker(gkrx_server *S)
+{
+    std::unique_lock<std::mutex> lk(S->m);
+    for (;;) {
+        S->pcv.wait(lk, [&] { return S->stop || !S->pq.empty(); });
+        if (S->pq.empty()) return;  // stopping and nothing left
+        SrvReq *r = S->pq.front();
+        S->pq.pop_front();
+        S->prefixing++;
+        lk.unlock();
+        compute_prefix(S->ctx, r);
+        lk.lock();
+        S->prefixing--;
+        S->ready.push_back(r);
+        S->cv.notify_all();
+    }
+}
+
+int ensure_pinned(Slot &sl, uint64_t tail_bytes)
+{
+    if (tail_bytes > sl.tail_cap) {
+        if (sl.tail_host) (void)hipHostFree(sl.tail_host);
+        sl.tail_host = nullptr;
+        sl.tail_cap = 0;
+        GKRX_HIP_CHECK(hipHostMalloc(&sl.tail_host, tail_bytes, hipHostMallocDefault));
+        sl.tail_cap = tail_bytes;
+    }
+    if (!sl.small_host) GKRX_HIP_CHECK(hipHostMalloc(&sl.small_host, 80 * 4, hipHostMallocDefault));
+    return GKRX_OK;
+}
+
+// Enqueues the slot's work (prefixes are ready); nothing is left enqueued on failure
+// except work that the caller drains.
 int launch(gkrx_server *S, Slot &sl)
 {
     gkrx_ctx *ctx = S->ctx;
     const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
-    if (ctx->bn254) {
+    if (sl.single) {
         SrvReq *r = sl.reqs[0];
         gkrx_circuit *c = r->c;
         const uint64_t tb = tail_bytes(ctx, c);
-        int rc = dev_bn_ensure_scratch(sl.bn, c->dev, tb);
+        int rc = ensure_pinned(sl, tb);
         if (rc) return rc;
-        if (tb > sl.bn_tail_cap) {
-            if (sl.bn_tail_host) (void)hipHostFree(sl.bn_tail_host);
-            sl.bn_tail_host = nullptr;
-            sl.bn_tail_cap = 0;
-            GKRX_HIP_CHECK(hipHostMalloc(&sl.bn_tail_host, tb, hipHostMallocDefault));
-            sl.bn_tail_cap = tb;
+        if (ctx->bn254) {
+            rc = dev_bn_ensure_scratch(sl.bn, c->dev, tb);
+            if (!rc) rc = dev_bn_forward(c->dev, sl.st, nullptr);
+            if (!rc) rc = dev_bn_prove(c->dev, sl.bn, kind, r->rz1.data(), r->tstate, r->grind, 32, sl.st, nullptr);
+            if (rc) return rc;
+            GKRX_HIP_CHECK(hipMemcpyAsync(sl.tail_host, sl.bn.tail, tb, hipMemcpyDeviceToHost, sl.st));
+            GKRX_HIP_CHECK(hipMemcpyAsync(sl.small_host, sl.bn.claims, 32, hipMemcpyDeviceToHost, sl.st));
+            GKRX_HIP_CHECK(hipMemcpyAsync(sl.small_host + 64, sl.bn.tstate, 40, hipMemcpyDeviceToHost, sl.st));
+        } else {
+            rc = dev_ensure_scratch(sl.ds, c->dev, ctx->R, tb);
+            if (!rc) rc = dev_forward(c->dev, sl.st, nullptr);
+            if (!rc)
+                rc = dev_prove(c->dev, sl.ds, ctx->R, kind, r->rz1.data(), r->tstate, r->grind, 32, sl.st, nullptr);
+            if (rc) return rc;
+            GKRX_HIP_CHECK(hipMemcpyAsync(sl.tail_host, sl.ds.tail, tb, hipMemcpyDeviceToHost, sl.st));
+            GKRX_HIP_CHECK(hipMemcpyAsync(sl.small_host, sl.ds.claims, (size_t)ctx->R * ctx->W * 4,
+                                          hipMemcpyDeviceToHost, sl.st));
+            GKRX_HIP_CHECK(hipMemcpyAsync(sl.small_host + 64, sl.ds.tstate, 40, hipMemcpyDeviceToHost, sl.st));
         }
-        if (!sl.bn_claims_host) GKRX_HIP_CHECK(hipHostMalloc(&sl.bn_claims_host, 32, hipHostMallocDefault));
-        rc = dev_bn_forward(c->dev, sl.st, nullptr);  // overlaps the host prefix
-        if (rc) return rc;
-        host_prefixes(ctx, sl.reqs);
-        rc = dev_bn_prove(c->dev, sl.bn, kind, r->rz1.data(), r->tstate, r->grind, 32, sl.st, nullptr);
-        if (rc) return rc;
-        GKRX_HIP_CHECK(hipMemcpyAsync(sl.bn_tail_host, sl.bn.tail, tb, hipMemcpyDeviceToHost, sl.st));
-        GKRX_HIP_CHECK(hipMemcpyAsync(sl.bn_claims_host, sl.bn.claims, 32, hipMemcpyDeviceToHost, sl.st));
     } else {
-        host_prefixes(ctx, sl.reqs);
         sl.items.resize(sl.reqs.size());
         for (size_t i = 0; i < sl.reqs.size(); i++) {
             SrvReq *r = sl.reqs[i];
@@ -168,19 +207,25 @@ int launch(gkrx_server *S, Slot &sl)
     return GKRX_OK;
 }
 
-// Hands the results of a finished slot to its requests (S->m not held).
-void collect(gkrx_server *S, Slot &sl)
+// Hands the results of a finished slot to its requests (S->m not held); returns a status.
+int collect(gkrx_server *S, Slot &sl)
 {
     gkrx_ctx *ctx = S->ctx;
     const size_t cw = (size_t)ctx->R * ctx->W;
-    if (ctx->bn254) {
+    if (sl.single) {
         SrvReq *r = sl.reqs[0];
-        memcpy(r->proof + commit_bytes(ctx, r->c), sl.bn_tail_host, tail_bytes(ctx, r->c));
-        memcpy(r->claims, sl.bn_claims_host, 32);
+        const uint64_t tb = tail_bytes(ctx, r->c);
+        if (sl.small_host[64] != tb) {
+            set_error("device transcript length mismatch");
+            return GKRX_ERR_DEVICE;
+        }
+        memcpy(r->proof + commit_bytes(ctx, r->c), sl.tail_host, tb);
+        memcpy(r->claims, sl.small_host, cw * 4);
     } else {
         dev_batch_collect(sl.items, sl.bs);
         for (size_t i = 0; i < sl.reqs.size(); i++) memcpy(sl.reqs[i]->claims, sl.bs.claims_host + i * cw, cw * 4);
     }
+    return GKRX_OK;
 }
 
 void finish_reqs(gkrx_server *S, std::vector<SrvReq *> &rs, int rc)
@@ -198,85 +243,103 @@ void finish_reqs(gkrx_server *S, std::vector<SrvReq *> &rs, int rc)
     }
 }
 
+// Takes the work for a free slot from the ready queue, oldest first: the first request of
+// the slot's kind that fits (single slot), or every request of its kind in order while the
+// budget lasts (batch slot).  A request that does not fit stops its kind's admission, so
+// large requests are not starved by small ones behind them.
+std::vector<SrvReq *> pick(gkrx_server *S, const Slot &sl)
+{
+    std::vector<SrvReq *> out;
+    const uint32_t cap = sl.single ? 1u : kMaxBatch;
+    for (auto it = S->ready.begin(); it != S->ready.end() && out.size() < cap;) {
+        SrvReq *r = *it;
+        if (r->single != sl.single) {
+            ++it;
+            continue;
+        }
+        if (S->in_use + r->bytes > S->budget) break;
+        S->in_use += r->bytes;
+        r->state = 1;
+        out.push_back(r);
+        it = S->ready.erase(it);
+    }
+    return out;
+}
+
 void scheduler(gkrx_server *S)
 {
     (void)hipSetDevice(S->ctx->device);
     std::unique_lock<std::mutex> lk(S->m);
     for (;;) {
-        // completions: the slots whose event has been reached (queried without the lock)
+        // completions: slots whose event has been reached (queried without the lock)
         std::vector<Slot *> busy;
         for (auto &sp : S->slots)
             if (sp->busy) busy.push_back(sp.get());
         bool progressed = false;
         if (!busy.empty()) {
             lk.unlock();
-            std::vector<Slot *> finished;
+            std::vector<std::pair<Slot *, int>> finished;
             for (Slot *sl : busy) {
                 const hipError_t q = hipEventQuery(sl->ev);
                 if (q == hipSuccess) {
-                    collect(S, *sl);
-                    finished.push_back(sl);
+                    finished.push_back({sl, collect(S, *sl)});
                 } else if (q != hipErrorNotReady) {
                     (void)hipStreamSynchronize(sl->st);
-                    finished.push_back(sl);
-                    sl->reqs.front()->rc = GKRX_ERR_DEVICE;  // marks the whole slot failed
+                    finished.push_back({sl, GKRX_ERR_DEVICE});
                 }
             }
             lk.lock();
-            for (Slot *sl : finished) {
-                const int rc = sl->reqs.front()->rc;
-                finish_reqs(S, sl->reqs, rc);
-                sl->reqs.clear();
-                sl->busy = false;
+            for (auto &f : finished) {
+                finish_reqs(S, f.first->reqs, f.second);
+                f.first->reqs.clear();
+                f.first->busy = false;
                 progressed = true;
             }
         }
         if (progressed) S->done_cv.notify_all();
-        Slot *free_slot = nullptr;
-        for (auto &sp : S->slots)
-            if (!sp->busy) {
-                free_slot = sp.get();
-                break;
+        // launches on free slots
+        bool launched = false;
+        for (auto &sp : S->slots) {
+            Slot &sl = *sp;
+            if (sl.busy || S->ready.empty()) continue;
+            std::vector<SrvReq *> batch = pick(S, sl);
+            if (batch.empty()) continue;
+            S->peak = std::max(S->peak, S->in_use);
+            S->batches++;
+            S->max_batch = std::max<uint64_t>(S->max_batch, batch.size());
+            sl.busy = true;
+            sl.reqs = batch;
+            lk.unlock();
+            int rc = launch(S, sl);
+            if (rc) (void)hipStreamSynchronize(sl.st);  // no completion event: drain the slot
+            lk.lock();
+            if (rc) {
+                finish_reqs(S, sl.reqs, rc);
+                sl.reqs.clear();
+                sl.busy = false;
+                S->done_cv.notify_all();
             }
-        if (free_slot && !S->queue.empty()) {
-            std::vector<SrvReq *> batch;
-            const uint32_t cap = S->ctx->bn254 ? 1u : kMaxBatch;
-            while (!S->queue.empty() && batch.size() < cap && S->in_use + S->queue.front()->bytes <= S->budget) {
-                SrvReq *r = S->queue.front();
-                S->queue.pop_front();
-                r->state = 1;
-                S->in_use += r->bytes;
-                batch.push_back(r);
-            }
-            if (!batch.empty()) {
-                S->peak = std::max(S->peak, S->in_use);
-                S->batches++;
-                S->max_batch = std::max<uint64_t>(S->max_batch, batch.size());
-                for (SrvReq *r : batch) r->rc = GKRX_OK;
-                free_slot->busy = true;
-                free_slot->reqs = batch;
-                lk.unlock();
-                const int rc = launch(S, *free_slot);
-                lk.lock();
-                if (rc) {
-                    // the completion event was not recorded: drain the slot, fail its requests
-                    lk.unlock();
-                    (void)hipStreamSynchronize(free_slot->st);
-                    lk.lock();
-                    finish_reqs(S, free_slot->reqs, rc);
-                    free_slot->reqs.clear();
-                    free_slot->busy = false;
-                    S->done_cv.notify_all();
-                }
-                continue;
-            }
+            launched = true;
         }
+        if (launched) continue;
         bool any_busy = false;
         for (auto &sp : S->slots) any_busy |= sp->busy;
-        if (S->stop && S->queue.empty() && !any_busy) break;
+        if (S->stop && S->pq.empty() && S->ready.empty() && S->prefixing == 0 && !any_busy) break;
         if (any_busy) S->cv.wait_for(lk, kPollPeriod);
         else S->cv.wait(lk);
     }
+}
+
+void free_slot(Slot &sl)
+{
+    if (sl.st) (void)hipStreamSynchronize(sl.st);
+    dev_free_batch(sl.bs);
+    dev_free_scratch(sl.ds);
+    dev_bn_free_scratch(sl.bn);
+    if (sl.tail_host) (void)hipHostFree(sl.tail_host);
+    if (sl.small_host) (void)hipHostFree(sl.small_host);
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.st) (void)hipStreamDestroy(sl.st);
 }
 
 }  // namespace
@@ -289,16 +352,15 @@ extern "C" int gkrx_server_create(gkrx_ctx *ctx, uint64_t hbm_budget_bytes, uint
     if (!S) return GKRX_ERR_NOMEM;
     S->ctx = ctx;
     S->budget = hbm_budget_bytes;
-    for (uint32_t i = 0; i < slots; i++) {
+    // M31: `slots` batch slots and one single slot for large proofs; BN254: `slots` single slots
+    const uint32_t total = ctx->bn254 ? slots : slots + 1;
+    for (uint32_t i = 0; i < total; i++) {
         auto sl = std::make_unique<Slot>();
-        sl->srv = S;
+        sl->single = ctx->bn254 || i == slots;
         if (hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl->ev, hipEventDisableTiming) != hipSuccess) {
-            if (sl->st) (void)hipStreamDestroy(sl->st);
-            for (auto &p : S->slots) {
-                (void)hipEventDestroy(p->ev);
-                (void)hipStreamDestroy(p->st);
-            }
+            free_slot(*sl);
+            for (auto &p : S->slots) free_slot(*p);
             delete S;
             set_error("hipStreamCreate / hipEventCreate failed");
             return GKRX_ERR_DEVICE;
@@ -306,6 +368,8 @@ extern "C" int gkrx_server_create(gkrx_ctx *ctx, uint64_t hbm_budget_bytes, uint
         S->slots.push_back(std::move(sl));
     }
     ctx->live_circuits++;  // the context outlives the server
+    const unsigned nw = std::max(1u, std::min(12u, std::thread::hardware_concurrency()));
+    for (unsigned i = 0; i < nw; i++) S->workers.emplace_back(prefix_worker, S);
     S->sched = std::thread(scheduler, S);
     *out = S;
     return GKRX_OK;
@@ -324,43 +388,45 @@ extern "C" int gkrx_server_submit(gkrx_server *S, gkrx_circuit *c, uint8_t *proo
         set_error("proof buffer too small");
         return GKRX_ERR_ARG;
     }
-    if (!ctx->bn254 && !dev_batch_supported(c->dev)) {
-        set_error("layers wider than 2^20 are not served (prove them with gkrx_prove)");
-        return GKRX_ERR_CIRCUIT;
-    }
-    auto *r = new (std::nothrow) SrvReq;
-    if (!r) return GKRX_ERR_NOMEM;
-    r->c = c;
-    r->proof = proof;
-    r->claims = claimed_v;
-    r->bytes = footprint(ctx, c);
-    if (r->bytes > S->budget) {
-        const uint64_t need = r->bytes;
-        delete r;
-        set_error("request needs " + std::to_string(need) + " bytes of HBM, above the server budget");
+    uint64_t rows = 0;
+    for (uint32_t l = 0; l < c->D; l++) rows += 1ull << c->n[l];
+    const bool single = ctx->bn254 || rows >= kBigRows || !dev_batch_supported(c->dev);
+    const uint64_t bytes = footprint(ctx, c);
+    if (bytes > S->budget) {
+        set_error("request needs " + std::to_string(bytes) + " bytes of HBM, above the server budget");
         return GKRX_ERR_NOMEM;
     }
     bool expect = false;
     if (!c->in_server.compare_exchange_strong(expect, true)) {
-        delete r;
         set_error("circuit already has a request in flight");
         return GKRX_ERR_STATE;
     }
+    auto *r = new (std::nothrow) SrvReq;
+    if (!r) {
+        c->in_server = false;
+        return GKRX_ERR_NOMEM;
+    }
+    r->c = c;
+    r->proof = proof;
+    r->claims = claimed_v;
+    r->bytes = bytes;
+    r->single = single;
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(S->m);
         if (S->stop) {
             c->in_server = false;
             delete r;
+            set_error("server is shutting down");
             return GKRX_ERR_STATE;
         }
         r->id = S->next_id++;
         S->reqs[r->id] = r;
-        S->queue.push_back(r);
+        S->pq.push_back(r);
         S->submitted++;
         *ticket = r->id;
     }
-    S->cv.notify_all();
+    S->pcv.notify_one();
     return GKRX_OK;
 }
 
@@ -406,19 +472,12 @@ extern "C" int gkrx_server_destroy(gkrx_server *S)
         std::lock_guard<std::mutex> lk(S->m);
         S->stop = true;
     }
+    S->pcv.notify_all();
     S->cv.notify_all();
+    for (auto &w : S->workers) w.join();
     if (S->sched.joinable()) S->sched.join();
     (void)hipSetDevice(S->ctx->device);
-    for (auto &sp : S->slots) {
-        Slot &sl = *sp;
-        (void)hipStreamSynchronize(sl.st);
-        dev_free_batch(sl.bs);
-        dev_bn_free_scratch(sl.bn);
-        if (sl.bn_tail_host) (void)hipHostFree(sl.bn_tail_host);
-        if (sl.bn_claims_host) (void)hipHostFree(sl.bn_claims_host);
-        (void)hipEventDestroy(sl.ev);
-        (void)hipStreamDestroy(sl.st);
-    }
+    for (auto &sp : S->slots) free_slot(*sp);
     for (auto &kv : S->reqs) delete kv.second;
     S->ctx->live_circuits--;
     delete S;

@@ -27,7 +27,8 @@ def ctx():
     c.close()
 
 
-SPECS = [(100 + i, 2 + (i * 5) % 7, 6 + (i * 3) % 9) for i in range(14)]
+# small proofs (batch slots) and two large ones (2^17 x 4 and 2^18 x 2 rows: the single slot)
+SPECS = [(100 + i, 2 + (i * 5) % 7, 6 + (i * 3) % 9) for i in range(14)] + [(150, 4, 17), (151, 2, 18)]
 
 
 def test_server_results_equal_alone(ctx):
