@@ -26,6 +26,11 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch of the phase-1 fused k_round on the 24 x 2^20 M31x16 workload,
+# measured with two rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, per
+# dispatch) and scaled by what the same counters read for a 256 MiB device-to-device copy
+# (--calibration-copy-mb 256): bytes read = 2 x FETCH_SIZE KiB, written = 1 x WRITE_SIZE KiB.
+K_ROUND_TRAFFIC_BYTES = 145104239
 
 
 def log(msg):
@@ -295,6 +300,26 @@ def bn254_leg(gkrx, fs, depth, log_width, steps, warmup):
             "verified": bool(ok)}
 
 
+def calibration_copy(mb):
+    """Device-to-device copies of mb MiB (3 times) through the HIP runtime's copy kernel:
+    the known-size reference that scales the FETCH_SIZE / WRITE_SIZE counters of a
+    rocprofv3 --pmc run into bytes."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mb << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, n)
+    for _ in range(3):
+        hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration: 3 device copies of {mb} MiB")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,6 +337,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--bn254-layers", type=int, default=12, help="layers of the BN254 leg (0 = skip)")
     ap.add_argument("--bn254-log-width", type=int, default=18)
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="also run device copies of this size (for scaling rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (proofs shard by proof id "
@@ -319,6 +346,9 @@ def main():
         sys.exit(2)
 
     import gkrx
+
+    if args.calibration_copy_mb > 0:
+        calibration_copy(args.calibration_copy_mb)
 
     L, seed = 16, 1
     fs = gkrx.FS_KECCAK256 if args.hash == "keccak256" else gkrx.FS_SHA256
@@ -363,7 +393,9 @@ def main():
         avg_ms = st["ms"] / st["count"]
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": K_ROUND_TRAFFIC_BYTES if (D, k, L) == (24, 20, 16) else None,
+                "algorithmic_bytes_per_launch": int(per_launch),
                 "kernel": "k_round<16,phase1,fused>", "avg_launch_us": round(avg_ms * 1e3, 2),
                 "launches": st["count"]}
     ctx.close()

@@ -311,6 +311,25 @@ extern "C" int gkrx_circuit_load_text(gkrx_ctx *ctx, const char *mul_path, const
     return GKRX_OK;
 }
 
+extern "C" int gkrx_circuit_load_circuit2(gkrx_ctx *ctx, const char *path, int coef_mode, gkrx_circuit **out)
+{
+    if (!ctx || !path || !out) return GKRX_ERR_ARG;
+    auto *c = new (std::nothrow) gkrx_circuit;
+    if (!c) return GKRX_ERR_NOMEM;
+    int rc = circuit2_flatten(path, ctx->cfg.field_type, coef_mode, c->mul, c->add);
+    if (!rc) {
+        c->D = (uint32_t)c->mul.size();
+        rc = finish_circuit(ctx, c);
+    }
+    if (rc) {
+        dev_free_circuit(c->dev);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return GKRX_OK;
+}
+
 extern "C" int gkrx_circuit_shape(const gkrx_circuit *c, uint32_t *nb_layers, uint32_t *nb_in_vars,
                                   uint32_t *nb_out_vars)
 {

@@ -176,6 +176,11 @@ inline uint64_t dev_circuit_bytes(const DevCircuit &dc) { return 4 * (dc.values_
 int dev_batch_supported(const DevCircuit &dc);
 void dev_free_batch(BatchScratch &bs);
 
+// CIRCUIT2 binary circuit -> per-layer gate lists (circuit2.cpp); mul entries
+// (in0, in1, out, coef), add entries (in0, out, coef), layer 0 first.
+int circuit2_flatten(const char *path, int field_type, int coef_mode, std::vector<std::vector<uint32_t>> &mul,
+                     std::vector<std::vector<uint32_t>> &add);
+
 // BN254 entry points (prover_bn254.hip).  A BN254 DevCircuit is uploaded with L = 8: one
 // entry is 8 u32 words.
 int dev_bn_to_mont(uint32_t *values, uint64_t n, hipStream_t st);

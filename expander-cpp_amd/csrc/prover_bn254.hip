@@ -26,6 +26,7 @@
 #include "bn254.hpp"
 #include "hash_dev.hpp"
 #include "internal.hpp"
+#include "xcd.hpp"
 
 namespace gkrx {
 
@@ -82,7 +83,8 @@ __global__ void __launch_bounds__(kBnThreads) k_bn_forward(const uint32_t *__res
                                                            const uint32_t *__restrict__ ptr,
                                                            const uint32_t *__restrict__ ent, uint32_t M)
 {
-    for (uint64_t o = (uint64_t)blockIdx.x * kBnThreads + threadIdx.x; o < M; o += (uint64_t)gridDim.x * kBnThreads) {
+    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    for (uint64_t o = tile * kBnThreads + threadIdx.x; o < M; o += (uint64_t)gridDim.x * kBnThreads) {
         Fr acc = fr_zero();
         const uint32_t e1 = ptr[o + 1];
         for (uint32_t e = ptr[o]; e < e1; e++) {
@@ -159,7 +161,8 @@ __global__ void __launch_bounds__(kBnThreads) k_bn_gx(uint32_t *__restrict__ hg,
                                                       const uint32_t *__restrict__ eqz, const uint32_t *__restrict__ ptr,
                                                       const uint32_t *__restrict__ ent, uint32_t N)
 {
-    for (uint64_t x = (uint64_t)blockIdx.x * kBnThreads + threadIdx.x; x < N; x += (uint64_t)gridDim.x * kBnThreads) {
+    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    for (uint64_t x = tile * kBnThreads + threadIdx.x; x < N; x += (uint64_t)gridDim.x * kBnThreads) {
         Fr acc = fr_zero();
         const uint32_t e1 = ptr[x + 1];
         for (uint32_t e = ptr[x]; e < e1; e++) {
@@ -177,7 +180,8 @@ __global__ void __launch_bounds__(kBnThreads) k_bn_hy(uint32_t *__restrict__ s, 
                                                       const uint32_t *__restrict__ eqx, const uint32_t *__restrict__ ptr,
                                                       const uint32_t *__restrict__ ent, uint32_t N)
 {
-    for (uint64_t y = (uint64_t)blockIdx.x * kBnThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kBnThreads) {
+    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    for (uint64_t y = tile * kBnThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kBnThreads) {
         Fr acc = fr_zero();
         const uint32_t e1 = ptr[y + 1];
         for (uint32_t e = ptr[y]; e < e1; e++) {

@@ -30,6 +30,7 @@
 #include "hash_dev.hpp"
 #include "internal.hpp"
 #include "m31.hpp"
+#include "xcd.hpp"
 
 namespace gkrx {
 
@@ -67,7 +68,8 @@ __global__ void __launch_bounds__(kThreads) k_forward(const uint32_t *__restrict
 {
     constexpr uint32_t LG = L / 4;
     const uint64_t total = (uint64_t)M * LG;
-    for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    for (uint64_t w = tile * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
         const uint32_t o = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
         uint4 acc = make_uint4(0, 0, 0, 0);
         const uint32_t e1 = ptr[o + 1];
@@ -148,7 +150,8 @@ __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint
 {
     constexpr uint32_t LG = L / 4;
     const uint64_t total = (uint64_t)N * LG;
-    for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    for (uint64_t w = tile * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
         const uint32_t x = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
         uint4 acc[3] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
         const uint32_t e1 = ptr[x + 1];
@@ -180,7 +183,8 @@ __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint6
                                                  const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
                                                  uint32_t N, uint32_t R)
 {
-    for (uint64_t y = (uint64_t)blockIdx.x * kThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kThreads) {
+    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    for (uint64_t y = tile * kThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kThreads) {
         uint32_t acc[3] = {0, 0, 0};
         const uint32_t e1 = ptr[y + 1];
         for (uint32_t e = ptr[y]; e < e1; e++) {

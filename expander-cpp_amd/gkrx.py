@@ -55,6 +55,8 @@ def lib():
     L.gkrx_circuit_from_arrays.argtypes = [vp, C.c_uint32, _u32p, C.POINTER(C.c_void_p), _u32p,
                                            C.POINTER(C.c_void_p), C.POINTER(vp)]
     L.gkrx_circuit_shape.argtypes = [vp, C.POINTER(C.c_uint32), vp, vp]
+    L.gkrx_circuit_load_circuit2.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(vp)]
+    L.gkrx_circuit2_to_text.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_char_p]
     L.gkrx_circuit_write_text.argtypes = [vp, C.c_char_p, C.c_char_p]
     L.gkrx_circuit_set_witness.argtypes = [vp, _u32p, C.c_size_t]
     L.gkrx_circuit_evaluate.argtypes = [vp]
@@ -137,6 +139,65 @@ def write_text(layers, mul_path: str, add_path: str) -> None:
         for mul, add in reversed(layers):
             fm.write(" ".join([str(len(mul))] + [str(int(x)) for x in np.asarray(mul).ravel()]) + "\n")
             fa.write(" ".join([str(len(add))] + [str(int(x)) for x in np.asarray(add).ravel()]) + "\n")
+
+
+CIRCUIT2_MAGIC = 3626604230490605891  # b"CIRCUIT2" little-endian
+
+
+def write_circuit2(path: str, segments, layers, rand_sentinel: int = 0) -> None:
+    """Write a CIRCUIT2 binary circuit (the layout CircuitRaw operator<< writes,
+    include/circuit/circuit_raw.hpp:205-291, 514-537).  segments: list of dicts with keys
+    i_len, o_len, children [(segment id, [(i_offset, o_offset), ...])], mul [(in0, in1, out,
+    coef)], add [(in0, out, coef)], const [(out, coef)], rand [gate indices with a random
+    coefficient, counted over mul, then add, then const]; layers: segment id per layer.
+    Field elements are written as 32 little-endian bytes."""
+    import struct
+
+    def u64(x):
+        return struct.pack("<Q", int(x))
+
+    def fe(x):
+        return int(x).to_bytes(32, "little")
+
+    out = [u64(CIRCUIT2_MAGIC), u64(len(segments))]
+    for s in segments:
+        out += [u64(s["i_len"]), u64(s["o_len"]), u64(len(s.get("children", [])))]
+        for sid, allocs in s.get("children", []):
+            out += [u64(sid), u64(len(allocs))] + [u64(v) for a in allocs for v in a]
+        muls, adds, consts = s.get("mul", []), s.get("add", []), s.get("const", [])
+        out.append(u64(len(muls)))
+        for g in muls:
+            out += [u64(g[0]), u64(g[1]), u64(g[2]), fe(g[3])]
+        out.append(u64(len(adds)))
+        for g in adds:
+            out += [u64(g[0]), u64(g[1]), fe(g[2])]
+        out.append(u64(len(consts)))
+        for g in consts:
+            out += [u64(g[0]), fe(g[1])]
+        rand = s.get("rand", [])
+        out += [u64(len(rand))] + [u64(i) for i in rand]
+    out += [u64(len(layers))] + [u64(l) for l in layers] + [fe(rand_sentinel)]
+    with open(path, "wb") as f:
+        f.write(b"".join(out))
+
+
+def circuit2_to_text(path: str, mul_path: str, add_path: str, field_type: int = FIELD_M31,
+                     coef_mode: int = 0) -> None:
+    """Flatten a CIRCUIT2 file into the text format with no GPU (gkrx_circuit2_to_text)."""
+    _check(lib().gkrx_circuit2_to_text(path.encode(), field_type, coef_mode, mul_path.encode(), add_path.encode()),
+           "gkrx_circuit2_to_text")
+
+
+def read_text(mul_path: str, add_path: str):
+    """The text format back into (mul[u,4], add[a,3]) per layer, layer 0 first."""
+    def parse(p, w):
+        lines = [ln for ln in open(p).read().split("\n") if ln.strip()]
+        out = []
+        for ln in lines:
+            t = np.array([int(x) for x in ln.split()], dtype=np.int64)
+            out.append(t[1:].reshape(-1, w).astype(np.uint32))
+        return out[::-1]
+    return list(zip(parse(mul_path, 4), parse(add_path, 3)))
 
 
 def kernel_stats(ctx: "Context") -> dict:
@@ -238,6 +299,15 @@ class Circuit:
         h = C.c_void_p()
         _check(lib().gkrx_circuit_load_text(ctx._h, mul_path.encode(), add_path.encode(), C.byref(h)),
                "gkrx_circuit_load_text")
+        return cls(ctx, h)
+
+    @classmethod
+    def load_circuit2(cls, ctx: Context, path: str, coef_mode: int = 0) -> "Circuit":
+        """CircuitRaw >> + Circuit::from_circuit_raw (circuit_raw.hpp:539-592,
+        circuit.hpp:217-251).  coef_mode 0: every coefficient 1, as the reference reads them."""
+        h = C.c_void_p()
+        _check(lib().gkrx_circuit_load_circuit2(ctx._h, path.encode(), coef_mode, C.byref(h)),
+               "gkrx_circuit_load_circuit2")
         return cls(ctx, h)
 
     @classmethod

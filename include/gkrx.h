@@ -72,6 +72,17 @@ int gkrx_circuit_from_arrays(gkrx_ctx *ctx, uint32_t nb_layers, const uint32_t *
                              const uint32_t *n_add, const uint32_t *const *add, gkrx_circuit **out);
 /* CircuitLayer::nb_input_vars / nb_output_vars, per layer (arrays of nb_layers). */
 int gkrx_circuit_shape(const gkrx_circuit *c, uint32_t *nb_layers, uint32_t *nb_in_vars, uint32_t *nb_out_vars);
+/* The CIRCUIT2 binary format: CircuitRaw operator>> (include/circuit/circuit_raw.hpp:539-592)
+ * followed by Circuit::from_circuit_raw (include/circuit/circuit.hpp:217-251), which flattens
+ * the segment tree into layers and drops constant gates.  coef_mode 0 reproduces the
+ * reference, whose element reader sets every coefficient read from such a file to 1
+ * (include/field/M31.hpp:124-135); coef_mode 1 reads each 32-byte coefficient as a
+ * little-endian integer reduced into the field. */
+int gkrx_circuit_load_circuit2(gkrx_ctx *ctx, const char *path, int coef_mode, gkrx_circuit **out);
+/* The same flattening with no context and no device: writes the circuit in the text format
+ * (Circuit::load_extracted_gates, circuit.hpp:152-215). */
+int gkrx_circuit2_to_text(const char *path, int field_type, int coef_mode, const char *mul_path,
+                          const char *add_path);
 /* Writes the circuit in the text format read by load_extracted_gates. */
 int gkrx_circuit_write_text(const gkrx_circuit *c, const char *mul_path, const char *add_path);
 /* layers[0].input_layer_vals.evals = witness: 2^nb_input_vars(0) elements x L lanes (host memory). */

@@ -7,6 +7,8 @@
 //   ref_driver prove  <mul.txt> <add.txt> <witness.bin> <proof.out> <claimed.out> [reps]
 //   ref_driver verify <mul.txt> <add.txt> <claimed.bin> <proof.bin>     (exit 0 = accepted)
 //   ref_driver time   <mul.txt> <add.txt> <witness.bin>                 (prints seconds of prove)
+//   ref_driver flatten <circuit2> <mul.out> <add.out>   (binary CIRCUIT2 -> flattened text format)
+//   ref_driver prove2  <circuit2> <witness.bin> <proof.out> <claimed.out>
 //
 // Build variants (oracle/Makefile):
 //   * as shipped: VectorizedM31 = 8 lanes, SHA-256 transcript (include/hash/hashes.hpp:22-31);
@@ -16,6 +18,7 @@
 //   * 16 lanes: the recipe rewrites `vectorize_size = 1` to 2 (include/field/M31.hpp:237)
 //     in the preprocessed translation unit, in a pipe, without copying any source.
 #include <chrono>
+#include <fstream>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -72,10 +75,62 @@ static void set_witness(gkr::Circuit<F, FP> &c, const std::vector<unsigned char>
     for (size_t i = 0; i < N0; i++) ev[i].from_bytes(w.data() + i * sizeof(F));
 }
 
+// The gates of a loaded circuit in the text format (output layer first, circuit.hpp:152-215).
+static void write_text(const gkr::Circuit<F, FP> &c, const char *mul_path, const char *add_path)
+{
+    FILE *fm = fopen(mul_path, "w"), *fa = fopen(add_path, "w");
+    if (!fm || !fa) { fprintf(stderr, "cannot write text files\n"); exit(2); }
+    for (size_t i = 0; i < c.layers.size(); i++) {
+        const auto &layer = c.layers[c.layers.size() - 1 - i];
+        fprintf(fm, "%zu", layer.mul.sparse_evals.size());
+        for (const auto &g : layer.mul.sparse_evals)
+            fprintf(fm, " %u %u %u %u", g.i_ids[0], g.i_ids[1], g.o_id, (unsigned)g.coef.x);
+        fputc('\n', fm);
+        fprintf(fa, "%zu", layer.add.sparse_evals.size());
+        for (const auto &g : layer.add.sparse_evals) fprintf(fa, " %u %u %u", g.i_ids[0], g.o_id, (unsigned)g.coef.x);
+        fputc('\n', fa);
+    }
+    fclose(fm);
+    fclose(fa);
+}
+
+// CircuitRaw operator>> (circuit_raw.hpp:539-592) and Circuit::from_circuit_raw (circuit.hpp:217-251)
+static gkr::Circuit<F, FP> load_circuit2(const char *path)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in) { fprintf(stderr, "cannot open %s\n", path); exit(2); }
+    CircuitRaw<FP> raw;
+    in >> raw;
+    return gkr::Circuit<F, FP>::from_circuit_raw(raw);
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) return 2;
     std::string cmd = argv[1];
+    //   ref_driver flatten <circuit2> <mul.out> <add.out>
+    //   ref_driver prove2  <circuit2> <witness.bin> <proof.out> <claimed.out>
+    if (cmd == "flatten" || cmd == "prove2") {
+        if ((cmd == "flatten" && argc < 5) || (cmd == "prove2" && argc < 6)) return 2;
+        auto circuit = load_circuit2(argv[2]);
+        if (cmd == "flatten") {
+            write_text(circuit, argv[3], argv[4]);
+            return 0;
+        }
+        set_witness(circuit, read_file(argv[3]));
+        circuit.evaluate();
+        Config config;
+        gkr::Prover<F, FP> prover(config);
+        prover.prepare_mem(circuit);
+        auto res = prover.prove(circuit);
+        auto &proof = std::get<1>(res);
+        auto &claimed = std::get<0>(res);
+        write_file(argv[4], proof.bytes.data(), proof.bytes.size());
+        std::vector<unsigned char> cb(claimed.size() * sizeof(F));
+        for (size_t j = 0; j < claimed.size(); j++) claimed[j].to_bytes(cb.data() + j * sizeof(F));
+        write_file(argv[5], cb.data(), cb.size());
+        return 0;
+    }
     if ((cmd == "prove" && argc < 7) || (cmd == "verify" && argc < 6) || (cmd == "time" && argc < 5)) {
         fprintf(stderr, "bad arguments\n");
         return 2;
