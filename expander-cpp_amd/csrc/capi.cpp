@@ -729,6 +729,61 @@ extern "C" int gkrx_verify_arrays(const gkrx_config *cfg, uint32_t nb_layers, co
                        proof, proof_len);
 }
 
+// Proof::write_and_compress / read_proof file layout (include/fiat_shamir/transcript.hpp:54-85)
+extern "C" int gkrx_proof_write(const char *path, const uint8_t *proof, size_t proof_len)
+{
+    if (!path || (!proof && proof_len)) return GKRX_ERR_ARG;
+    if (proof_len > 0xffffffffull) {
+        set_error("a proof file holds at most 2^32 - 1 bytes");
+        return GKRX_ERR_ARG;
+    }
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_error(std::string("cannot write ") + path);
+        return GKRX_ERR_IO;
+    }
+    const uint32_t n = (uint32_t)proof_len;
+    const bool ok = fwrite(&n, 4, 1, f) == 1 && (proof_len == 0 || fwrite(proof, 1, proof_len, f) == proof_len);
+    if (fclose(f) != 0 || !ok) {
+        set_error(std::string("write failed: ") + path);
+        return GKRX_ERR_IO;
+    }
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_proof_read(const char *path, uint8_t *proof, size_t proof_cap, size_t *proof_len)
+{
+    if (!path || !proof_len) return GKRX_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        set_error(std::string("cannot open ") + path);
+        return GKRX_ERR_IO;
+    }
+    uint32_t n = 0;
+    if (fread(&n, 4, 1, f) != 1) {
+        fclose(f);
+        set_error("truncated proof file");
+        return GKRX_ERR_IO;
+    }
+    *proof_len = n;
+    if (!proof) {
+        fclose(f);
+        return GKRX_OK;
+    }
+    if (proof_cap < n) {
+        fclose(f);
+        set_error("proof buffer too small");
+        return GKRX_ERR_ARG;
+    }
+    const bool ok = fread(proof, 1, n, f) == n;
+    fclose(f);
+    if (!ok) {
+        set_error("truncated proof file");
+        return GKRX_ERR_IO;
+    }
+    return GKRX_OK;
+}
+
 extern "C" int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3)
 {
     if (!ctx || !ms3) return GKRX_ERR_ARG;

@@ -57,6 +57,8 @@ def lib():
     L.gkrx_circuit_shape.argtypes = [vp, C.POINTER(C.c_uint32), vp, vp]
     L.gkrx_circuit_load_circuit2.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(vp)]
     L.gkrx_circuit2_to_text.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_char_p]
+    L.gkrx_proof_write.argtypes = [C.c_char_p, _u8p, C.c_size_t]
+    L.gkrx_proof_read.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.gkrx_circuit_write_text.argtypes = [vp, C.c_char_p, C.c_char_p]
     L.gkrx_circuit_set_witness.argtypes = [vp, _u32p, C.c_size_t]
     L.gkrx_circuit_evaluate.argtypes = [vp]
@@ -473,3 +475,18 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def proof_write(path: str, proof) -> None:
+    """Proof file: [u32 byte count][bytes] (Proof::write_and_compress / read_proof,
+    include/fiat_shamir/transcript.hpp:54-85)."""
+    b = np.ascontiguousarray(np.frombuffer(bytes(proof), dtype=np.uint8))
+    _check(lib().gkrx_proof_write(path.encode(), b, b.size), "gkrx_proof_write")
+
+
+def proof_read(path: str) -> bytes:
+    n = C.c_size_t()
+    _check(lib().gkrx_proof_read(path.encode(), None, 0, C.byref(n)), "gkrx_proof_read")
+    buf = np.empty(max(n.value, 1), dtype=np.uint8)
+    _check(lib().gkrx_proof_read(path.encode(), buf.ctypes.data, buf.size, C.byref(n)), "gkrx_proof_read")
+    return bytes(buf[:n.value])

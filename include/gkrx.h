@@ -122,6 +122,14 @@ int gkrx_verify_arrays(const gkrx_config *cfg, uint32_t nb_layers, const uint32_
                        const uint32_t *n_add, const uint32_t *const *add, const uint32_t *claimed_v,
                        const uint8_t *proof, size_t proof_len);
 
+/* ---- Proof files: the layout Proof::write_and_compress writes before compressing and
+ * Proof::read_proof reads (include/fiat_shamir/transcript.hpp:54-85): a little-endian u32
+ * byte count, then the proof bytes.  gkrx_proof_read with proof = NULL reports the size in
+ * *proof_len.  (The reference then runs `tar -czf` on the file; compression is left to the
+ * caller.) */
+int gkrx_proof_write(const char *path, const uint8_t *proof, size_t proof_len);
+int gkrx_proof_read(const char *path, uint8_t *proof, size_t proof_cap, size_t *proof_len);
+
 /* ---- Serving.  No reference counterpart (the reference proves one circuit per
  * Prover::prove call): a continuous-batching server for callers that prove a stream of
  * requests.  Requests queue in arrival order and are admitted while the HBM they need

@@ -427,6 +427,25 @@ class Proof {
         step(sizeof(F));
         return f;
     }
+    // transcript.hpp:54-85: [u32 byte count][bytes]; the compressed copy is made with tar as
+    // the reference does
+    void write_and_compress(const char *proof_loc = "/tmp/proof.txt",
+                            const char *compressed_proof_loc = "/tmp/compressed_proof.tar.gz")
+    {
+        if (gkrx_proof_write(proof_loc, bytes.data(), bytes.size()) != GKRX_OK) std::abort();
+        if (compressed_proof_loc) {
+            const std::string cmd = std::string("tar -czf ") + compressed_proof_loc + " --absolute-names " + proof_loc;
+            if (std::system(cmd.c_str()) != 0) std::abort();
+        }
+    }
+    void read_proof(const char *proof_loc = "/tmp/proof.txt")
+    {
+        size_t n = 0;
+        if (gkrx_proof_read(proof_loc, nullptr, 0, &n) != GKRX_OK) std::abort();
+        bytes.resize(n);
+        if (gkrx_proof_read(proof_loc, bytes.data(), bytes.size(), &n) != GKRX_OK) std::abort();
+        idx = 0;
+    }
 };
 
 // LinearGKR.hpp:34-91

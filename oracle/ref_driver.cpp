@@ -9,6 +9,8 @@
 //   ref_driver time   <mul.txt> <add.txt> <witness.bin>                 (prints seconds of prove)
 //   ref_driver flatten <circuit2> <mul.out> <add.out>   (binary CIRCUIT2 -> flattened text format)
 //   ref_driver prove2  <circuit2> <witness.bin> <proof.out> <claimed.out>
+//   ref_driver proof_write <raw.bin> <proof.out> <compressed.out>  (Proof::write_and_compress)
+//   ref_driver proof_read  <proof.in> <raw.out>                     (Proof::read_proof)
 //
 // Build variants (oracle/Makefile):
 //   * as shipped: VectorizedM31 = 8 lanes, SHA-256 transcript (include/hash/hashes.hpp:22-31);
@@ -108,8 +110,18 @@ int main(int argc, char **argv)
 {
     if (argc < 2) return 2;
     std::string cmd = argv[1];
-    //   ref_driver flatten <circuit2> <mul.out> <add.out>
-    //   ref_driver prove2  <circuit2> <witness.bin> <proof.out> <claimed.out>
+    if (cmd == "proof_write" && argc >= 5) {
+        gkr::Proof<F> p;
+        p.bytes = read_file(argv[2]);
+        p.write_and_compress(argv[3], argv[4]);
+        return 0;
+    }
+    if (cmd == "proof_read" && argc >= 4) {
+        gkr::Proof<F> p;
+        p.read_proof(argv[2]);
+        write_file(argv[3], p.bytes.data(), p.bytes.size());
+        return 0;
+    }
     if (cmd == "flatten" || cmd == "prove2") {
         if ((cmd == "flatten" && argc < 5) || (cmd == "prove2" && argc < 6)) return 2;
         auto circuit = load_circuit2(argv[2]);
