@@ -101,8 +101,8 @@ def batch_specs(count, seed=3):
 
 
 def batch_leg(gkrx, fs, count, steps, warmup):
-    """Batched proving (one launch for the whole batch) against the plain path (gkrx_prove
-    once per proof) on the same uneven batch of M31x16 proofs."""
+    """Batched proving (the autotuned batch engine) against the plain path (gkrx_prove once
+    per proof) on the same uneven batch of M31x16 proofs."""
     from concurrent.futures import ThreadPoolExecutor
 
     L = 16
@@ -124,12 +124,18 @@ def batch_leg(gkrx, fs, count, steps, warmup):
     log(f"batch of {count} proofs, {gates} gates, built in {time.perf_counter() - t0:.1f}s")
     prover = gkrx.Prover(ctx)
     outs = [np.empty(prover.proof_size(c), dtype=np.uint8) for c in circs]
+    # autotuner: every batch engine and block size on this batch's device work
+    engine, block, tune_ms = ctx.batch_autotune(circs)
+    names = {gkrx.BATCH_PER_PROOF: "per_proof", gkrx.BATCH_LOCKSTEP: "lockstep"}
+    tuned = {f"{names[e]}_{b}": round(v, 3) for (e, b), v in tune_ms.items()}
+    log(f"autotune: {tuned} -> {names[engine]} block {block}")
     for _ in range(warmup):
         prover.prove_batch(circs, outs)
     t0 = time.perf_counter()
     for _ in range(steps):
         res = prover.prove_batch(circs, outs)
     batched_s = (time.perf_counter() - t0) / steps
+    split = prover.last_timing()
     # the plain path on the same proofs, one gkrx_prove per proof; its proofs must equal
     # the batched ones
     plain_lat = []
@@ -152,7 +158,9 @@ def batch_leg(gkrx, fs, count, steps, warmup):
            "plain_gates_per_s": round(gates / plain_s, 1), "plain_ms": round(plain_s * 1e3, 3),
            "ratio": round(plain_s / batched_s, 3),
            "plain_latency_ms": {"p50": round(percentile(plain_lat, 50), 3), "p99": round(percentile(plain_lat, 99), 3)},
-           "identical_to_plain": same, "verified_sample": ok}
+           "identical_to_plain": same, "verified_sample": ok,
+           "engine": names[engine], "block": block, "autotune_device_ms": tuned,
+           "batched_split_ms": {k: round(v, 3) for k, v in split.items()}}
     # the context and circuits stay resident for the serving leg (the caller closes ctx)
     return out, (ctx, circs, [g for _, g in made], batched_s / count)
 

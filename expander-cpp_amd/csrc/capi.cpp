@@ -689,6 +689,97 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
     return GKRX_OK;
 }
 
+extern "C" int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block)
+{
+    if (!ctx) return GKRX_ERR_ARG;
+    if ((engine != GKRX_BATCH_PER_PROOF && engine != GKRX_BATCH_LOCKSTEP) ||
+        (block != 64 && block != 128 && block != 256 && block != 512)) {
+        set_error("batch engine must be GKRX_BATCH_PER_PROOF or GKRX_BATCH_LOCKSTEP with block 64, 128, 256 or 512");
+        return GKRX_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->batch.engine = engine;
+    ctx->batch.block = block;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_ctx_get_batch_engine(const gkrx_ctx *ctx, int *engine, uint32_t *block)
+{
+    if (!ctx) return GKRX_ERR_ARG;
+    if (engine) *engine = ctx->batch.engine;
+    if (block) *block = ctx->batch.block;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, int *engine,
+                                   uint32_t *block, double *ms_each)
+{
+    if (!ctx || n == 0 || !circuits) return GKRX_ERR_ARG;
+    if (ctx->bn254) {
+        set_error("the batch engines prove M31 circuits");
+        return GKRX_ERR_CONFIG;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        gkrx_circuit *c = circuits[i];
+        if (!c || c->ctx != ctx) return GKRX_ERR_ARG;
+        if (c->in_server || !c->has_witness || !dev_batch_supported(c->dev)) {
+            set_error("autotune item " + std::to_string(i) + " has no witness, is in a server or is too wide");
+            return GKRX_ERR_STATE;
+        }
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    GKRX_HIP_CHECK(hipSetDevice(ctx->device));
+    const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
+    // placeholder transcript prefixes: the device work of a proof does not depend on them
+    const uint32_t ts[10] = {32, 32, 0, 0, 0, 0, 0, 0, 0, 0};
+    const uint8_t grind[32] = {0};
+    std::vector<std::vector<uint32_t>> rz(n);
+    std::vector<std::vector<uint8_t>> tails(n);
+    std::vector<BatchItem> items(n);
+    for (uint32_t i = 0; i < n; i++) {
+        gkrx_circuit *c = circuits[i];
+        rz[i].assign((size_t)ctx->R * c->m[c->D - 1], 1u);
+        tails[i].resize(tail_bytes(ctx, c));
+        items[i] = BatchItem{&c->dev, rz[i].data(), ts, grind, tails[i].size(), tails[i].data()};
+    }
+    struct Cand {
+        int engine;
+        uint32_t block;
+    };
+    const Cand cands[5] = {{GKRX_BATCH_PER_PROOF, 256}, {GKRX_BATCH_LOCKSTEP, 64}, {GKRX_BATCH_LOCKSTEP, 128},
+                           {GKRX_BATCH_LOCKSTEP, 256}, {GKRX_BATCH_LOCKSTEP, 512}};
+    const int saved_engine = ctx->batch.engine;
+    const uint32_t saved_block = ctx->batch.block;
+    double best = 1e300;
+    Cand pick{saved_engine, saved_block};
+    for (int k = 0; k < 5; k++) {
+        ctx->batch.engine = cands[k].engine;
+        ctx->batch.block = cands[k].block;
+        double ms = 0;
+        for (int rep = 0; rep < 2; rep++) {  // the first run allocates and loads code
+            const auto t0 = std::chrono::steady_clock::now();
+            int rc = dev_prove_batch(items, ctx->R, kind, ctx->batch, ctx->stream, nullptr);
+            if (rc == GKRX_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = GKRX_ERR_DEVICE;
+            if (rc) {
+                ctx->batch.engine = saved_engine;
+                ctx->batch.block = saved_block;
+                return rc;
+            }
+            ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        if (ms_each) ms_each[k] = ms;
+        if (ms < best) {
+            best = ms;
+            pick = cands[k];
+        }
+    }
+    ctx->batch.engine = pick.engine;
+    ctx->batch.block = pick.block;
+    if (engine) *engine = pick.engine;
+    if (block) *block = pick.block;
+    return GKRX_OK;
+}
+
 extern "C" int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v, const uint8_t *proof,
                            size_t proof_len)
 {

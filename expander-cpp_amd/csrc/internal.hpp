@@ -64,13 +64,21 @@ struct BatchItem {
     uint8_t *tail_out;       // host destination of the tail
 };
 
+// Batched prover engines: the whole proof in one workgroup (one launch per batch), or
+// the lockstep engine (one launch per step of every proof, each proof's step on as many
+// workgroups as its tables need).
+enum BatchEngine : int { kBatchPerProof = 1, kBatchLockstep = 2 };
+
 // Device scratch of the batched prover, grown to the largest batch seen.
 struct BatchScratch {
+    int engine = kBatchLockstep;
+    uint32_t block = 256;           // workgroup size of the lockstep work kernel: 64, 128, 256 or 512
     uint32_t *pool = nullptr;
     uint64_t pool_u32 = 0;
-    uint8_t *meta = nullptr;        // job table + order
+    uint8_t *meta = nullptr;        // job table, order or step schedule
     uint64_t meta_bytes = 0;
-    uint8_t *host_meta = nullptr;   // pinned staging of the job table and seeds
+    uint8_t *host_meta = nullptr;   // pinned staging of the metadata and seeds
+    uint64_t host_meta_bytes = 0;
     uint32_t *claims_host = nullptr;
     uint64_t claims_cap = 0;
     // pinned staging of the proof tails: the device copies land here (a copy into pageable
@@ -161,7 +169,7 @@ int dev_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf);
 int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, const uint32_t *rz1_host,
               const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,
               Profiler *pf);
-// Proves every item in one launch (one workgroup per proof).  Enqueues the copies of the
+// Proves every item with the engine bs.engine selects.  Enqueues the copies of the
 // tails and of the claims (item-major, R * L each, to bs.host_claims()) into pinned
 // staging; once the stream has reached them, dev_batch_collect copies each tail to
 // items[i].tail_out.

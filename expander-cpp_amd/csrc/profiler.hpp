@@ -23,6 +23,8 @@ enum KernelClass : int {
     kKTailPh2,
     kKBatch,
     kKRoundFin,
+    kKLsWork,
+    kKLsFin,
     kKClassCount
 };
 
@@ -31,7 +33,8 @@ inline const char *kernel_class_name(int c)
     static const char *names[kKClassCount] = {"k_forward",      "k_eq",      "k_gx",           "k_hy",
                                               "k_dot",          "k_round_ph1_first", "k_round_ph1",
                                               "k_round_ph2_first", "k_round_ph2", "k_tail_ph1",
-                                              "k_tail_ph2",     "k_prove_batch", "k_round_fin"};
+                                              "k_tail_ph2",     "k_prove_batch", "k_round_fin",
+                                              "k_ls_work",      "k_ls_fin"};
     return (c >= 0 && c < kKClassCount) ? names[c] : "";
 }
 

@@ -61,6 +61,25 @@ __device__ __forceinline__ uint4 fold4(uint4 a, uint4 b, uint32_t r)
 }
 
 // ------------------------------------------------------------------ forward pass
+// Work item w: output o = w / LG, lanes 4g .. 4g+3 (g = w % LG).
+template <int L>
+__device__ __forceinline__ void forward_item(const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
+                                             const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
+                                             uint64_t w)
+{
+    constexpr uint32_t LG = L / 4;
+    const uint32_t o = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    const uint32_t e1 = ptr[o + 1];
+    for (uint32_t e = ptr[o]; e < e1; e++) {
+        const uint32_t a = ent[3 * (uint64_t)e], b = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+        uint4 va = ld4(vin + (uint64_t)a * L + 4 * g);
+        if (b != kAddMark) va = mul4(va, ld4(vin + (uint64_t)b * L + 4 * g));
+        acc = add4(acc, mul4s(va, c));
+    }
+    st4(vout + (uint64_t)o * L + 4 * g, acc);
+}
+
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_forward(const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
                                                       const uint32_t *__restrict__ ptr,
@@ -69,38 +88,30 @@ __global__ void __launch_bounds__(kThreads) k_forward(const uint32_t *__restrict
     constexpr uint32_t LG = L / 4;
     const uint64_t total = (uint64_t)M * LG;
     const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    for (uint64_t w = tile * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
-        const uint32_t o = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
-        uint4 acc = make_uint4(0, 0, 0, 0);
-        const uint32_t e1 = ptr[o + 1];
-        for (uint32_t e = ptr[o]; e < e1; e++) {
-            const uint32_t a = ent[3 * (uint64_t)e], b = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-            uint4 va = ld4(vin + (uint64_t)a * L + 4 * g);
-            if (b != kAddMark) va = mul4(va, ld4(vin + (uint64_t)b * L + 4 * g));
-            acc = add4(acc, mul4s(va, c));
-        }
-        st4(vout + (uint64_t)o * L + 4 * g, acc);
-    }
+    for (uint64_t w = tile * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
+        forward_item<L>(vin, vout, ptr, ent, w);
 }
 
 // ------------------------------------------------------------------ eq tables
 // out[rep][z] = f1 * eq(r1_rep, z) + f2 * eq(r2_rep, z), z < 2^m, with
 // eq(r, z) = prod_i (z_i ? r_i : 1 - r_i) (the reference's _eq at y = 0 / 1).
 // f1, f2 come from ab[0], ab[1] when ab is given, else f1 = 1 and the second term is absent.
+// A block of BS threads writes z in [chunk * kEqChunk * BS, (chunk + 1) * kEqChunk * BS):
+// the low log2(BS) bits of z index a table of BS partial products, the high bits one of
+// kEqChunk, both in LDS.
 constexpr int kEqChunk = 16;  // z values per thread
-__global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, uint64_t out_rs,
-                                                 const uint32_t *__restrict__ chal, uint32_t r1_off, uint32_t r2_off,
-                                                 const uint32_t *__restrict__ ab, uint32_t m)
+template <int BS>
+__device__ void eq_block(uint32_t *__restrict__ o, const uint32_t *__restrict__ r1, const uint32_t *__restrict__ r2,
+                         const uint32_t *__restrict__ ab, uint32_t m, uint32_t chunk)
 {
-    __shared__ uint32_t lo1[256], lo2[256], hi1[kEqChunk], hi2[kEqChunk];
-    const uint32_t rep = blockIdx.y;
-    const uint32_t *r1 = chal + (uint64_t)rep * kChalStride + r1_off;
-    const uint32_t *r2 = chal + (uint64_t)rep * kChalStride + r2_off;
+    constexpr uint32_t LOG = BS == 64 ? 6 : BS == 128 ? 7 : BS == 256 ? 8 : 9;
+    static_assert((1 << LOG) == BS, "block size must be 64, 128, 256 or 512");
+    __shared__ uint32_t lo1[BS], lo2[BS], hi1[kEqChunk], hi2[kEqChunk];
     const bool two = ab != nullptr;
     const uint32_t f1 = two ? ab[0] : 1u, f2 = two ? ab[1] : 0u;
-    const uint32_t h = m < 8 ? m : 8, hb = m - h;
+    const uint32_t h = m < LOG ? m : LOG, hb = m - h;
     const uint32_t t = threadIdx.x;
-    const uint64_t base = (uint64_t)blockIdx.x * (kEqChunk * 256);
+    const uint64_t base = (uint64_t)chunk * (kEqChunk * BS);
     if (t < (1u << h)) {
         uint32_t a = f1, b = f2;
         for (uint32_t i = 0; i < h; i++) {
@@ -125,9 +136,8 @@ __global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, uin
     __syncthreads();
     const uint64_t M = 1ull << m;
     const uint32_t mask = (1u << h) - 1u;
-    uint32_t *o = out + rep * out_rs;
     for (int c = 0; c < kEqChunk; c++) {
-        const uint64_t z = base + (uint64_t)c * 256 + t;
+        const uint64_t z = base + (uint64_t)c * BS + t;
         if (z < M) {
             const uint32_t li = (uint32_t)z & mask;
             const uint32_t hidx = (uint32_t)((z >> h) - (base >> h));
@@ -138,10 +148,49 @@ __global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, uin
     }
 }
 
+__global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, uint64_t out_rs,
+                                                 const uint32_t *__restrict__ chal, uint32_t r1_off, uint32_t r2_off,
+                                                 const uint32_t *__restrict__ ab, uint32_t m)
+{
+    const uint32_t rep = blockIdx.y;
+    eq_block<kThreads>(out + rep * out_rs, chal + (uint64_t)rep * kChalStride + r1_off,
+                       chal + (uint64_t)rep * kChalStride + r2_off, ab, m, blockIdx.x);
+}
+
 // ------------------------------------------------------------------ g_x and h_y
 // hg_j[x] = sum over the gates with first input x of v[y] * (c * eqZ_j[z]) (mul) or
 // c * eqZ_j[z] (add, broadcast to every lane), for every rep j at once: each wiring entry
-// and each gathered v[y] row is read once for all R repetitions.
+// and each gathered v[y] row is read once for all R repetitions.  Work item w: x = w / LG,
+// lanes 4g .. 4g+3.
+template <int L>
+__device__ __forceinline__ void gx_item(uint32_t *__restrict__ hg, uint64_t hg_rs, const uint32_t *__restrict__ v,
+                                        const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
+                                        const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
+                                        uint32_t R, uint64_t w)
+{
+    constexpr uint32_t LG = L / 4;
+    const uint32_t x = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
+    uint4 acc[3] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+    const uint32_t e1 = ptr[x + 1];
+    for (uint32_t e = ptr[x]; e < e1; e++) {
+        const uint32_t b = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+        uint32_t ce[3];
+#pragma unroll
+        for (uint32_t j = 0; j < 3; j++) ce[j] = j < R ? m31_mul(c, eqz[j * eqz_rs + z]) : 0u;
+        if (b != kAddMark) {
+            const uint4 vb = ld4(v + (uint64_t)b * L + 4 * g);
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], mul4s(vb, ce[j]));
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], make_uint4(ce[j], ce[j], ce[j], ce[j]));
+        }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 3; j++)
+        if (j < R) st4(hg + j * hg_rs + (uint64_t)x * L + 4 * g, acc[j]);
+}
+
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint64_t hg_rs,
                                                  const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz,
@@ -151,32 +200,30 @@ __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint
     constexpr uint32_t LG = L / 4;
     const uint64_t total = (uint64_t)N * LG;
     const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    for (uint64_t w = tile * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
-        const uint32_t x = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
-        uint4 acc[3] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
-        const uint32_t e1 = ptr[x + 1];
-        for (uint32_t e = ptr[x]; e < e1; e++) {
-            const uint32_t b = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-            uint32_t ce[3];
-#pragma unroll
-            for (uint32_t j = 0; j < 3; j++) ce[j] = j < R ? m31_mul(c, eqz[j * eqz_rs + z]) : 0u;
-            if (b != kAddMark) {
-                const uint4 vb = ld4(v + (uint64_t)b * L + 4 * g);
-#pragma unroll
-                for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], mul4s(vb, ce[j]));
-            } else {
-#pragma unroll
-                for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], make_uint4(ce[j], ce[j], ce[j], ce[j]));
-            }
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < 3; j++)
-            if (j < R) st4(hg + j * hg_rs + (uint64_t)x * L + 4 * g, acc[j]);
-    }
+    for (uint64_t w = tile * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
+        gx_item<L>(hg, hg_rs, v, eqz, eqz_rs, ptr, ent, R, w);
 }
 
 // s_j[y] = sum over the mul gates with second input y of eqZ_j[z] * eqX_j[x] * c, for every
 // rep j at once (phase-2 hg_j = vx_j * s_j).
+__device__ __forceinline__ void hy_item(uint32_t *__restrict__ s, uint64_t s_rs, const uint32_t *__restrict__ eqz,
+                                        uint64_t eqz_rs, const uint32_t *__restrict__ eqx, uint64_t eqx_rs,
+                                        const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
+                                        uint32_t R, uint64_t y)
+{
+    uint32_t acc[3] = {0, 0, 0};
+    const uint32_t e1 = ptr[y + 1];
+    for (uint32_t e = ptr[y]; e < e1; e++) {
+        const uint32_t x = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+#pragma unroll
+        for (uint32_t j = 0; j < 3; j++)
+            if (j < R) acc[j] = m31_add(acc[j], m31_mul(m31_mul(eqz[j * eqz_rs + z], eqx[j * eqx_rs + x]), c));
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 3; j++)
+        if (j < R) s[j * s_rs + y] = acc[j];
+}
+
 __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint64_t s_rs,
                                                  const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
                                                  const uint32_t *__restrict__ eqx, uint64_t eqx_rs,
@@ -184,19 +231,8 @@ __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint6
                                                  uint32_t N, uint32_t R)
 {
     const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    for (uint64_t y = tile * kThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kThreads) {
-        uint32_t acc[3] = {0, 0, 0};
-        const uint32_t e1 = ptr[y + 1];
-        for (uint32_t e = ptr[y]; e < e1; e++) {
-            const uint32_t x = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-#pragma unroll
-            for (uint32_t j = 0; j < 3; j++)
-                if (j < R) acc[j] = m31_add(acc[j], m31_mul(m31_mul(eqz[j * eqz_rs + z], eqx[j * eqx_rs + x]), c));
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < 3; j++)
-            if (j < R) s[j * s_rs + y] = acc[j];
-    }
+    for (uint64_t y = tile * kThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kThreads)
+        hy_item(s, s_rs, eqz, eqz_rs, eqx, eqx_rs, ptr, ent, R, y);
 }
 
 // ------------------------------------------------------------------ reductions
@@ -631,14 +667,15 @@ struct TailRepParams {
     uint32_t len0, hs0;  // transcript length and hash start (bytes of the tail) at round k0
 };
 
+// The body of k_tail_rep for repetition `rep`; blockDim is kRepThreads.
 template <int L, bool PH2>
-__global__ void __launch_bounds__(kRepThreads) k_tail_rep(TailRepParams q)
+__device__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L, W = kRepThreads / 64;
     const TailParams &p = q.t;
     __shared__ uint32_t red[W][3 * L];
     __shared__ uint32_t sums[Q], rr, hb[L + Q];
-    const uint32_t R = p.ts.R, rep = blockIdx.x;
+    const uint32_t R = p.ts.R;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63, g = t % LG;
     uint32_t *chal = p.ts.chal + rep * kChalStride + (PH2 ? 32u : 0u);
     if (t == 0) rr = p.k0 > 0 ? chal[p.k0 - 1] : 0u;
@@ -712,6 +749,12 @@ __global__ void __launch_bounds__(kRepThreads) k_tail_rep(TailRepParams q)
     }
 }
 
+template <int L, bool PH2>
+__global__ void __launch_bounds__(kRepThreads) k_tail_rep(TailRepParams q)
+{
+    tail_rep_body<L, PH2>(q, blockIdx.x);
+}
+
 // Dot product sum_z eq[rep][z] * vals[z] (lanes): claimed_v = eval_multilinear(output, rz1)
 // (include/LinearGKR/gkr.hpp:51-55).  Each block leaves value-major partials (the p0 slot
 // of the round layout; the other slots are zero) that k_dot_fin adds.
@@ -766,6 +809,7 @@ struct BatchJob {
     uint32_t *tail;          // proof bytes after the commitment
     uint32_t *claims;        // R L
     const uint32_t *seed;    // rz1 in slot layout [R * kChalStride], tstate [10], grind bytes [8]
+    uint32_t *partials;      // R * 3L * kMaxBlocksPerRep block partials (lockstep engine)
 };
 
 template <int L>
@@ -773,18 +817,7 @@ __device__ void block_forward(const uint32_t *__restrict__ vin, uint32_t *__rest
                               const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t M)
 {
     constexpr uint32_t LG = L / 4;
-    for (uint64_t w = threadIdx.x; w < (uint64_t)M * LG; w += blockDim.x) {
-        const uint32_t o = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
-        uint4 acc = make_uint4(0, 0, 0, 0);
-        const uint32_t e1 = ptr[o + 1];
-        for (uint32_t e = ptr[o]; e < e1; e++) {
-            const uint32_t a = ent[3 * (uint64_t)e], b = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-            uint4 va = ld4(vin + (uint64_t)a * L + 4 * g);
-            if (b != kAddMark) va = mul4(va, ld4(vin + (uint64_t)b * L + 4 * g));
-            acc = add4(acc, mul4s(va, c));
-        }
-        st4(vout + (uint64_t)o * L + 4 * g, acc);
-    }
+    for (uint64_t w = threadIdx.x; w < (uint64_t)M * LG; w += blockDim.x) forward_item<L>(vin, vout, ptr, ent, w);
 }
 
 // out[j][z] = f1 eq(r1_j, z) + f2 eq(r2_j, z) for z < 2^m (m <= 20), rep by rep, from
@@ -860,47 +893,15 @@ __device__ void block_gx(uint32_t *__restrict__ hg, uint64_t hg_rs, const uint32
                          const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
 {
     constexpr uint32_t LG = L / 4;
-    for (uint64_t w = threadIdx.x; w < (uint64_t)N * LG; w += blockDim.x) {
-        const uint32_t x = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
-        uint4 acc[3] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
-        const uint32_t e1 = ptr[x + 1];
-        for (uint32_t e = ptr[x]; e < e1; e++) {
-            const uint32_t b = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-            uint32_t ce[3];
-#pragma unroll
-            for (uint32_t j = 0; j < 3; j++) ce[j] = j < R ? m31_mul(c, eqz[j * eqz_rs + z]) : 0u;
-            if (b != kAddMark) {
-                const uint4 vb = ld4(v + (uint64_t)b * L + 4 * g);
-#pragma unroll
-                for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], mul4s(vb, ce[j]));
-            } else {
-#pragma unroll
-                for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], make_uint4(ce[j], ce[j], ce[j], ce[j]));
-            }
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < 3; j++)
-            if (j < R) st4(hg + j * hg_rs + (uint64_t)x * L + 4 * g, acc[j]);
-    }
+    for (uint64_t w = threadIdx.x; w < (uint64_t)N * LG; w += blockDim.x)
+        gx_item<L>(hg, hg_rs, v, eqz, eqz_rs, ptr, ent, R, w);
 }
 
 __device__ void block_hy(uint32_t *__restrict__ s, uint64_t s_rs, const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
                          const uint32_t *__restrict__ eqx, uint64_t eqx_rs, const uint32_t *__restrict__ ptr,
                          const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
 {
-    for (uint32_t y = threadIdx.x; y < N; y += blockDim.x) {
-        uint32_t acc[3] = {0, 0, 0};
-        const uint32_t e1 = ptr[y + 1];
-        for (uint32_t e = ptr[y]; e < e1; e++) {
-            const uint32_t x = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-#pragma unroll
-            for (uint32_t j = 0; j < 3; j++)
-                if (j < R) acc[j] = m31_add(acc[j], m31_mul(m31_mul(eqz[j * eqz_rs + z], eqx[j * eqx_rs + x]), c));
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < 3; j++)
-            if (j < R) s[j * s_rs + y] = acc[j];
-    }
+    for (uint32_t y = threadIdx.x; y < N; y += blockDim.x) hy_item(s, s_rs, eqz, eqz_rs, eqx, eqx_rs, ptr, ent, R, y);
 }
 
 template <int L>
@@ -951,6 +952,337 @@ __global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *_
         tp.h0 = J.s0; tp.h0_rs = J.s0_rs; tp.hA = J.sA; tp.hB = J.sB; tp.hA_rs = J.sA_rs; tp.hB_rs = J.sB_rs;
         tp.ts.phase = 2;
         phase_rounds<L, true>(tp);
+    }
+}
+
+// ------------------------------------------------------------------ lockstep batched proving
+// The batch advances in steps.  Every proof is a fixed sequence of ops (its forward pass,
+// then per layer: eqZ, g_x, the phase-1 rounds, eqX, h_y, the phase-2 rounds); step t runs
+// op t of every proof that has one.  One launch of k_ls_work runs the wide ops of a step,
+// each task on its own range of workgroups, so a wide proof gets as many workgroups as its
+// tables need; then one launch of k_ls_fin runs the serial ends: the partial sums and
+// transcript step of a grid round, the claims, and the rounds of a phase whose tables are
+// small.  The launch boundaries order the steps, so no workgroup waits on another.  The
+// schedule depends only on the circuit shapes: the host builds it and uploads it once.
+enum LsOp : uint32_t {
+    kOpSeed = 0,  // work, 1 block: transcript seed from the host prefix
+    kOpFwd,       // work: forward pass of layer l
+    kOpEqz,       // work, R x nb blocks: eqZ of layer l
+    kOpGx,        // work: g_x of layer l
+    kOpDot,       // work, R x nb blocks: claimed_v partials
+    kOpRound,     // work, R x nb blocks: grid round k of a phase
+    kOpEqx,       // work, R x nb blocks: eqX of layer l
+    kOpHy,        // work: h_y of layer l
+    kOpRoundFin,  // fin, 1 block: partials and transcript step of grid round k
+    kOpDotFin,    // fin, 1 block: claimed_v
+    kOpTailRep,   // fin, R blocks: rounds k .. kend - 1 of a phase, one block per rep
+    kOpLast,      // fin, 1 block: the last round of a phase
+};
+
+struct LsFwd {
+    const uint32_t *vin;
+    uint32_t *vout;
+    const uint32_t *ptr, *ent;
+    uint32_t M;
+};
+struct LsGx {
+    uint32_t *hg;
+    uint64_t hg_rs;
+    const uint32_t *v, *eqz;
+    uint64_t eqz_rs;
+    const uint32_t *ptr, *ent;
+    uint32_t N;
+};
+struct LsHy {
+    uint32_t *s;
+    uint64_t s_rs;
+    const uint32_t *eqz;
+    uint64_t eqz_rs;
+    const uint32_t *eqx;
+    uint64_t eqx_rs;
+    const uint32_t *ptr, *ent;
+    uint32_t N;
+};
+struct LsEq {
+    uint32_t *out;
+    uint64_t out_rs;
+    const uint32_t *r1, *r2;  // rep 0; rep stride kChalStride
+    const uint32_t *ab;       // alpha, beta, or nullptr for a single eq table
+    uint32_t m;
+};
+struct LsDot {
+    const uint32_t *vals, *eq;
+    uint64_t eq_rs;
+    uint32_t M;
+    uint32_t *partials;
+};
+
+// One op of one proof in one step, with the pointers of a work op resolved on the host.
+struct LsTask {
+    uint32_t op, nb;     // nb: blocks per repetition (R x nb ops) or in total
+    uint32_t job, l, phase;
+    uint32_t k, kend;    // round (kOpTailRep: rounds k .. kend - 1)
+    uint32_t len0, hs0;  // kOpTailRep: transcript length and hash start at round k
+    uint32_t pad;
+    union {
+        LsFwd fwd;
+        LsGx gx;
+        LsHy hy;
+        LsEq eq;
+        LsDot dot;
+        RoundParams rp;
+    };
+};
+
+// Task of workgroup b in a step, and b's index within it.  start[i] is the first
+// workgroup of task i, strictly increasing from start[0] = 0; ntasks <= 64 * 64.  Wave 0
+// narrows the range with two 64-way ballots (two dependent loads).
+__device__ __forceinline__ uint2 ls_find(const uint32_t *__restrict__ start, uint32_t ntasks, uint32_t b)
+{
+    __shared__ uint32_t found[2];
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        const uint32_t stride = (ntasks + 63) / 64;
+        const uint32_t i = lane * stride;
+        const uint32_t si = i < ntasks ? start[i] : 0xffffffffu;
+        const uint64_t m1 = __ballot(si <= b);
+        const uint32_t lo = (uint32_t)(__popcll(m1) - 1) * stride;
+        const uint32_t j = lo + lane;
+        const uint32_t sj = (lane < stride && j < ntasks) ? start[j] : 0xffffffffu;
+        const uint64_t m2 = __ballot(sj <= b);
+        const uint32_t pos = (uint32_t)__popcll(m2) - 1;
+        const uint32_t s_at = __shfl(sj, (int)pos, 64);
+        if (lane == 0) {
+            found[0] = lo + pos;
+            found[1] = b - s_at;
+        }
+    }
+    __syncthreads();
+    return make_uint2(__builtin_amdgcn_readfirstlane(found[0]), __builtin_amdgcn_readfirstlane(found[1]));
+}
+
+// Adds the 3 x 4 lane sums of a block of BS threads (lanes sharing t % LG) into
+// out[q * ostride], q < 3L.  Every thread must call it.
+template <int L, int BS>
+__device__ __forceinline__ void block_reduce_to(uint64_t (&acc)[3][4], uint32_t *out, uint32_t ostride)
+{
+    constexpr uint32_t LG = L / 4, W = BS / 64;
+    static_assert(BS >= 3 * L, "block too small for the round sums");
+    __shared__ uint32_t red[W][3 * L];
+    uint32_t v[3][4];
+#pragma unroll
+    for (int pk = 0; pk < 3; pk++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) v[pk][c] = m31_reduce64(acc[pk][c]);
+#pragma unroll
+    for (uint32_t off = LG; off < 64; off <<= 1) {
+#pragma unroll
+        for (int pk = 0; pk < 3; pk++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) v[pk][c] = m31_add(v[pk][c], __shfl_xor(v[pk][c], off, 64));
+    }
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < LG) {
+#pragma unroll
+        for (int pk = 0; pk < 3; pk++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) red[wave][pk * L + 4 * lane + c] = v[pk][c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * L) {
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < W; w++) s = m31_add(s, red[w][threadIdx.x]);
+        out[threadIdx.x * ostride] = s;
+    }
+}
+
+// Workgroup lb of a grid round task (rep = lb / nb): k_round on a block range.
+template <int L, int BS, bool PH2, bool FUSED>
+__device__ void ls_round(const RoundParams &p, uint32_t nb, uint32_t lb)
+{
+    constexpr uint32_t LG = L / 4, Q = 3 * L;
+    const uint32_t rep = lb / nb, bx = lb % nb, g = threadIdx.x % LG;
+    const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
+    uint32_t *fd = FUSED ? p.fdst + rep * p.fdst_rs : nullptr;
+    const uint32_t *hs = p.hsrc + rep * p.hsrc_rs;
+    uint32_t *hd = FUSED ? p.hdst + rep * p.hdst_rs : nullptr;
+    const uint32_t r = FUSED ? p.rprev[(uint64_t)rep * kChalStride] : 0u;
+    uint64_t acc[3][4] = {};
+    const uint64_t total = (uint64_t)p.items * LG;
+    for (uint64_t w = (uint64_t)bx * BS + threadIdx.x; w < total; w += (uint64_t)nb * BS)
+        round_item<L, PH2, FUSED>(fs, fd, hs, hd, w / LG, g, r, acc);
+    block_reduce_to<L, BS>(acc, p.partials + (uint64_t)rep * Q * nb + bx, nb);
+}
+
+template <int L, int BS>
+__global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ jobs, const LsTask *__restrict__ tasks,
+                                                const uint32_t *__restrict__ start, uint32_t ntasks, uint32_t R)
+{
+    constexpr uint32_t LG = L / 4, Q = 3 * L;
+    const uint2 f = ls_find(start, ntasks, blockIdx.x);
+    const LsTask &T = tasks[f.x];
+    const uint32_t lb = f.y, t = threadIdx.x;
+    switch (T.op) {
+    case kOpSeed: {
+        // rz1 -> chal slot D, alpha = 1, beta = 0, transcript state, grind bytes
+        const BatchJob &J = jobs[T.job];
+        uint32_t *slot = J.chal + (uint64_t)J.D * R * kChalStride;
+        for (uint32_t i = t; i < R * kChalStride; i += BS) slot[i] = J.seed[i];
+        if (t < 10) J.tstate[t] = J.seed[R * kChalStride + t];
+        if (t < 8) J.tail[t] = J.seed[R * kChalStride + 10 + t];
+        if (t == 0) {
+            J.ab[2ull * J.D] = 1u;
+            J.ab[2ull * J.D + 1] = 0u;
+        }
+        break;
+    }
+    case kOpFwd: {
+        const LsFwd &a = T.fwd;
+        const uint64_t total = (uint64_t)a.M * LG, tile = xcd_tile(lb, T.nb);
+        for (uint64_t w = tile * BS + t; w < total; w += (uint64_t)T.nb * BS)
+            forward_item<L>(a.vin, a.vout, a.ptr, a.ent, w);
+        break;
+    }
+    case kOpEqz:
+    case kOpEqx: {
+        const LsEq &a = T.eq;
+        const uint32_t rep = lb / T.nb;
+        eq_block<BS>(a.out + rep * a.out_rs, a.r1 + (uint64_t)rep * kChalStride, a.r2 + (uint64_t)rep * kChalStride,
+                     a.ab, a.m, lb % T.nb);
+        break;
+    }
+    case kOpGx: {
+        const LsGx &a = T.gx;
+        const uint64_t total = (uint64_t)a.N * LG, tile = xcd_tile(lb, T.nb);
+        for (uint64_t w = tile * BS + t; w < total; w += (uint64_t)T.nb * BS)
+            gx_item<L>(a.hg, a.hg_rs, a.v, a.eqz, a.eqz_rs, a.ptr, a.ent, R, w);
+        break;
+    }
+    case kOpHy: {
+        const LsHy &a = T.hy;
+        const uint64_t tile = xcd_tile(lb, T.nb);
+        for (uint64_t y = tile * BS + t; y < a.N; y += (uint64_t)T.nb * BS)
+            hy_item(a.s, a.s_rs, a.eqz, a.eqz_rs, a.eqx, a.eqx_rs, a.ptr, a.ent, R, y);
+        break;
+    }
+    case kOpDot: {
+        const LsDot &a = T.dot;
+        const uint32_t rep = lb / T.nb, bx = lb % T.nb, g = t % LG;
+        const uint32_t *e = a.eq + rep * a.eq_rs;
+        uint64_t acc[3][4] = {};
+        for (uint64_t w = (uint64_t)bx * BS + t; w < (uint64_t)a.M * LG; w += (uint64_t)T.nb * BS) {
+            const uint64_t z = w / LG;
+            const uint4 v = mul4s(ld4(a.vals + z * L + 4 * g), e[z]);
+            acc[0][0] += v.x; acc[0][1] += v.y; acc[0][2] += v.z; acc[0][3] += v.w;
+        }
+        block_reduce_to<L, BS>(acc, a.partials + (uint64_t)rep * Q * T.nb + bx, T.nb);
+        break;
+    }
+    case kOpRound: {
+        if (T.phase == 1) {
+            if (T.k == 0) ls_round<L, BS, false, false>(T.rp, T.nb, lb);
+            else ls_round<L, BS, false, true>(T.rp, T.nb, lb);
+        } else {
+            if (T.k == 0) ls_round<L, BS, true, false>(T.rp, T.nb, lb);
+            else ls_round<L, BS, true, true>(T.rp, T.nb, lb);
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+template <int L>
+__device__ TsArgs ls_ts(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t R, int hash_kind)
+{
+    TsArgs ts{};
+    ts.R = R;
+    ts.phase = (int)phase;
+    ts.chal = J.chal + (uint64_t)l * R * kChalStride;
+    ts.vx = J.vxy;
+    ts.vy = J.vxy + R * L;
+    ts.tstate = J.tstate;
+    ts.tail = J.tail;
+    ts.ab_out = J.ab + 2ull * l;
+    ts.hash_kind = hash_kind;
+    return ts;
+}
+
+// The small-table rounds of layer l, phase `phase`, from round k0 on (sources as in the
+// single-proof path: table k - 1 feeds round k, table 0 is the layer input and hg0 / s0).
+template <int L>
+__device__ TailParams ls_tail_params(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t R, int hash_kind,
+                                     uint32_t k0)
+{
+    const LayerDev &ly = J.layers[l];
+    const bool p2 = phase == 2;
+    uint32_t *hA = p2 ? J.sA : J.hgA, *hB = p2 ? J.sB : J.hgB;
+    const uint64_t hA_rs = p2 ? J.sA_rs : J.hgA_rs, hB_rs = p2 ? J.sB_rs : J.hgB_rs;
+    TailParams tp{};
+    tp.fA = J.fA; tp.fB = J.fB; tp.hA = hA; tp.hB = hB;
+    tp.fA_rs = J.fA_rs; tp.fB_rs = J.fB_rs; tp.hA_rs = hA_rs; tp.hB_rs = hB_rs;
+    tp.n = ly.n;
+    tp.k0 = k0;
+    if (k0 <= 1) {
+        tp.f0 = J.values + ly.v_off;
+        tp.f0_rs = 0;
+        tp.h0 = p2 ? J.s0 : J.hg0;
+        tp.h0_rs = p2 ? J.s0_rs : J.hg0_rs;
+    } else {
+        tp.f0 = (k0 - 1) & 1 ? J.fA : J.fB;
+        tp.f0_rs = (k0 - 1) & 1 ? J.fA_rs : J.fB_rs;
+        tp.h0 = (k0 - 1) & 1 ? hA : hB;
+        tp.h0_rs = (k0 - 1) & 1 ? hA_rs : hB_rs;
+    }
+    tp.ts = ls_ts<L>(J, l, phase, R, hash_kind);
+    return tp;
+}
+
+static_assert(kFinThreads == kRepThreads, "k_ls_fin runs tail_rep_body");
+template <int L>
+__global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restrict__ jobs,
+                                                        const LsTask *__restrict__ tasks,
+                                                        const uint32_t *__restrict__ start, uint32_t ntasks,
+                                                        uint32_t R, int hash_kind)
+{
+    constexpr uint32_t Q = 3 * L;
+    __shared__ uint32_t sums[3 * Q], rr[3];
+    const uint2 f = ls_find(start, ntasks, blockIdx.x);
+    const LsTask &T = tasks[f.x];
+    const BatchJob &J = jobs[T.job];
+    switch (T.op) {
+    case kOpRoundFin: {
+        reduce_partials<L>(J.partials, T.nb, R * Q, sums);
+        TsArgs ts = ls_ts<L>(J, T.l, T.phase, R, hash_kind);
+        ts.k = T.k;
+        transcript_step<L>(sums, ts, rr);
+        break;
+    }
+    case kOpDotFin: {
+        reduce_partials<L>(J.partials, T.nb, R * Q, sums);
+        if (threadIdx.x < R * L) J.claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
+        break;
+    }
+    case kOpTailRep: {
+        TailRepParams q{};
+        q.t = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, T.k);
+        q.kend = T.kend;
+        q.len0 = T.len0;
+        q.hs0 = T.hs0;
+        if (T.phase == 1) tail_rep_body<L, false>(q, f.y);
+        else tail_rep_body<L, true>(q, f.y);
+        break;
+    }
+    case kOpLast: {
+        const TailParams tp = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, J.layers[T.l].n - 1);
+        if (T.phase == 1) phase_rounds<L, false>(tp);
+        else phase_rounds<L, true>(tp);
+        break;
+    }
+    default:
+        break;
     }
 }
 
@@ -1337,6 +1669,232 @@ void dev_free_scratch(DevScratch &s)
 }
 
 // ------------------------------------------------------------------ batched prove (host)
+namespace {
+
+// u32 of one proof's tables in the batch pool, in the order dev_prove_batch hands them out
+uint64_t batch_tables_u32(const DevCircuit &dc, uint32_t R)
+{
+    const uint32_t L = dc.L;
+    const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
+    const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
+    uint64_t u = 0;
+    auto add = [&](uint64_t n) { u += (n + 63) & ~63ull; };
+    add(R * maxM); add(R * maxN); add(R * maxN * L); add(R * N2 * L); add(R * N4 * L); add(R * N2 * L);
+    add(R * N4 * L); add(R * N1); add(R * N2); add(R * N4); add((uint64_t)(dc.D + 1) * R * kChalStride);
+    add(2ull * (dc.D + 1)); add(2ull * R * L); add(16); add((uint64_t)R * 3 * L * kMaxBlocksPerRep);
+    return u;
+}
+
+constexpr uint32_t kLsGatherItems = 4;  // work items per thread of a gather task
+constexpr uint32_t kLsMaxJobs = 2048;   // at most two work tasks per proof in a step, 4096 per step
+
+uint32_t ls_grid(uint64_t work, uint32_t bs)
+{
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + bs - 1) / bs, kMaxBlocksPerRep));
+}
+
+// Blocks of a gather task: about kLsGatherItems items per thread, rounded up to a multiple
+// of 8 so that every gather task (they lead their step) starts on the first XCD and
+// xcd_tile applies to its block index.
+uint32_t ls_gather_blocks(uint64_t work, uint32_t bs)
+{
+    uint64_t b = (work + (uint64_t)bs * kLsGatherItems - 1) / ((uint64_t)bs * kLsGatherItems);
+    b = std::max<uint64_t>(1, std::min<uint64_t>(b, 65536));
+    return (uint32_t)((b + 7) & ~7ull);
+}
+
+bool ls_is_gather(uint32_t op) { return op == kOpFwd || op == kOpGx || op == kOpHy; }
+
+uint32_t ls_work_blocks(const LsTask &t, uint32_t R)
+{
+    return (t.op == kOpEqz || t.op == kOpEqx || t.op == kOpDot || t.op == kOpRound) ? R * t.nb : t.nb;
+}
+
+uint32_t ls_fin_blocks(const LsTask &t, uint32_t R) { return t.op == kOpTailRep ? R : 1u; }
+
+LsTask ls_task(uint32_t op, uint32_t job, uint32_t l)
+{
+    LsTask t;
+    memset(&t, 0, sizeof t);
+    t.op = op;
+    t.job = job;
+    t.l = l;
+    return t;
+}
+
+// One step of one proof: up to two independent work tasks and one fin task.
+struct LsEntry {
+    LsTask w[2];
+    uint32_t nw = 0;
+    LsTask f;
+    uint32_t nf = 0;
+};
+
+// The steps of one proof, in the order of the single-proof path (prove_impl): the same
+// kernels, round split and transcript positions.  J holds the device addresses of the
+// proof's tables; ts_len / ts_hs is the transcript position after the host prefix.
+void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t R, uint32_t BS, uint64_t ts_len,
+                uint64_t ts_hs, std::vector<LsEntry> &ops)
+{
+    const uint32_t L = dc.L, LG = L / 4, Q = 3 * L, D = dc.D;
+    auto push_w = [&](const LsTask &t) {
+        LsEntry e;
+        e.w[0] = t;
+        e.nw = 1;
+        ops.push_back(e);
+    };
+    auto push_f = [&](const LsTask &t) {
+        LsEntry e;
+        e.f = t;
+        e.nf = 1;
+        ops.push_back(e);
+    };
+    auto chal_slot = [&](uint32_t l) { return J.chal + (uint64_t)l * R * kChalStride; };
+    for (uint32_t l = 0; l < D; l++) {
+        const LayerDev &ly = dc.layers[l];
+        LsEntry e;
+        LsTask t = ls_task(kOpFwd, job, l);
+        t.fwd = LsFwd{dc.values + ly.v_off, dc.values + ly.out_off, ly.fw_ptr, ly.fw_ent, 1u << ly.m};
+        t.nb = ls_gather_blocks((1ull << ly.m) * LG, BS);
+        e.w[0] = t;
+        e.nw = 1;
+        if (l == 0) {
+            LsTask s = ls_task(kOpSeed, job, 0);
+            s.nb = 1;
+            e.w[1] = s;
+            e.nw = 2;
+        }
+        ops.push_back(e);
+    }
+    for (int li = (int)D - 1; li >= 0; li--) {
+        const uint32_t l = (uint32_t)li;
+        const LayerDev &ly = dc.layers[l];
+        const uint32_t n = ly.n, m = ly.m;
+        const uint64_t N = 1ull << n, M = 1ull << m;
+        const uint32_t *V = dc.values + ly.v_off;
+        {
+            LsTask t = ls_task(kOpEqz, job, l);
+            t.eq = LsEq{J.eqz, J.eqz_rs, chal_slot(l + 1), chal_slot(l + 1) + 32, J.ab + 2ull * (l + 1), m};
+            t.nb = (uint32_t)((M + kEqChunk * BS - 1) / (kEqChunk * BS));
+            push_w(t);
+        }
+        {
+            LsEntry e;
+            LsTask t = ls_task(kOpGx, job, l);
+            t.gx = LsGx{J.hg0, J.hg0_rs, V, J.eqz, J.eqz_rs, ly.gx_ptr, ly.gx_ent, (uint32_t)N};
+            t.nb = ls_gather_blocks(N * LG, BS);
+            e.w[0] = t;
+            e.nw = 1;
+            if (l == D - 1) {
+                LsTask d = ls_task(kOpDot, job, l);
+                d.dot = LsDot{dc.values + ly.out_off, J.eqz, J.eqz_rs, (uint32_t)M, J.partials};
+                d.nb = ls_grid(M * LG, BS);
+                e.w[1] = d;
+                e.nw = 2;
+                LsTask fz = ls_task(kOpDotFin, job, l);
+                fz.nb = d.nb;
+                e.f = fz;
+                e.nf = 1;
+            }
+            ops.push_back(e);
+        }
+        for (uint32_t phase = 1; phase <= 2; phase++) {
+            const bool p2 = phase == 2;
+            if (p2) {
+                LsTask t = ls_task(kOpEqx, job, l);
+                t.eq = LsEq{J.eqx, J.eqx_rs, chal_slot(l), chal_slot(l), nullptr, n};
+                t.nb = (uint32_t)((N + kEqChunk * BS - 1) / (kEqChunk * BS));
+                push_w(t);
+                LsTask h = ls_task(kOpHy, job, l);
+                h.hy = LsHy{J.s0, J.s0_rs, J.eqz, J.eqz_rs, J.eqx, J.eqx_rs, ly.hy_ptr, ly.hy_ent, (uint32_t)N};
+                h.nb = ls_gather_blocks(N, BS);
+                push_w(h);
+            }
+            uint32_t *h0 = p2 ? J.s0 : J.hg0, *hA = p2 ? J.sA : J.hgA, *hB = p2 ? J.sB : J.hgB;
+            const uint64_t h0_rs = p2 ? J.s0_rs : J.hg0_rs, hA_rs = p2 ? J.sA_rs : J.hgA_rs,
+                           hB_rs = p2 ? J.sB_rs : J.hgB_rs;
+            uint32_t k0 = 0;
+            while (k0 < n && (N >> k0) > kTailRows) k0++;
+            for (uint32_t k = 0; k < k0; k++) {
+                RoundParams rp{};
+                rp.items = (uint32_t)(N >> (k + 1));
+                rp.partials = J.partials;
+                if (k == 0) {
+                    rp.fsrc = V; rp.fsrc_rs = 0;
+                    rp.hsrc = h0; rp.hsrc_rs = h0_rs;
+                } else {
+                    rp.fsrc = (k == 1) ? V : ((k - 1) & 1 ? J.fA : J.fB);
+                    rp.fsrc_rs = (k == 1) ? 0 : ((k - 1) & 1 ? J.fA_rs : J.fB_rs);
+                    rp.fdst = (k & 1) ? J.fA : J.fB;
+                    rp.fdst_rs = (k & 1) ? J.fA_rs : J.fB_rs;
+                    rp.hsrc = (k == 1) ? h0 : ((k - 1) & 1 ? hA : hB);
+                    rp.hsrc_rs = (k == 1) ? h0_rs : ((k - 1) & 1 ? hA_rs : hB_rs);
+                    rp.hdst = (k & 1) ? hA : hB;
+                    rp.hdst_rs = (k & 1) ? hA_rs : hB_rs;
+                    rp.rprev = chal_slot(l) + (p2 ? 32 : 0) + (k - 1);
+                }
+                LsEntry e;
+                LsTask t = ls_task(kOpRound, job, l);
+                t.phase = phase;
+                t.k = k;
+                t.rp = rp;
+                t.nb = ls_grid((uint64_t)rp.items * LG, BS);
+                e.w[0] = t;
+                e.nw = 1;
+                LsTask fz = ls_task(kOpRoundFin, job, l);
+                fz.phase = phase;
+                fz.k = k;
+                fz.nb = t.nb;
+                e.f = fz;
+                e.nf = 1;
+                ops.push_back(e);
+                ts_len += (uint64_t)R * 4 * Q;
+                ts_hs = ts_len;
+            }
+            const uint32_t klast = n - 1;
+            if (k0 < klast) {
+                LsTask fz = ls_task(kOpTailRep, job, l);
+                fz.phase = phase;
+                fz.k = k0;
+                fz.kend = klast;
+                fz.len0 = (uint32_t)ts_len;
+                fz.hs0 = (uint32_t)ts_hs;
+                fz.nb = 1;
+                push_f(fz);
+                ts_len += (uint64_t)(klast - k0) * R * 4 * Q;
+                ts_hs = ts_len;
+            }
+            LsTask fz = ls_task(kOpLast, job, l);
+            fz.phase = phase;
+            fz.nb = 1;
+            push_f(fz);
+            ts_len += (uint64_t)R * (4 * Q + 4 * L);
+            ts_hs = p2 ? ts_len : ts_len - 4 * L;
+        }
+    }
+}
+
+template <int L>
+void ls_launch_work(uint32_t BS, uint32_t blocks, hipStream_t st, const BatchJob *jobs, const LsTask *tasks,
+                    const uint32_t *start, uint32_t n, uint32_t R)
+{
+    switch (BS) {
+    case 64: hipLaunchKernelGGL((k_ls_work<L, 64>), dim3(blocks), dim3(64), 0, st, jobs, tasks, start, n, R); break;
+    case 128: hipLaunchKernelGGL((k_ls_work<L, 128>), dim3(blocks), dim3(128), 0, st, jobs, tasks, start, n, R); break;
+    case 512: hipLaunchKernelGGL((k_ls_work<L, 512>), dim3(blocks), dim3(512), 0, st, jobs, tasks, start, n, R); break;
+    default: hipLaunchKernelGGL((k_ls_work<L, 256>), dim3(blocks), dim3(256), 0, st, jobs, tasks, start, n, R); break;
+    }
+}
+
+struct LsStep {
+    uint32_t w_task, w_start, w_n, w_blocks;  // work tasks of the step and their workgroups
+    uint32_t f_task, f_start, f_n, f_blocks;  // fin tasks
+};
+
+inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+}  // namespace
+
 int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_kind, BatchScratch &bs, hipStream_t st,
                     Profiler *pf)
 {
@@ -1344,38 +1902,25 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     if (nj == 0) return GKRX_OK;
     if (R > 3) return GKRX_ERR_CONFIG;
     const uint32_t L = items[0].dc->L;
+    const bool lockstep = bs.engine != kBatchPerProof && nj <= kLsMaxJobs;
+    const uint32_t BS = (bs.block == 64 || bs.block == 128 || bs.block == 512) ? bs.block : 256u;
     // sizes of every job's scratch, one pool
     std::vector<uint64_t> job_off(nj + 1, 0), tail_off(nj + 1, 0);
     const uint64_t seed_u32 = (uint64_t)R * kChalStride + 10 + 8;
     for (size_t i = 0; i < nj; i++) {
         const DevCircuit &dc = *items[i].dc;
         if (dc.L != L) return GKRX_ERR_ARG;
-        const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
-        const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
-        uint64_t u = 0;
-        auto add = [&](uint64_t n) { u += (n + 63) & ~63ull; };
-        add(R * maxM); add(R * maxN); add(R * maxN * L); add(R * N2 * L); add(R * N4 * L); add(R * N2 * L);
-        add(R * N4 * L); add(R * N1); add(R * N2); add(R * N4); add((uint64_t)(dc.D + 1) * R * kChalStride);
-        add(2ull * (dc.D + 1)); add(2ull * R * L); add(16);
-        job_off[i + 1] = job_off[i] + u;
+        job_off[i + 1] = job_off[i] + batch_tables_u32(dc, R);
         tail_off[i + 1] = tail_off[i] + ((items[i].tail_bytes + 3) / 4 + 63) / 64 * 64;
     }
     const uint64_t need = job_off[nj] + tail_off[nj] + nj * (uint64_t)R * L + nj * seed_u32 + 64;
-    const uint64_t need_jobs = nj * sizeof(BatchJob) + nj * 4 + 64;
     GKRX_HIP_CHECK(hipStreamSynchronize(st));  // the previous batch may still use the buffers
-    if (need > bs.pool_u32 || need_jobs > bs.meta_bytes) {
+    if (need > bs.pool_u32) {
         if (bs.pool) (void)hipFree(bs.pool);
-        if (bs.meta) (void)hipFree(bs.meta);
-        if (bs.host_meta) (void)hipHostFree(bs.host_meta);
         bs.pool = nullptr;
-        bs.meta = nullptr;
-        bs.host_meta = nullptr;
-        bs.pool_u32 = bs.meta_bytes = 0;
+        bs.pool_u32 = 0;
         GKRX_HIP_CHECK(hipMalloc(&bs.pool, need * 4));
-        GKRX_HIP_CHECK(hipMalloc(&bs.meta, need_jobs));
-        GKRX_HIP_CHECK(hipHostMalloc(&bs.host_meta, need_jobs + nj * seed_u32 * 4, hipHostMallocDefault));
         bs.pool_u32 = need;
-        bs.meta_bytes = need_jobs;
     }
     if (tail_off[nj] * 4 > bs.tails_cap) {
         if (bs.tails_host) (void)hipHostFree(bs.tails_host);
@@ -1388,9 +1933,8 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     uint32_t *tails = bs.pool + job_off[nj];
     uint32_t *claims = tails + tail_off[nj];
     uint32_t *seeds = claims + nj * (uint64_t)R * L;
-    BatchJob *hjobs = reinterpret_cast<BatchJob *>(bs.host_meta);
-    uint32_t *horder = reinterpret_cast<uint32_t *>(bs.host_meta + nj * sizeof(BatchJob));
-    uint32_t *hseed = reinterpret_cast<uint32_t *>(bs.host_meta + need_jobs);
+    std::vector<BatchJob> hjobs(nj);
+    std::vector<uint32_t> hseed(nj * seed_u32, 0);
     std::vector<std::pair<uint64_t, uint32_t>> work(nj);
     for (size_t i = 0; i < nj; i++) {
         const DevCircuit &dc = *items[i].dc;
@@ -1417,12 +1961,12 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         J.ab = take(2ull * (dc.D + 1));
         J.vxy = take(2ull * R * L);
         J.tstate = take(16);
+        J.partials = take((uint64_t)R * 3 * L * kMaxBlocksPerRep);
         J.tail = tails + tail_off[i];
         J.claims = claims + i * (uint64_t)R * L;
         J.seed = seeds + i * seed_u32;
         // seed record: rz1 in chal-slot layout, transcript state, grind bytes
-        uint32_t *sd = hseed + i * seed_u32;
-        memset(sd, 0, seed_u32 * 4);
+        uint32_t *sd = hseed.data() + i * seed_u32;
         const uint32_t mlast = dc.layers[dc.D - 1].m;
         for (uint32_t j = 0; j < R; j++)
             for (uint32_t v = 0; v < mlast; v++) sd[j * kChalStride + v] = items[i].rz1[(size_t)j * mlast + v];
@@ -1432,20 +1976,114 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         for (uint32_t l = 0; l < dc.D; l++) w += (1ull << dc.layers[l].n);
         work[i] = {w, (uint32_t)i};
     }
-    std::stable_sort(work.begin(), work.end(), [](auto &a, auto &b) { return a.first > b.first; });
-    for (size_t i = 0; i < nj; i++) horder[i] = work[i].second;
-    GKRX_HIP_CHECK(hipMemcpyAsync(bs.meta, bs.host_meta, need_jobs, hipMemcpyHostToDevice, st));
-    GKRX_HIP_CHECK(hipMemcpyAsync(seeds, hseed, nj * seed_u32 * 4, hipMemcpyHostToDevice, st));
+    // per-proof engine: workgroup order, largest proof first; lockstep: the step schedule
+    std::vector<uint32_t> horder;
+    std::vector<LsTask> tasks;
+    std::vector<uint32_t> starts;
+    std::vector<LsStep> steps;
+    if (!lockstep) {
+        std::stable_sort(work.begin(), work.end(), [](auto &a, auto &b) { return a.first > b.first; });
+        horder.resize(nj);
+        for (size_t i = 0; i < nj; i++) horder[i] = work[i].second;
+    } else {
+        std::vector<std::vector<LsEntry>> ops(nj);
+        size_t nsteps = 0;
+        for (size_t i = 0; i < nj; i++) {
+            ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, items[i].tstate[0], items[i].tstate[1], ops[i]);
+            nsteps = std::max(nsteps, ops[i].size());
+        }
+        for (size_t s = 0; s < nsteps; s++) {
+            LsStep S{};
+            S.w_task = (uint32_t)tasks.size();
+            S.w_start = (uint32_t)starts.size();
+            uint32_t blocks = 0;
+            for (int pass = 0; pass < 2; pass++)  // gather tasks first (multiples of 8 blocks)
+                for (size_t i = 0; i < nj; i++) {
+                    if (s >= ops[i].size()) continue;
+                    const LsEntry &e = ops[i][s];
+                    for (uint32_t q = 0; q < e.nw; q++)
+                        if (ls_is_gather(e.w[q].op) == (pass == 0)) {
+                            starts.push_back(blocks);
+                            tasks.push_back(e.w[q]);
+                            blocks += ls_work_blocks(e.w[q], R);
+                        }
+                }
+            S.w_n = (uint32_t)tasks.size() - S.w_task;
+            S.w_blocks = blocks;
+            starts.push_back(blocks);
+            S.f_task = (uint32_t)tasks.size();
+            S.f_start = (uint32_t)starts.size();
+            blocks = 0;
+            for (size_t i = 0; i < nj; i++) {
+                if (s >= ops[i].size() || !ops[i][s].nf) continue;
+                starts.push_back(blocks);
+                tasks.push_back(ops[i][s].f);
+                blocks += ls_fin_blocks(ops[i][s].f, R);
+            }
+            S.f_n = (uint32_t)tasks.size() - S.f_task;
+            S.f_blocks = blocks;
+            starts.push_back(blocks);
+            steps.push_back(S);
+        }
+    }
+    // device metadata: [jobs][order][tasks][starts]; the pinned staging adds the seeds
+    const uint64_t o_order = align256(nj * sizeof(BatchJob));
+    const uint64_t o_tasks = align256(o_order + 4 * horder.size());
+    const uint64_t o_starts = align256(o_tasks + sizeof(LsTask) * tasks.size());
+    const uint64_t meta_need = align256(o_starts + 4 * starts.size());
+    const uint64_t host_need = meta_need + nj * seed_u32 * 4;
+    if (meta_need > bs.meta_bytes) {
+        if (bs.meta) (void)hipFree(bs.meta);
+        bs.meta = nullptr;
+        bs.meta_bytes = 0;
+        GKRX_HIP_CHECK(hipMalloc(&bs.meta, meta_need));
+        bs.meta_bytes = meta_need;
+    }
+    if (host_need > bs.host_meta_bytes) {
+        if (bs.host_meta) (void)hipHostFree(bs.host_meta);
+        bs.host_meta = nullptr;
+        bs.host_meta_bytes = 0;
+        GKRX_HIP_CHECK(hipHostMalloc(&bs.host_meta, host_need, hipHostMallocDefault));
+        bs.host_meta_bytes = host_need;
+    }
+    memcpy(bs.host_meta, hjobs.data(), nj * sizeof(BatchJob));
+    if (!horder.empty()) memcpy(bs.host_meta + o_order, horder.data(), 4 * horder.size());
+    if (!tasks.empty()) memcpy(bs.host_meta + o_tasks, tasks.data(), sizeof(LsTask) * tasks.size());
+    if (!starts.empty()) memcpy(bs.host_meta + o_starts, starts.data(), 4 * starts.size());
+    memcpy(bs.host_meta + meta_need, hseed.data(), hseed.size() * 4);
+    GKRX_HIP_CHECK(hipMemcpyAsync(bs.meta, bs.host_meta, meta_need, hipMemcpyHostToDevice, st));
+    GKRX_HIP_CHECK(hipMemcpyAsync(seeds, bs.host_meta + meta_need, hseed.size() * 4, hipMemcpyHostToDevice, st));
     const BatchJob *djobs = reinterpret_cast<const BatchJob *>(bs.meta);
-    const uint32_t *dorder = reinterpret_cast<const uint32_t *>(bs.meta + nj * sizeof(BatchJob));
-    timed(pf, kKBatch, st, [&] {
-        if (L == 8)
-            hipLaunchKernelGGL(k_prove_batch<8>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder, R,
-                               hash_kind);
-        else
-            hipLaunchKernelGGL(k_prove_batch<16>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder, R,
-                               hash_kind);
-    });
+    if (!lockstep) {
+        const uint32_t *dorder = reinterpret_cast<const uint32_t *>(bs.meta + o_order);
+        timed(pf, kKBatch, st, [&] {
+            if (L == 8)
+                hipLaunchKernelGGL(k_prove_batch<8>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder,
+                                   R, hash_kind);
+            else
+                hipLaunchKernelGGL(k_prove_batch<16>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder,
+                                   R, hash_kind);
+        });
+    } else {
+        const LsTask *dtasks = reinterpret_cast<const LsTask *>(bs.meta + o_tasks);
+        const uint32_t *dstarts = reinterpret_cast<const uint32_t *>(bs.meta + o_starts);
+        for (const LsStep &S : steps) {
+            if (S.w_n)
+                timed(pf, kKLsWork, st, [&] {
+                    if (L == 8) ls_launch_work<8>(BS, S.w_blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R);
+                    else ls_launch_work<16>(BS, S.w_blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R);
+                });
+            if (S.f_n)
+                timed(pf, kKLsFin, st, [&] {
+                    if (L == 8)
+                        hipLaunchKernelGGL(k_ls_fin<8>, dim3(S.f_blocks), dim3(kFinThreads), 0, st, djobs,
+                                           dtasks + S.f_task, dstarts + S.f_start, S.f_n, R, hash_kind);
+                    else
+                        hipLaunchKernelGGL(k_ls_fin<16>, dim3(S.f_blocks), dim3(kFinThreads), 0, st, djobs,
+                                           dtasks + S.f_task, dstarts + S.f_start, S.f_n, R, hash_kind);
+                });
+        }
+    }
     GKRX_HIP_CHECK(hipGetLastError());
     GKRX_HIP_CHECK(hipMemcpyAsync(bs.tails_host, tails, tail_off[nj] * 4, hipMemcpyDeviceToHost, st));
     uint32_t *hc = bs.host_claims(nj * R * L);
@@ -1462,16 +2100,10 @@ void dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &
 
 uint64_t dev_batch_job_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes)
 {
-    const uint32_t L = dc.L;
-    const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
-    const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
-    uint64_t u = 0;
+    uint64_t u = batch_tables_u32(dc, R);
     auto add = [&](uint64_t n) { u += (n + 63) & ~63ull; };
-    add(R * maxM); add(R * maxN); add(R * maxN * L); add(R * N2 * L); add(R * N4 * L); add(R * N2 * L);
-    add(R * N4 * L); add(R * N1); add(R * N2); add(R * N4); add((uint64_t)(dc.D + 1) * R * kChalStride);
-    add(2ull * (dc.D + 1)); add(2ull * R * L); add(16);
     add((tail_bytes + 3) / 4);
-    add((uint64_t)R * L + (uint64_t)R * kChalStride + 18);
+    add((uint64_t)dc.L * R + (uint64_t)R * kChalStride + 18);
     return 4 * u;
 }
 

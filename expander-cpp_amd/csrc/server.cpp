@@ -357,6 +357,8 @@ extern "C" int gkrx_server_create(gkrx_ctx *ctx, uint64_t hbm_budget_bytes, uint
     for (uint32_t i = 0; i < total; i++) {
         auto sl = std::make_unique<Slot>();
         sl->single = ctx->bn254 || i == slots;
+        sl->bs.engine = ctx->batch.engine;  // the context's batch engine (gkrx_ctx_set_batch_engine)
+        sl->bs.block = ctx->batch.block;
         if (hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl->ev, hipEventDisableTiming) != hipSuccess) {
             free_slot(*sl);

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libgkrx.so")
 
 FIELD_M31, FIELD_BABYBEAR, FIELD_BN254 = 0, 1, 2
 FS_SHA256, FS_KECCAK256 = 0, 1
+BATCH_PER_PROOF, BATCH_LOCKSTEP = 1, 2
 PC_RAW = 0
 P31 = (1 << 31) - 1
 
@@ -85,6 +86,10 @@ def lib():
     L.gkrx_server_stats.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.gkrx_server_destroy.argtypes = [vp]
     L.gkrx_ctx_profile.argtypes = [vp, C.c_int]
+    L.gkrx_ctx_set_batch_engine.argtypes = [vp, C.c_int, C.c_uint32]
+    L.gkrx_ctx_get_batch_engine.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]
+    L.gkrx_batch_autotune.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_int), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_double)]
     L.gkrx_ctx_kernel_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_double)]
     _lib = L
@@ -278,6 +283,26 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_batch_engine(self, engine: int, block: int = 256) -> None:
+        """BATCH_PER_PROOF or BATCH_LOCKSTEP (block 64, 128, 256 or 512) for prove_batch."""
+        _check(lib().gkrx_ctx_set_batch_engine(self._h, engine, block), "gkrx_ctx_set_batch_engine")
+
+    def batch_engine(self):
+        e, b = C.c_int(), C.c_uint32()
+        _check(lib().gkrx_ctx_get_batch_engine(self._h, C.byref(e), C.byref(b)), "gkrx_ctx_get_batch_engine")
+        return e.value, b.value
+
+    def batch_autotune(self, circuits):
+        """Times every batch engine / block size on these circuits and keeps the fastest.
+        Returns (engine, block, {(engine, block): ms})."""
+        n = len(circuits)
+        hs = (C.c_void_p * n)(*[c._h.value for c in circuits])
+        e, b = C.c_int(), C.c_uint32()
+        ms = (C.c_double * 5)()
+        _check(lib().gkrx_batch_autotune(self._h, n, hs, C.byref(e), C.byref(b), ms), "gkrx_batch_autotune")
+        keys = [(BATCH_PER_PROOF, 256)] + [(BATCH_LOCKSTEP, x) for x in (64, 128, 256, 512)]
+        return e.value, b.value, {k: ms[i] for i, k in enumerate(keys)}
 
 
 class Circuit:

@@ -106,11 +106,26 @@ int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap,
 /* n independent proofs at once (no reference counterpart: the reference proves one
  * proof per thread, benchmarks/bench_keccak.cpp:32-42).  Proof i is byte-identical to
  * gkrx_prove on circuits[i].  The transcript prefixes run on a pool of host threads, then
- * one launch proves every circuit, one workgroup per proof (circuits must be at most
- * 2^20 wide per layer).  proofs[i] holds proof_caps[i] >= gkrx_proof_size bytes;
+ * the device proves every circuit with the context's batch engine (circuits must be at
+ * most 2^20 wide per layer).  proofs[i] holds proof_caps[i] >= gkrx_proof_size bytes;
  * claimed_v[i] receives R * L u32. */
 int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, uint8_t *const *proofs,
                      const size_t *proof_caps, size_t *proof_lens, uint32_t *const *claimed_v);
+
+/* Batch engines (no reference counterpart).  PER_PROOF: one launch, one workgroup runs a
+ * whole proof.  LOCKSTEP (the default): the batch advances one op of every proof per
+ * step; a step is one launch of the wide ops (each proof's op on as many workgroups of
+ * `block` threads as its tables need) and one launch of the transcript steps.  block is
+ * 64, 128, 256 or 512.  Results do not depend on the choice. */
+#define GKRX_BATCH_PER_PROOF 1
+#define GKRX_BATCH_LOCKSTEP 2
+int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block);
+int gkrx_ctx_get_batch_engine(const gkrx_ctx *ctx, int *engine, uint32_t *block);
+/* Autotuner: times the device part of a batch of these circuits (witnesses set) under
+ * every engine and block size, keeps the fastest in the context, and reports it.
+ * ms_each (optional, 5 entries): PER_PROOF, then LOCKSTEP with block 64, 128, 256, 512. */
+int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, int *engine, uint32_t *block,
+                        double *ms_each);
 /* Verifier::verify (LinearGKR.hpp:104-149), on the host.  Returns 1 = accepted,
  * 0 = rejected, negative = error. */
 int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v, const uint8_t *proof,

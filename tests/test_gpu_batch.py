@@ -94,3 +94,57 @@ def test_result_independent_of_batch_mix(ctx):
     assert bytes(mix[7][1]) == bytes(alone[0][1]) and np.array_equal(mix[7][0], alone[0][0])
     for c in circs:
         c.close()
+
+
+ENGINES = [(gkrx.BATCH_PER_PROOF, 256)] + [(gkrx.BATCH_LOCKSTEP, b) for b in (64, 128, 256, 512)]
+
+
+@pytest.fixture(scope="module")
+def mixed_plain(ctx):
+    """A mix with grid rounds (k >= 12), tail-only layers and two-variable layers (the
+    narrowest the loader accepts), and each circuit's plain proof."""
+    specs = _mixed(6, 14, 2, 14, 1, 5) + [(960, 3, 15), (961, 1, 2), (962, 2, 12)]
+    circs = _circuits(ctx, specs)
+    prover = gkrx.Prover(ctx)
+    plain = [prover.prove(c) for c in circs]
+    yield circs, plain
+    for c in circs:
+        c.close()
+
+
+@pytest.mark.parametrize("engine,block", ENGINES, ids=[f"e{e}_b{b}" for e, b in ENGINES])
+def test_every_engine_equals_plain(ctx, mixed_plain, engine, block):
+    print(f"[gpu] batch engine {engine} block {block} vs plain proofs", flush=True)
+    circs, plain = mixed_plain
+    saved = ctx.batch_engine()
+    ctx.set_batch_engine(engine, block)
+    try:
+        batch = gkrx.Prover(ctx).prove_batch(circs)
+        again = gkrx.Prover(ctx).prove_batch(circs[::-1])[::-1]  # buffers reused, other order
+    finally:
+        ctx.set_batch_engine(*saved)
+    for (claimed, proof), (c1, p1), (c2, p2) in zip(batch, plain, again):
+        assert np.array_equal(claimed, c1) and bytes(proof) == bytes(p1)
+        assert np.array_equal(c2, c1) and bytes(p2) == bytes(p1)
+
+
+def test_autotune_picks_an_engine_and_keeps_results(ctx, mixed_plain):
+    print("[gpu] batch autotune", flush=True)
+    circs, plain = mixed_plain
+    saved = ctx.batch_engine()
+    try:
+        engine, block, ms = ctx.batch_autotune(circs)
+        assert (engine, block) in ENGINES and ctx.batch_engine() == (engine, block)
+        assert all(v > 0 for v in ms.values()) and ms[(engine, block)] == min(ms.values())
+        batch = gkrx.Prover(ctx).prove_batch(circs)
+    finally:
+        ctx.set_batch_engine(*saved)
+    for (claimed, proof), (c1, p1) in zip(batch, plain):
+        assert np.array_equal(claimed, c1) and bytes(proof) == bytes(p1)
+
+
+def test_engine_arguments_checked(ctx):
+    with pytest.raises(gkrx.GkrxError):
+        ctx.set_batch_engine(gkrx.BATCH_LOCKSTEP, 96)
+    with pytest.raises(gkrx.GkrxError):
+        ctx.set_batch_engine(7, 256)
