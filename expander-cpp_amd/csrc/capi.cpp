@@ -644,34 +644,29 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
     for (uint32_t i = 0; i < n; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t b) { return circuits[a]->n[0] > circuits[b]->n[0]; });
-    std::atomic<uint32_t> next{0};
-    auto worker = [&] {
-        for (uint32_t k; (k = next++) < n;) {
-            const uint32_t i = order[k];
-            gkrx_circuit *c = circuits[i];
-            const uint32_t mlast = c->m[c->D - 1];
-            pre[i].rz1.resize((size_t)ctx->R * mlast);
-            host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()),
-                                   ((size_t)1 << c->n[0]) * S, ctx->R, mlast, (uint32_t)ctx->cfg.grinding_bits,
-                                   pre[i].grind, pre[i].rz1.data(), pre[i].tstate);
-        }
-    };
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < std::min<unsigned>(hw, n); t++) pool.emplace_back(worker);
-    worker();
-    for (auto &th : pool) th.join();
+    parallel_for(n, 16, [&](size_t k) {
+        const uint32_t i = order[k];
+        gkrx_circuit *c = circuits[i];
+        const uint32_t mlast = c->m[c->D - 1];
+        pre[i].rz1.resize((size_t)ctx->R * mlast);
+        host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()), ((size_t)1 << c->n[0]) * S,
+                               ctx->R, mlast, (uint32_t)ctx->cfg.grinding_bits, pre[i].grind, pre[i].rz1.data(),
+                               pre[i].tstate);
+    });
     const auto t1 = std::chrono::steady_clock::now();
     std::vector<BatchItem> items(n);
     for (uint32_t i = 0; i < n; i++) {
         gkrx_circuit *c = circuits[i];
         const size_t commit_len = ((size_t)1 << c->n[0]) * S;
-        memcpy(proofs[i], c->witness.data(), commit_len);
         items[i] = BatchItem{&c->dev, pre[i].rz1.data(), pre[i].tstate, pre[i].grind, tail_bytes(ctx, c),
                              proofs[i] + commit_len};
     }
     int rc = dev_prove_batch(items, ctx->R, kind, ctx->batch, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr);
     if (rc) return rc;
+    // while the device proves: the commitments (the input layers) into the proof buffers
+    parallel_for(n, 16, [&](size_t i) {
+        memcpy(proofs[i], circuits[i]->witness.data(), ((size_t)1 << circuits[i]->n[0]) * S);
+    });
     GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     dev_batch_collect(items, ctx->batch);
     const uint32_t *cl = ctx->batch.claims_host;

@@ -4,8 +4,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gkrx.h"
@@ -25,6 +28,22 @@ void set_error(const std::string &msg);
     } while (0)
 
 constexpr uint32_t kAddMark = 0xffffffffu;  // "no second input" in a CSR entry
+
+// Runs fn(i) for i < n on up to max_threads host threads (the caller is one of them).
+template <typename F>
+void parallel_for(size_t n, unsigned max_threads, F &&fn)
+{
+    const unsigned hw = std::max(1u, std::min(max_threads, std::thread::hardware_concurrency()));
+    const unsigned nt = (unsigned)std::min<size_t>(hw, n);
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t i; (i = next++) < n;) fn(i);
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+}
 constexpr int kMaxVars = 30;
 
 // One layer in device form.  Every table is in the reference's natural index order
@@ -73,12 +92,21 @@ enum BatchEngine : int { kBatchPerProof = 1, kBatchLockstep = 2 };
 struct BatchScratch {
     int engine = kBatchLockstep;
     uint32_t block = 256;           // workgroup size of the lockstep work kernel: 64, 128, 256 or 512
+    // lockstep: run the serial chains on a second stream beside the next step's wide ops
+    bool chain_stream = true;
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_main = nullptr, ev_aux = nullptr;
     uint32_t *pool = nullptr;
     uint64_t pool_u32 = 0;
     uint8_t *meta = nullptr;        // job table, order or step schedule
     uint64_t meta_bytes = 0;
     uint8_t *host_meta = nullptr;   // pinned staging of the metadata and seeds
     uint64_t host_meta_bytes = 0;
+    // what `meta` holds: the key of the batch it was built for (circuit addresses and
+    // shapes, transcript positions, pool), its size, section offsets and the step records
+    std::vector<uint64_t> meta_key;
+    uint64_t meta_used = 0, o_order = 0, o_tasks = 0, o_starts = 0;
+    std::vector<uint32_t> meta_steps;
     uint32_t *claims_host = nullptr;
     uint64_t claims_cap = 0;
     // pinned staging of the proof tails: the device copies land here (a copy into pageable

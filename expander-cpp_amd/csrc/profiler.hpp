@@ -25,6 +25,7 @@ enum KernelClass : int {
     kKRoundFin,
     kKLsWork,
     kKLsFin,
+    kKLsChain,
     kKClassCount
 };
 
@@ -34,7 +35,7 @@ inline const char *kernel_class_name(int c)
                                               "k_dot",          "k_round_ph1_first", "k_round_ph1",
                                               "k_round_ph2_first", "k_round_ph2", "k_tail_ph1",
                                               "k_tail_ph2",     "k_prove_batch", "k_round_fin",
-                                              "k_ls_work",      "k_ls_fin"};
+                                              "k_ls_work",      "k_ls_fin",  "k_ls_chain"};
     return (c >= 0 && c < kKClassCount) ? names[c] : "";
 }
 

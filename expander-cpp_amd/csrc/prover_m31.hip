@@ -960,10 +960,12 @@ __global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *_
 // then per layer: eqZ, g_x, the phase-1 rounds, eqX, h_y, the phase-2 rounds); step t runs
 // op t of every proof that has one.  One launch of k_ls_work runs the wide ops of a step,
 // each task on its own range of workgroups, so a wide proof gets as many workgroups as its
-// tables need; then one launch of k_ls_fin runs the serial ends: the partial sums and
-// transcript step of a grid round, the claims, and the rounds of a phase whose tables are
-// small.  The launch boundaries order the steps, so no workgroup waits on another.  The
-// schedule depends only on the circuit shapes: the host builds it and uploads it once.
+// tables need; then one launch of k_ls_fin runs the short ends (the partial sums and
+// transcript step of a grid round, the claims).  The serial chains (the rounds of a phase
+// whose tables are small, and the last round of a phase) run in k_ls_chain on a second
+// stream, beside the next step's wide ops.  Launch boundaries and events order the steps,
+// so no workgroup waits on another.  The schedule depends only on the circuit shapes: the
+// host builds it and uploads it once.
 enum LsOp : uint32_t {
     kOpSeed = 0,  // work, 1 block: transcript seed from the host prefix
     kOpFwd,       // work: forward pass of layer l
@@ -975,8 +977,8 @@ enum LsOp : uint32_t {
     kOpHy,        // work: h_y of layer l
     kOpRoundFin,  // fin, 1 block: partials and transcript step of grid round k
     kOpDotFin,    // fin, 1 block: claimed_v
-    kOpTailRep,   // fin, R blocks: rounds k .. kend - 1 of a phase, one block per rep
-    kOpLast,      // fin, 1 block: the last round of a phase
+    kOpTailRep,   // chain, R blocks: rounds k .. kend - 1 of a phase, one block per rep
+    kOpLast,      // chain, 1 block: the last round of a phase
 };
 
 struct LsFwd {
@@ -1240,7 +1242,8 @@ __device__ TailParams ls_tail_params(const BatchJob &J, uint32_t l, uint32_t pha
     return tp;
 }
 
-static_assert(kFinThreads == kRepThreads, "k_ls_fin runs tail_rep_body");
+// The short ends of a step (one workgroup each): the partials and transcript step after a
+// grid round, and the claims.
 template <int L>
 __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restrict__ jobs,
                                                         const LsTask *__restrict__ tasks,
@@ -1252,20 +1255,30 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restri
     const uint2 f = ls_find(start, ntasks, blockIdx.x);
     const LsTask &T = tasks[f.x];
     const BatchJob &J = jobs[T.job];
-    switch (T.op) {
-    case kOpRoundFin: {
-        reduce_partials<L>(J.partials, T.nb, R * Q, sums);
+    reduce_partials<L>(J.partials, T.nb, R * Q, sums);
+    if (T.op == kOpRoundFin) {
         TsArgs ts = ls_ts<L>(J, T.l, T.phase, R, hash_kind);
         ts.k = T.k;
         transcript_step<L>(sums, ts, rr);
-        break;
+    } else if (threadIdx.x < R * L) {
+        J.claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
     }
-    case kOpDotFin: {
-        reduce_partials<L>(J.partials, T.nb, R * Q, sums);
-        if (threadIdx.x < R * L) J.claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
-        break;
-    }
-    case kOpTailRep: {
+}
+
+// The serial chains of a step: the small-table rounds of a phase (one workgroup per
+// repetition) and the last round of a phase.  They run on a second stream, beside the
+// wide ops of the next step of the other proofs.
+static_assert(kFinThreads == kRepThreads, "k_ls_chain runs tail_rep_body");
+template <int L>
+__global__ void __launch_bounds__(kFinThreads) k_ls_chain(const BatchJob *__restrict__ jobs,
+                                                          const LsTask *__restrict__ tasks,
+                                                          const uint32_t *__restrict__ start, uint32_t ntasks,
+                                                          uint32_t R, int hash_kind)
+{
+    const uint2 f = ls_find(start, ntasks, blockIdx.x);
+    const LsTask &T = tasks[f.x];
+    const BatchJob &J = jobs[T.job];
+    if (T.op == kOpTailRep) {
         TailRepParams q{};
         q.t = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, T.k);
         q.kend = T.kend;
@@ -1273,16 +1286,10 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restri
         q.hs0 = T.hs0;
         if (T.phase == 1) tail_rep_body<L, false>(q, f.y);
         else tail_rep_body<L, true>(q, f.y);
-        break;
-    }
-    case kOpLast: {
+    } else {
         const TailParams tp = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, J.layers[T.l].n - 1);
         if (T.phase == 1) phase_rounds<L, false>(tp);
         else phase_rounds<L, true>(tp);
-        break;
-    }
-    default:
-        break;
     }
 }
 
@@ -1886,10 +1893,14 @@ void ls_launch_work(uint32_t BS, uint32_t blocks, hipStream_t st, const BatchJob
     }
 }
 
+// One step: its work, fin and chain tasks (first task, first start, count, workgroups).
 struct LsStep {
-    uint32_t w_task, w_start, w_n, w_blocks;  // work tasks of the step and their workgroups
-    uint32_t f_task, f_start, f_n, f_blocks;  // fin tasks
+    uint32_t w_task, w_start, w_n, w_blocks;
+    uint32_t f_task, f_start, f_n, f_blocks;
+    uint32_t c_task, c_start, c_n, c_blocks;
 };
+
+bool ls_is_chain(uint32_t op) { return op == kOpTailRep || op == kOpLast; }
 
 inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
@@ -1919,6 +1930,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         if (bs.pool) (void)hipFree(bs.pool);
         bs.pool = nullptr;
         bs.pool_u32 = 0;
+        bs.meta_key.clear();  // the job tables point into the pool
         GKRX_HIP_CHECK(hipMalloc(&bs.pool, need * 4));
         bs.pool_u32 = need;
     }
@@ -1933,129 +1945,172 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     uint32_t *tails = bs.pool + job_off[nj];
     uint32_t *claims = tails + tail_off[nj];
     uint32_t *seeds = claims + nj * (uint64_t)R * L;
-    std::vector<BatchJob> hjobs(nj);
+    // seed records: rz1 in chal-slot layout, transcript state, grind bytes
     std::vector<uint32_t> hseed(nj * seed_u32, 0);
-    std::vector<std::pair<uint64_t, uint32_t>> work(nj);
     for (size_t i = 0; i < nj; i++) {
         const DevCircuit &dc = *items[i].dc;
-        const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
-        const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
-        uint32_t *at = bs.pool + job_off[i];
-        auto take = [&](uint64_t n) { uint32_t *p = at; at += (n + 63) & ~63ull; return p; };
-        BatchJob &J = hjobs[i];
-        J = BatchJob{};
-        J.D = dc.D;
-        J.layers = dc.d_layers;
-        J.values = dc.values;
-        J.eqz_rs = maxM; J.eqz = take(R * maxM);
-        J.eqx_rs = maxN; J.eqx = take(R * maxN);
-        J.hg0_rs = maxN * L; J.hg0 = take(R * maxN * L);
-        J.hgA_rs = N2 * L; J.hgA = take(R * N2 * L);
-        J.hgB_rs = N4 * L; J.hgB = take(R * N4 * L);
-        J.fA_rs = N2 * L; J.fA = take(R * N2 * L);
-        J.fB_rs = N4 * L; J.fB = take(R * N4 * L);
-        J.s0_rs = N1; J.s0 = take(R * N1);
-        J.sA_rs = N2; J.sA = take(R * N2);
-        J.sB_rs = N4; J.sB = take(R * N4);
-        J.chal = take((uint64_t)(dc.D + 1) * R * kChalStride);
-        J.ab = take(2ull * (dc.D + 1));
-        J.vxy = take(2ull * R * L);
-        J.tstate = take(16);
-        J.partials = take((uint64_t)R * 3 * L * kMaxBlocksPerRep);
-        J.tail = tails + tail_off[i];
-        J.claims = claims + i * (uint64_t)R * L;
-        J.seed = seeds + i * seed_u32;
-        // seed record: rz1 in chal-slot layout, transcript state, grind bytes
         uint32_t *sd = hseed.data() + i * seed_u32;
         const uint32_t mlast = dc.layers[dc.D - 1].m;
         for (uint32_t j = 0; j < R; j++)
             for (uint32_t v = 0; v < mlast; v++) sd[j * kChalStride + v] = items[i].rz1[(size_t)j * mlast + v];
         memcpy(sd + R * kChalStride, items[i].tstate, 40);
         memcpy(sd + R * kChalStride + 10, items[i].grind, 32);
-        uint64_t w = 0;
-        for (uint32_t l = 0; l < dc.D; l++) w += (1ull << dc.layers[l].n);
-        work[i] = {w, (uint32_t)i};
     }
-    // per-proof engine: workgroup order, largest proof first; lockstep: the step schedule
-    std::vector<uint32_t> horder;
-    std::vector<LsTask> tasks;
-    std::vector<uint32_t> starts;
-    std::vector<LsStep> steps;
-    if (!lockstep) {
-        std::stable_sort(work.begin(), work.end(), [](auto &a, auto &b) { return a.first > b.first; });
-        horder.resize(nj);
-        for (size_t i = 0; i < nj; i++) horder[i] = work[i].second;
-    } else {
-        std::vector<std::vector<LsEntry>> ops(nj);
-        size_t nsteps = 0;
+    // The job table and the schedule depend only on the circuits' shapes, their device
+    // addresses, the transcript position after the prefix and the pool: a batch with the
+    // same key reuses the metadata already on the device.
+    std::vector<uint64_t> key = {(uint64_t)lockstep, BS, L, R, nj, (uint64_t)(uintptr_t)bs.pool};
+    for (size_t i = 0; i < nj; i++) {
+        const DevCircuit &dc = *items[i].dc;
+        key.insert(key.end(), {(uint64_t)(uintptr_t)dc.values, (uint64_t)(uintptr_t)dc.csr_pool,
+                               (uint64_t)(uintptr_t)dc.d_layers, dc.D, items[i].tstate[0], items[i].tstate[1],
+                               items[i].tail_bytes});
+        for (uint32_t l = 0; l < dc.D; l++) {
+            const LayerDev &ly = dc.layers[l];
+            key.insert(key.end(), {((uint64_t)ly.n << 32) | ly.m, ly.v_off, ly.out_off, (uint64_t)(uintptr_t)ly.fw_ptr,
+                                   (uint64_t)(uintptr_t)ly.fw_ent, (uint64_t)(uintptr_t)ly.gx_ptr,
+                                   (uint64_t)(uintptr_t)ly.gx_ent, (uint64_t)(uintptr_t)ly.hy_ptr,
+                                   (uint64_t)(uintptr_t)ly.hy_ent});
+        }
+    }
+    const bool reuse = bs.meta && bs.meta_used && key == bs.meta_key;
+    if (!reuse) {
+        bs.meta_key.clear();
+        bs.meta_used = 0;
+        std::vector<BatchJob> hjobs(nj);
+        std::vector<std::pair<uint64_t, uint32_t>> work(nj);
         for (size_t i = 0; i < nj; i++) {
-            ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, items[i].tstate[0], items[i].tstate[1], ops[i]);
-            nsteps = std::max(nsteps, ops[i].size());
+            const DevCircuit &dc = *items[i].dc;
+            const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
+            const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
+            uint32_t *at = bs.pool + job_off[i];
+            auto take = [&](uint64_t n) { uint32_t *p = at; at += (n + 63) & ~63ull; return p; };
+            BatchJob &J = hjobs[i];
+            J = BatchJob{};
+            J.D = dc.D;
+            J.layers = dc.d_layers;
+            J.values = dc.values;
+            J.eqz_rs = maxM; J.eqz = take(R * maxM);
+            J.eqx_rs = maxN; J.eqx = take(R * maxN);
+            J.hg0_rs = maxN * L; J.hg0 = take(R * maxN * L);
+            J.hgA_rs = N2 * L; J.hgA = take(R * N2 * L);
+            J.hgB_rs = N4 * L; J.hgB = take(R * N4 * L);
+            J.fA_rs = N2 * L; J.fA = take(R * N2 * L);
+            J.fB_rs = N4 * L; J.fB = take(R * N4 * L);
+            J.s0_rs = N1; J.s0 = take(R * N1);
+            J.sA_rs = N2; J.sA = take(R * N2);
+            J.sB_rs = N4; J.sB = take(R * N4);
+            J.chal = take((uint64_t)(dc.D + 1) * R * kChalStride);
+            J.ab = take(2ull * (dc.D + 1));
+            J.vxy = take(2ull * R * L);
+            J.tstate = take(16);
+            J.partials = take((uint64_t)R * 3 * L * kMaxBlocksPerRep);
+            J.tail = tails + tail_off[i];
+            J.claims = claims + i * (uint64_t)R * L;
+            J.seed = seeds + i * seed_u32;
+            uint64_t w = 0;
+            for (uint32_t l = 0; l < dc.D; l++) w += (1ull << dc.layers[l].n);
+            work[i] = {w, (uint32_t)i};
         }
-        for (size_t s = 0; s < nsteps; s++) {
-            LsStep S{};
-            S.w_task = (uint32_t)tasks.size();
-            S.w_start = (uint32_t)starts.size();
-            uint32_t blocks = 0;
-            for (int pass = 0; pass < 2; pass++)  // gather tasks first (multiples of 8 blocks)
-                for (size_t i = 0; i < nj; i++) {
-                    if (s >= ops[i].size()) continue;
-                    const LsEntry &e = ops[i][s];
-                    for (uint32_t q = 0; q < e.nw; q++)
-                        if (ls_is_gather(e.w[q].op) == (pass == 0)) {
-                            starts.push_back(blocks);
-                            tasks.push_back(e.w[q]);
-                            blocks += ls_work_blocks(e.w[q], R);
-                        }
-                }
-            S.w_n = (uint32_t)tasks.size() - S.w_task;
-            S.w_blocks = blocks;
-            starts.push_back(blocks);
-            S.f_task = (uint32_t)tasks.size();
-            S.f_start = (uint32_t)starts.size();
-            blocks = 0;
+        // per-proof engine: workgroup order, largest proof first; lockstep: the step schedule
+        std::vector<uint32_t> horder;
+        std::vector<LsTask> tasks;
+        std::vector<uint32_t> starts;
+        std::vector<LsStep> steps;
+        if (!lockstep) {
+            std::stable_sort(work.begin(), work.end(), [](auto &x, auto &y) { return x.first > y.first; });
+            horder.resize(nj);
+            for (size_t i = 0; i < nj; i++) horder[i] = work[i].second;
+        } else {
+            std::vector<std::vector<LsEntry>> ops(nj);
+            size_t nsteps = 0, ntask = 0;
             for (size_t i = 0; i < nj; i++) {
-                if (s >= ops[i].size() || !ops[i][s].nf) continue;
-                starts.push_back(blocks);
-                tasks.push_back(ops[i][s].f);
-                blocks += ls_fin_blocks(ops[i][s].f, R);
+                ops[i].reserve((size_t)items[i].dc->D * (8 + 2 * items[i].dc->max_n));
+                ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, items[i].tstate[0], items[i].tstate[1],
+                           ops[i]);
+                nsteps = std::max(nsteps, ops[i].size());
+                ntask += 2 * ops[i].size();
             }
-            S.f_n = (uint32_t)tasks.size() - S.f_task;
-            S.f_blocks = blocks;
-            starts.push_back(blocks);
-            steps.push_back(S);
+            tasks.reserve(ntask);
+            starts.reserve(ntask + 3 * nsteps);
+            steps.reserve(nsteps);
+            for (size_t s = 0; s < nsteps; s++) {
+                LsStep S{};
+                S.w_task = (uint32_t)tasks.size();
+                S.w_start = (uint32_t)starts.size();
+                uint32_t blocks = 0;
+                for (int pass = 0; pass < 2; pass++)  // gather tasks first (multiples of 8 blocks)
+                    for (size_t i = 0; i < nj; i++) {
+                        if (s >= ops[i].size()) continue;
+                        const LsEntry &e = ops[i][s];
+                        for (uint32_t q = 0; q < e.nw; q++)
+                            if (ls_is_gather(e.w[q].op) == (pass == 0)) {
+                                starts.push_back(blocks);
+                                tasks.push_back(e.w[q]);
+                                blocks += ls_work_blocks(e.w[q], R);
+                            }
+                    }
+                S.w_n = (uint32_t)tasks.size() - S.w_task;
+                S.w_blocks = blocks;
+                starts.push_back(blocks);
+                for (int chain = 0; chain < 2; chain++) {
+                    uint32_t &t0 = chain ? S.c_task : S.f_task, &s0 = chain ? S.c_start : S.f_start;
+                    uint32_t &cnt = chain ? S.c_n : S.f_n, &nb = chain ? S.c_blocks : S.f_blocks;
+                    t0 = (uint32_t)tasks.size();
+                    s0 = (uint32_t)starts.size();
+                    blocks = 0;
+                    for (size_t i = 0; i < nj; i++) {
+                        if (s >= ops[i].size() || !ops[i][s].nf || ls_is_chain(ops[i][s].f.op) != (chain == 1))
+                            continue;
+                        starts.push_back(blocks);
+                        tasks.push_back(ops[i][s].f);
+                        blocks += ls_fin_blocks(ops[i][s].f, R);
+                    }
+                    cnt = (uint32_t)tasks.size() - t0;
+                    nb = blocks;
+                    starts.push_back(blocks);
+                }
+                steps.push_back(S);
+            }
         }
+        // device metadata: [jobs][order][tasks][starts]; the pinned staging adds the seeds
+        bs.o_order = align256(nj * sizeof(BatchJob));
+        bs.o_tasks = align256(bs.o_order + 4 * horder.size());
+        bs.o_starts = align256(bs.o_tasks + sizeof(LsTask) * tasks.size());
+        const uint64_t meta_need = align256(bs.o_starts + 4 * starts.size());
+        if (meta_need > bs.meta_bytes) {
+            if (bs.meta) (void)hipFree(bs.meta);
+            bs.meta = nullptr;
+            bs.meta_bytes = 0;
+            GKRX_HIP_CHECK(hipMalloc(&bs.meta, meta_need));
+            bs.meta_bytes = meta_need;
+        }
+        if (meta_need + hseed.size() * 4 > bs.host_meta_bytes) {
+            if (bs.host_meta) (void)hipHostFree(bs.host_meta);
+            bs.host_meta = nullptr;
+            bs.host_meta_bytes = 0;
+            GKRX_HIP_CHECK(hipHostMalloc(&bs.host_meta, meta_need + hseed.size() * 4, hipHostMallocDefault));
+            bs.host_meta_bytes = meta_need + hseed.size() * 4;
+        }
+        memcpy(bs.host_meta, hjobs.data(), nj * sizeof(BatchJob));
+        if (!horder.empty()) memcpy(bs.host_meta + bs.o_order, horder.data(), 4 * horder.size());
+        if (!tasks.empty()) memcpy(bs.host_meta + bs.o_tasks, tasks.data(), sizeof(LsTask) * tasks.size());
+        if (!starts.empty()) memcpy(bs.host_meta + bs.o_starts, starts.data(), 4 * starts.size());
+        GKRX_HIP_CHECK(hipMemcpyAsync(bs.meta, bs.host_meta, meta_need, hipMemcpyHostToDevice, st));
+        bs.meta_steps.resize(steps.size() * 12);
+        if (!steps.empty()) memcpy(bs.meta_steps.data(), steps.data(), steps.size() * sizeof(LsStep));
+        bs.meta_used = meta_need;
+        bs.meta_key.swap(key);
     }
-    // device metadata: [jobs][order][tasks][starts]; the pinned staging adds the seeds
-    const uint64_t o_order = align256(nj * sizeof(BatchJob));
-    const uint64_t o_tasks = align256(o_order + 4 * horder.size());
-    const uint64_t o_starts = align256(o_tasks + sizeof(LsTask) * tasks.size());
-    const uint64_t meta_need = align256(o_starts + 4 * starts.size());
-    const uint64_t host_need = meta_need + nj * seed_u32 * 4;
-    if (meta_need > bs.meta_bytes) {
-        if (bs.meta) (void)hipFree(bs.meta);
-        bs.meta = nullptr;
-        bs.meta_bytes = 0;
-        GKRX_HIP_CHECK(hipMalloc(&bs.meta, meta_need));
-        bs.meta_bytes = meta_need;
-    }
-    if (host_need > bs.host_meta_bytes) {
-        if (bs.host_meta) (void)hipHostFree(bs.host_meta);
-        bs.host_meta = nullptr;
-        bs.host_meta_bytes = 0;
-        GKRX_HIP_CHECK(hipHostMalloc(&bs.host_meta, host_need, hipHostMallocDefault));
-        bs.host_meta_bytes = host_need;
-    }
-    memcpy(bs.host_meta, hjobs.data(), nj * sizeof(BatchJob));
-    if (!horder.empty()) memcpy(bs.host_meta + o_order, horder.data(), 4 * horder.size());
-    if (!tasks.empty()) memcpy(bs.host_meta + o_tasks, tasks.data(), sizeof(LsTask) * tasks.size());
-    if (!starts.empty()) memcpy(bs.host_meta + o_starts, starts.data(), 4 * starts.size());
-    memcpy(bs.host_meta + meta_need, hseed.data(), hseed.size() * 4);
-    GKRX_HIP_CHECK(hipMemcpyAsync(bs.meta, bs.host_meta, meta_need, hipMemcpyHostToDevice, st));
-    GKRX_HIP_CHECK(hipMemcpyAsync(seeds, bs.host_meta + meta_need, hseed.size() * 4, hipMemcpyHostToDevice, st));
+    static_assert(sizeof(LsStep) == 12 * 4, "step records are 12 words");
+    std::vector<LsStep> steps(bs.meta_steps.size() / 12);
+    if (!steps.empty()) memcpy(steps.data(), bs.meta_steps.data(), steps.size() * sizeof(LsStep));
+    // the seeds change with every batch
+    memcpy(bs.host_meta + bs.meta_used, hseed.data(), hseed.size() * 4);
+    GKRX_HIP_CHECK(hipMemcpyAsync(seeds, bs.host_meta + bs.meta_used, hseed.size() * 4, hipMemcpyHostToDevice, st));
     const BatchJob *djobs = reinterpret_cast<const BatchJob *>(bs.meta);
     if (!lockstep) {
-        const uint32_t *dorder = reinterpret_cast<const uint32_t *>(bs.meta + o_order);
+        const uint32_t *dorder = reinterpret_cast<const uint32_t *>(bs.meta + bs.o_order);
         timed(pf, kKBatch, st, [&] {
             if (L == 8)
                 hipLaunchKernelGGL(k_prove_batch<8>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder,
@@ -2065,9 +2120,34 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                                    R, hash_kind);
         });
     } else {
-        const LsTask *dtasks = reinterpret_cast<const LsTask *>(bs.meta + o_tasks);
-        const uint32_t *dstarts = reinterpret_cast<const uint32_t *>(bs.meta + o_starts);
+        const LsTask *dtasks = reinterpret_cast<const LsTask *>(bs.meta + bs.o_tasks);
+        const uint32_t *dstarts = reinterpret_cast<const uint32_t *>(bs.meta + bs.o_starts);
+        // chains on the second stream: step t's chains wait for the main stream's step t - 1
+        // (ev_main), and the main stream's step t + 1 waits for them (ev_aux)
+        hipStream_t cs = st;
+        if (bs.chain_stream) {
+            if (!bs.aux) {
+                GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux, hipStreamNonBlocking));
+                GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_main, hipEventDisableTiming));
+                GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_aux, hipEventDisableTiming));
+            }
+            cs = bs.aux;
+            GKRX_HIP_CHECK(hipEventRecord(bs.ev_main, st));  // the uploads above
+        }
+        bool chains_pending = false;
         for (const LsStep &S : steps) {
+            if (S.c_n) {
+                if (cs != st) GKRX_HIP_CHECK(hipStreamWaitEvent(cs, bs.ev_main, 0));
+                timed(pf, kKLsChain, cs, [&] {
+                    if (L == 8)
+                        hipLaunchKernelGGL(k_ls_chain<8>, dim3(S.c_blocks), dim3(kFinThreads), 0, cs, djobs,
+                                           dtasks + S.c_task, dstarts + S.c_start, S.c_n, R, hash_kind);
+                    else
+                        hipLaunchKernelGGL(k_ls_chain<16>, dim3(S.c_blocks), dim3(kFinThreads), 0, cs, djobs,
+                                           dtasks + S.c_task, dstarts + S.c_start, S.c_n, R, hash_kind);
+                });
+            }
+            if (chains_pending) GKRX_HIP_CHECK(hipStreamWaitEvent(st, bs.ev_aux, 0));
             if (S.w_n)
                 timed(pf, kKLsWork, st, [&] {
                     if (L == 8) ls_launch_work<8>(BS, S.w_blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R);
@@ -2082,7 +2162,17 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                         hipLaunchKernelGGL(k_ls_fin<16>, dim3(S.f_blocks), dim3(kFinThreads), 0, st, djobs,
                                            dtasks + S.f_task, dstarts + S.f_start, S.f_n, R, hash_kind);
                 });
+            chains_pending = false;
+            if (cs != st) {
+                GKRX_HIP_CHECK(hipEventRecord(bs.ev_main, st));
+                if (S.c_n) {
+                    GKRX_HIP_CHECK(hipEventRecord(bs.ev_aux, cs));
+                    chains_pending = true;
+                }
+            }
         }
+        // the copies below (and the next batch, which synchronizes st) follow every chain
+        if (chains_pending) GKRX_HIP_CHECK(hipStreamWaitEvent(st, bs.ev_aux, 0));
     }
     GKRX_HIP_CHECK(hipGetLastError());
     GKRX_HIP_CHECK(hipMemcpyAsync(bs.tails_host, tails, tail_off[nj] * 4, hipMemcpyDeviceToHost, st));
@@ -2094,8 +2184,9 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
 
 void dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &bs)
 {
-    for (size_t i = 0; i < items.size(); i++)
+    parallel_for(items.size(), 16, [&](size_t i) {
         memcpy(items[i].tail_out, bs.tails_host + bs.tail_stage_off[i] * 4, items[i].tail_bytes);
+    });
 }
 
 uint64_t dev_batch_job_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes)
@@ -2114,6 +2205,12 @@ int dev_batch_supported(const DevCircuit &dc)
 
 void dev_free_batch(BatchScratch &bs)
 {
+    if (bs.aux) {
+        (void)hipStreamSynchronize(bs.aux);
+        (void)hipStreamDestroy(bs.aux);
+    }
+    if (bs.ev_main) (void)hipEventDestroy(bs.ev_main);
+    if (bs.ev_aux) (void)hipEventDestroy(bs.ev_aux);
     if (bs.pool) (void)hipFree(bs.pool);
     if (bs.meta) (void)hipFree(bs.meta);
     if (bs.host_meta) (void)hipHostFree(bs.host_meta);

@@ -148,3 +148,30 @@ def test_engine_arguments_checked(ctx):
         ctx.set_batch_engine(gkrx.BATCH_LOCKSTEP, 96)
     with pytest.raises(gkrx.GkrxError):
         ctx.set_batch_engine(7, 256)
+
+
+def test_metadata_reuse_follows_witness_and_circuit_changes(ctx):
+    """A repeated batch reuses the job table and schedule on the device; the proofs must
+    still follow new witnesses and a circuit replaced by another of the same shape."""
+    print("[gpu] batch metadata reuse", flush=True)
+    specs = _mixed(7, 6, 3, 13, 1, 4)
+    circs = _circuits(ctx, specs)
+    prover = gkrx.Prover(ctx)
+    first = prover.prove_batch(circs)
+    for (s, d, k), c in zip(specs, circs):
+        c.set_witness(gkrx.synth_witness(s + 77, 1 << k, 16))
+    second = prover.prove_batch(circs)
+    for (cl, pr), c in zip(second, circs):
+        c1, p1 = prover.prove(c)
+        assert np.array_equal(cl, c1) and bytes(pr) == bytes(p1)
+    assert all(bytes(a[1]) != bytes(b[1]) for a, b in zip(first, second))
+    s, d, k = specs[2]
+    circs[2].close()
+    circs[2] = gkrx.Circuit.from_layers(ctx, gkrx.synth_circuit(s + 5, d, k, 300, False))
+    circs[2].set_witness(gkrx.synth_witness(s, 1 << k, 16))
+    third = prover.prove_batch(circs)
+    for (cl, pr), c in zip(third, circs):
+        c1, p1 = prover.prove(c)
+        assert np.array_equal(cl, c1) and bytes(pr) == bytes(p1)
+    for c in circs:
+        c.close()
