@@ -644,15 +644,44 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
     for (uint32_t i = 0; i < n; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t b) { return circuits[a]->n[0] > circuits[b]->n[0]; });
-    parallel_for(n, 16, [&](size_t k) {
-        const uint32_t i = order[k];
-        gkrx_circuit *c = circuits[i];
-        const uint32_t mlast = c->m[c->D - 1];
-        pre[i].rz1.resize((size_t)ctx->R * mlast);
-        host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()), ((size_t)1 << c->n[0]) * S,
-                               ctx->R, mlast, (uint32_t)ctx->cfg.grinding_bits, pre[i].grind, pre[i].rz1.data(),
-                               pre[i].tstate);
-    });
+    for (uint32_t i = 0; i < n; i++) pre[i].rz1.resize((size_t)ctx->R * circuits[i]->m[circuits[i]->D - 1]);
+    if (kind == 1 && host_keccak_x8()) {
+        // groups of up to eight commitments of one length: eight sponges per core
+        std::vector<std::pair<uint32_t, uint32_t>> groups;  // [first, last) in `order`
+        for (uint32_t k = 0; k < n;) {
+            uint32_t e = k + 1;
+            while (e < n && e - k < 8 && circuits[order[e]]->n[0] == circuits[order[k]]->n[0]) e++;
+            groups.push_back({k, e});
+            k = e;
+        }
+        parallel_for(groups.size(), 16, [&](size_t g) {
+            const uint32_t k0 = groups[g].first, cnt = groups[g].second - k0;
+            const uint8_t *commits[8];
+            uint32_t ml[8], *rz[8];
+            uint8_t grind[8][32];
+            uint32_t ts[8][10];
+            for (uint32_t l = 0; l < cnt; l++) {
+                gkrx_circuit *c = circuits[order[k0 + l]];
+                commits[l] = reinterpret_cast<const uint8_t *>(c->witness.data());
+                ml[l] = c->m[c->D - 1];
+                rz[l] = pre[order[k0 + l]].rz1.data();
+            }
+            host_transcript_prefix_x8(cnt, commits, ((size_t)1 << circuits[order[k0]]->n[0]) * S, ctx->R, ml,
+                                      (uint32_t)ctx->cfg.grinding_bits, grind, rz, ts);
+            for (uint32_t l = 0; l < cnt; l++) {
+                memcpy(pre[order[k0 + l]].grind, grind[l], 32);
+                memcpy(pre[order[k0 + l]].tstate, ts[l], 40);
+            }
+        });
+    } else {
+        parallel_for(n, 16, [&](size_t k) {
+            const uint32_t i = order[k];
+            gkrx_circuit *c = circuits[i];
+            host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()),
+                                   ((size_t)1 << c->n[0]) * S, ctx->R, c->m[c->D - 1],
+                                   (uint32_t)ctx->cfg.grinding_bits, pre[i].grind, pre[i].rz1.data(), pre[i].tstate);
+        });
+    }
     const auto t1 = std::chrono::steady_clock::now();
     std::vector<BatchItem> items(n);
     for (uint32_t i = 0; i < n; i++) {
@@ -905,4 +934,18 @@ extern "C" int gkrx_hash(int fs_hash, const uint8_t *in, size_t len, uint8_t out
     if (fs_hash != GKRX_FS_SHA256 && fs_hash != GKRX_FS_KECCAK256) return GKRX_ERR_CONFIG;
     host_hash(fs_hash == GKRX_FS_KECCAK256 ? 1 : 0, in, len, out);
     return GKRX_OK;
+}
+
+extern "C" int gkrx_hash_many(int fs_hash, uint32_t n, const uint8_t *const *in, size_t len, uint8_t *out)
+{
+    if ((n && (!in || !out))) return GKRX_ERR_ARG;
+    for (uint32_t i = 0; i < n; i++)
+        if (!in[i] && len) return GKRX_ERR_ARG;
+    if (fs_hash == GKRX_FS_KECCAK256) {
+        host_keccak256_many(n, in, len, out);
+        return host_keccak_x8() ? 1 : 0;
+    }
+    if (fs_hash != GKRX_FS_SHA256) return GKRX_ERR_CONFIG;
+    for (uint32_t i = 0; i < n; i++) host_sha256(in[i], len, out + 32 * (size_t)i);
+    return 0;
 }

@@ -152,6 +152,99 @@ void keccak_f(uint64_t A[25])
     }
 }
 
+// ---- eight Keccak-256 sponges at once: lane l of every state word belongs to message l
+// (AVX-512; the permutation is the one above, word for word, on eight states).
+__attribute__((target("avx512f"))) void keccak_f_x8(__m512i A[25])
+{
+    for (int round = 0; round < 24; round++) {
+        __m512i C[5], D[5], B[25];
+        for (int x = 0; x < 5; x++)
+            C[x] = _mm512_ternarylogic_epi64(_mm512_ternarylogic_epi64(A[x], A[x + 5], A[x + 10], 0x96), A[x + 15],
+                                             A[x + 20], 0x96);  // a ^ b ^ c
+        for (int x = 0; x < 5; x++) D[x] = _mm512_xor_si512(C[(x + 4) % 5], _mm512_rol_epi64(C[(x + 1) % 5], 1));
+        B[0] = _mm512_xor_si512(A[0], D[0]);
+        B[10] = _mm512_rol_epi64(_mm512_xor_si512(A[1], D[1]), 1);
+        B[20] = _mm512_rol_epi64(_mm512_xor_si512(A[2], D[2]), 62);
+        B[5] = _mm512_rol_epi64(_mm512_xor_si512(A[3], D[3]), 28);
+        B[15] = _mm512_rol_epi64(_mm512_xor_si512(A[4], D[4]), 27);
+        B[16] = _mm512_rol_epi64(_mm512_xor_si512(A[5], D[0]), 36);
+        B[1] = _mm512_rol_epi64(_mm512_xor_si512(A[6], D[1]), 44);
+        B[11] = _mm512_rol_epi64(_mm512_xor_si512(A[7], D[2]), 6);
+        B[21] = _mm512_rol_epi64(_mm512_xor_si512(A[8], D[3]), 55);
+        B[6] = _mm512_rol_epi64(_mm512_xor_si512(A[9], D[4]), 20);
+        B[7] = _mm512_rol_epi64(_mm512_xor_si512(A[10], D[0]), 3);
+        B[17] = _mm512_rol_epi64(_mm512_xor_si512(A[11], D[1]), 10);
+        B[2] = _mm512_rol_epi64(_mm512_xor_si512(A[12], D[2]), 43);
+        B[12] = _mm512_rol_epi64(_mm512_xor_si512(A[13], D[3]), 25);
+        B[22] = _mm512_rol_epi64(_mm512_xor_si512(A[14], D[4]), 39);
+        B[23] = _mm512_rol_epi64(_mm512_xor_si512(A[15], D[0]), 41);
+        B[8] = _mm512_rol_epi64(_mm512_xor_si512(A[16], D[1]), 45);
+        B[18] = _mm512_rol_epi64(_mm512_xor_si512(A[17], D[2]), 15);
+        B[3] = _mm512_rol_epi64(_mm512_xor_si512(A[18], D[3]), 21);
+        B[13] = _mm512_rol_epi64(_mm512_xor_si512(A[19], D[4]), 8);
+        B[14] = _mm512_rol_epi64(_mm512_xor_si512(A[20], D[0]), 18);
+        B[24] = _mm512_rol_epi64(_mm512_xor_si512(A[21], D[1]), 2);
+        B[9] = _mm512_rol_epi64(_mm512_xor_si512(A[22], D[2]), 61);
+        B[19] = _mm512_rol_epi64(_mm512_xor_si512(A[23], D[3]), 56);
+        B[4] = _mm512_rol_epi64(_mm512_xor_si512(A[24], D[4]), 14);
+        for (int y = 0; y < 25; y += 5)
+            for (int x = 0; x < 5; x++)  // a ^ (~b & c)
+                A[y + x] = _mm512_ternarylogic_epi64(B[y + x], B[y + (x + 1) % 5], B[y + (x + 2) % 5], 0xD2);
+        A[0] = _mm512_xor_si512(A[0], _mm512_set1_epi64((long long)kRC[round]));
+    }
+}
+
+// Keccak-256 of eight messages of the same length; lane l reads in[l].
+__attribute__((target("avx512f"))) void keccak256_x8(const uint8_t *const in[8], size_t len, uint64_t out[4][8])
+{
+    constexpr size_t kRate = 136;
+    __m512i A[25];
+    for (auto &a : A) a = _mm512_setzero_si512();
+    const __m512i base = _mm512_set_epi64((long long)(uintptr_t)in[7], (long long)(uintptr_t)in[6],
+                                          (long long)(uintptr_t)in[5], (long long)(uintptr_t)in[4],
+                                          (long long)(uintptr_t)in[3], (long long)(uintptr_t)in[2],
+                                          (long long)(uintptr_t)in[1], (long long)(uintptr_t)in[0]);
+    size_t off = 0;
+    for (; len - off >= kRate; off += kRate) {
+        for (int i = 0; i < 17; i++) {  // word i of the block of every lane (absolute addresses)
+            const __m512i at = _mm512_add_epi64(base, _mm512_set1_epi64((long long)(off + 8 * i)));
+            A[i] = _mm512_xor_si512(A[i], _mm512_i64gather_epi64(at, nullptr, 1));
+        }
+        keccak_f_x8(A);
+    }
+    alignas(64) uint8_t blk[8][kRate];
+    const size_t rem = len - off;
+    for (int l = 0; l < 8; l++) {
+        memset(blk[l], 0, kRate);
+        memcpy(blk[l], in[l] + off, rem);
+        blk[l][rem] ^= 0x01;
+        blk[l][kRate - 1] ^= 0x80;
+    }
+    const __m512i lanes = _mm512_set_epi64(7 * kRate, 6 * kRate, 5 * kRate, 4 * kRate, 3 * kRate, 2 * kRate, kRate, 0);
+    for (int i = 0; i < 17; i++) {
+        const __m512i at = _mm512_add_epi64(lanes, _mm512_set1_epi64(8 * i));
+        A[i] = _mm512_xor_si512(A[i], _mm512_i64gather_epi64(at, &blk[0][0], 1));
+    }
+    keccak_f_x8(A);
+    for (int i = 0; i < 4; i++) _mm512_storeu_si512(out[i], A[i]);
+}
+
+// Keccak-256 of eight 32-byte messages held as 4 words per lane, in place.
+__attribute__((target("avx512f"))) void keccak256_32_x8(__m512i m[4])
+{
+    __m512i A[25];
+    for (auto &a : A) a = _mm512_setzero_si512();
+    for (int i = 0; i < 4; i++) A[i] = m[i];
+    A[4] = _mm512_set1_epi64(0x01);                      // pad byte after the 32 message bytes
+    A[16] = _mm512_set1_epi64((long long)0x8000000000000000ULL);  // last rate byte
+    keccak_f_x8(A);
+    for (int i = 0; i < 4; i++) m[i] = A[i];
+}
+
+bool cpu_has_avx512() { return __builtin_cpu_supports("avx512f"); }
+
+const bool kHasAvx512 = cpu_has_avx512();
+
 }  // namespace
 
 void host_sha256(const uint8_t *in, size_t len, uint8_t out[32])
@@ -254,6 +347,98 @@ void host_transcript_prefix(int kind, const uint8_t *commit, size_t commit_len, 
     tstate[0] = 32;
     tstate[1] = pending ? 0 : 32;
     memcpy(&tstate[2], digest, 32);
+}
+
+bool host_keccak_x8() { return kHasAvx512; }
+
+void host_keccak256_many(uint32_t n, const uint8_t *const *in, size_t len, uint8_t *out)
+{
+    for (uint32_t g = 0; g < n; g += 8) {
+        const uint32_t cnt = n - g < 8 ? n - g : 8;
+        if (!kHasAvx512) {
+            for (uint32_t l = 0; l < cnt; l++) host_keccak256(in[g + l], len, out + 32 * (size_t)(g + l));
+            continue;
+        }
+        const uint8_t *p[8];
+        for (uint32_t l = 0; l < 8; l++) p[l] = in[g + (l < cnt ? l : 0)];
+        uint64_t o[4][8];
+        keccak256_x8(p, len, o);
+        for (uint32_t l = 0; l < cnt; l++)
+            for (int i = 0; i < 4; i++) memcpy(out + 32 * (size_t)(g + l) + 8 * i, &o[i][l], 8);
+    }
+}
+
+__attribute__((target("avx512f"))) static void prefix_x8_body(const uint8_t *const commit[8], size_t commit_len,
+                                                              uint32_t grinding_bits, uint8_t grind_out[8][32],
+                                                              uint8_t digest_out[8][32])
+{
+    // challenge_fs(8) after the commitment: the first hashes the commitment, the others the
+    // previous digest; each contributes 4 LE bytes of M31 to the grind buffer
+    uint64_t o[4][8];
+    keccak256_x8(commit, commit_len, o);
+    __m512i d[4], hb[4];
+    for (int i = 0; i < 4; i++) d[i] = _mm512_loadu_si512(o[i]);
+    alignas(64) uint32_t h[8][8];  // h[i][lane]
+    for (int i = 0; i < 8; i++) {
+        if (i > 0) keccak256_32_x8(d);
+        alignas(64) uint64_t w0[8];
+        _mm512_store_si512(w0, d[0]);
+        for (int l = 0; l < 8; l++) {
+            uint32_t x = (uint32_t)w0[l];
+            x = (x & 0x7fffffffu) + (x >> 31);
+            if (x >= 0x7fffffffu) x -= 0x7fffffffu;
+            h[i][l] = x;
+        }
+    }
+    alignas(64) uint64_t hw[4][8];
+    for (int i = 0; i < 4; i++)
+        for (int l = 0; l < 8; l++) hw[i][l] = (uint64_t)h[2 * i][l] | ((uint64_t)h[2 * i + 1][l] << 32);
+    for (int i = 0; i < 4; i++) hb[i] = _mm512_load_si512(hw[i]);
+    for (uint32_t r = 0; r < (1u << grinding_bits); r++) keccak256_32_x8(hb);
+    for (int i = 0; i < 4; i++) {
+        _mm512_store_si512(hw[i], hb[i]);
+        _mm512_storeu_si512(o[i], d[i]);
+    }
+    for (int l = 0; l < 8; l++)
+        for (int i = 0; i < 4; i++) {
+            memcpy(grind_out[l] + 8 * i, &hw[i][l], 8);
+            memcpy(digest_out[l] + 8 * i, &o[i][l], 8);
+        }
+}
+
+void host_transcript_prefix_x8(uint32_t n, const uint8_t *const *commit, size_t commit_len, uint32_t R,
+                               const uint32_t *mlast, uint32_t grinding_bits, uint8_t (*grind_out)[32],
+                               uint32_t *const *rz1, uint32_t (*tstate)[10])
+{
+    if (!kHasAvx512) {
+        for (uint32_t l = 0; l < n; l++)
+            host_transcript_prefix(1, commit[l], commit_len, R, mlast[l], grinding_bits, grind_out[l], rz1[l],
+                                   tstate[l]);
+        return;
+    }
+    const uint8_t *p[8];
+    for (uint32_t l = 0; l < 8; l++) p[l] = commit[l < n ? l : 0];
+    uint8_t grind[8][32], digest[8][32];
+    prefix_x8_body(p, commit_len, grinding_bits, grind, digest);
+    for (uint32_t l = 0; l < n; l++) {
+        memcpy(grind_out[l], grind[l], 32);
+        // the output challenges, as in host_transcript_prefix (a few dozen short hashes)
+        uint8_t dg[32];
+        bool pending = true;
+        for (uint32_t i = 0; i < mlast[l]; i++)
+            for (uint32_t j = 0; j < R; j++) {
+                if (pending) {
+                    host_keccak256(grind[l], 32, dg);
+                    pending = false;
+                } else {
+                    host_keccak256(dg, 32, dg);
+                }
+                rz1[l][(size_t)j * mlast[l] + i] = digest_to_m31(dg);
+            }
+        tstate[l][0] = 32;
+        tstate[l][1] = pending ? 0 : 32;
+        memcpy(&tstate[l][2], pending ? digest[l] : dg, 32);
+    }
 }
 
 // the digest as a 256-bit little-endian integer reduced mod r

@@ -78,6 +78,7 @@ def lib():
     L.gkrx_synth_witness_m31.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, _u32p]
     L.gkrx_synth_witness_bn254.argtypes = [C.c_uint64, C.c_uint64, _u32p]
     L.gkrx_hash.argtypes = [C.c_int, C.c_char_p, C.c_size_t, C.c_char_p]
+    L.gkrx_hash_many.argtypes = [C.c_int, C.c_uint32, C.POINTER(C.c_char_p), C.c_size_t, C.c_char_p]
     L.gkrx_prove_batch.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_void_p), C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]
     L.gkrx_server_create.argtypes = [vp, C.c_uint64, C.c_uint32, C.POINTER(vp)]
@@ -225,6 +226,18 @@ def host_hash(fs_hash: int, data: bytes) -> bytes:
     out = C.create_string_buffer(32)
     _check(lib().gkrx_hash(fs_hash, data, len(data), out), "gkrx_hash")
     return out.raw
+
+
+def host_hash_many(fs_hash: int, msgs) -> tuple:
+    """Hashes of equal-length messages at once; returns (digests, eight_lane_path_ran)."""
+    n = len(msgs)
+    ln = len(msgs[0]) if n else 0
+    assert all(len(m) == ln for m in msgs)
+    bufs = [C.create_string_buffer(bytes(m), max(ln, 1)) for m in msgs]
+    ptrs = (C.c_char_p * n)(*[C.cast(b, C.c_char_p) for b in bufs])
+    out = C.create_string_buffer(32 * max(n, 1))
+    rc = _check(lib().gkrx_hash_many(fs_hash, n, ptrs, ln, out), "gkrx_hash_many")
+    return [out.raw[32 * i:32 * i + 32] for i in range(n)], rc == 1
 
 
 # ----------------------------------------------------------------- API mirror

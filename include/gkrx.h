@@ -200,6 +200,10 @@ int gkrx_synth_witness_bn254(uint64_t seed, uint64_t n_elems, uint32_t *out);
 
 /* Transcript hashes of the library (host implementations), for tests. */
 int gkrx_hash(int fs_hash, const uint8_t *in, size_t len, uint8_t out[32]);
+/* n messages of one length hashed at once (Keccak-256: eight sponges per core in AVX-512
+ * lanes where the CPU has them -- the path the batched prover's transcript prefixes
+ * take); out receives 32 bytes per message.  Returns 1 when the eight-lane path ran. */
+int gkrx_hash_many(int fs_hash, uint32_t n, const uint8_t *const *in, size_t len, uint8_t *out);
 
 #ifdef __cplusplus
 }

@@ -125,3 +125,16 @@ def test_transcript_framing_cross_platform_scenario(kind):
     d3 = H(d2)
     le = lambda d: struct.unpack("<I", d[:4])[0] % P
     assert (alpha, beta, gamma) == (le(d1), le(d2), le(d3))
+
+
+@pytest.mark.parametrize("length", [0, 1, 31, 32, 135, 136, 137, 272, 1000, 65536 + 5])
+def test_keccak_many_equals_one_at_a_time(length):
+    """The eight-lane (AVX-512) Keccak-256 of the batched prover's transcript prefixes gives
+    the scalar digest of every message, for partial groups too."""
+    rng = random.Random(length)
+    for n in (1, 3, 8, 11):
+        msgs = [bytes(rng.getrandbits(8) for _ in range(length)) for _ in range(n)]
+        got, _ = gkrx.host_hash_many(gkrx.FS_KECCAK256, msgs)
+        assert got == [gkrx.host_hash(gkrx.FS_KECCAK256, m) for m in msgs]
+    got, _ = gkrx.host_hash_many(gkrx.FS_SHA256, msgs)
+    assert got == [hashlib.sha256(m).digest() for m in msgs]
