@@ -176,7 +176,21 @@ def serve_leg(gkrx, fs, m31_state, seconds, budget_gb, slots):
     import random
     import threading
 
-    mctx, mcircs, mgates, m31_s = m31_state
+    mctx, mcircs0, mgates0, m31_s = m31_state
+    # a second resident copy of every M31 circuit (same wiring and witness), so that a
+    # circuit with a request in flight does not hold back arrivals
+    from concurrent.futures import ThreadPoolExecutor
+
+    def clone(spec):
+        s, d, k = spec
+        c = gkrx.Circuit.from_layers(mctx, gkrx.synth_circuit(s, d, k, 700, True))
+        c.set_witness(gkrx.synth_witness(s, 1 << k, 16))
+        return c
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        clones = list(ex.map(clone, batch_specs(len(mcircs0))))
+    mcircs, mgates = mcircs0 + clones, mgates0 + mgates0
+    n0 = len(mcircs0)
     bctx = gkrx.Context(gkrx.Config(fs_hash=fs, field_type=gkrx.FIELD_BN254), device=0)
     rng = random.Random(5)
     bcircs, bgates = [], []
@@ -189,7 +203,7 @@ def serve_leg(gkrx, fs, m31_state, seconds, budget_gb, slots):
         bgates.append(sum(len(m) + len(a) for m, a in layers))
     # reference proofs made alone, for the identity check
     mprover, bprover = gkrx.Prover(mctx), gkrx.Prover(bctx)
-    check_m = {i: mprover.prove(mcircs[i]) for i in range(0, len(mcircs), 37)}
+    check_m = {i: mprover.prove(mcircs[i]) for i in range(0, n0, 37)}
     t0 = time.perf_counter()
     check_b = {i: bprover.prove(c) for i, c in enumerate(bcircs)}
     bn_s = (time.perf_counter() - t0) / len(bcircs)
@@ -197,7 +211,7 @@ def serve_leg(gkrx, fs, m31_state, seconds, budget_gb, slots):
     log(f"serving: {rate:.0f} requests/s for {seconds}s (M31 batched {m31_s * 1e3:.2f} ms/proof, "
         f"BN254 {bn_s * 1e3:.1f} ms/proof alone)")
     budget = int(budget_gb * (1 << 30))
-    # M31: `slots` batch streams + one stream for large proofs; BN254: one stream
+    # M31: `slots` batch streams (+ one for circuits too wide to batch); BN254: one stream
     msrv, bsrv = gkrx.Server(mctx, budget, slots), gkrx.Server(bctx, budget // 8, 1)
     free = {"m": list(range(len(mcircs))), "b": list(range(len(bcircs)))}
     pending = []  # (server, ticket, kind, index, arrival)
@@ -217,7 +231,7 @@ def serve_leg(gkrx, fs, m31_state, seconds, budget_gb, slots):
                     continue
                 cl, pr, _ = r
                 now = time.perf_counter()
-                ref = (check_m if kind == "m" else check_b).get(i)
+                ref = check_m.get(i % n0) if kind == "m" else check_b.get(i)
                 if ref is not None and (bytes(pr) != bytes(ref[1]) or not np.array_equal(cl, ref[0])):
                     mismatches[0] += 1
                 with lock:
@@ -260,10 +274,12 @@ def serve_leg(gkrx, fs, m31_state, seconds, budget_gb, slots):
     msrv.close()
     bsrv.close()
     bctx.close()
+    for c in clones:
+        c.close()
     log(f"served {arrivals} requests in {elapsed:.1f}s, p50 {percentile(lat, 50):.1f} ms, p99 {percentile(lat, 99):.1f} ms")
     return {"workload": f"Poisson arrivals for {seconds}s at 80% of the saturated rate of the mix; 95% M31x16 proofs "
                         f"of the uneven batch, 5% BN254 proofs (2^14..2^16 gates x 8..12 layers); HBM budget "
-                        f"{budget_gb} GB; {slots} M31 slots", "rate_per_s": round(rate, 1), "requests": arrivals,
+                        f"{budget_gb} GB; {slots} M31 batch slots; two resident copies of each M31 circuit", "rate_per_s": round(rate, 1), "requests": arrivals,
             "gates_per_s": round(done_gates[0] / elapsed, 1), "proofs_per_s": round(arrivals / elapsed, 1),
             "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)},
             "m31_batches": ms["batches"], "m31_largest_batch": ms["max_batch"],

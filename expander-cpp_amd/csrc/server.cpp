@@ -10,11 +10,12 @@
 //      HBM they need -- the circuit's resident values and wiring plus the prover tables,
 //      both computed from the circuit shape -- fits the budget.
 //   3. Launch.  Slots are streams with their own scratch.  A batch slot launches every
-//      admitted small M31 request at once (k_prove_batch: one workgroup per proof); a
-//      single slot runs one large M31 proof (or one BN254 proof) with the multi-block
-//      path, which puts the whole GPU on it instead of one workgroup.  Whichever slot is
-//      free takes the next work, so requests that arrive while others run start without
-//      waiting for them, and their kernels overlap on the device.
+//      admitted M31 request at once with the context's batch engine (the lockstep engine
+//      by default: each proof's step on as many workgroups as its tables need); a single
+//      slot runs one BN254 proof, or one M31 proof too wide for the batch engines, with
+//      the multi-block path.  Whichever slot is free takes the next work, so requests that
+//      arrive while others run start without waiting for them, and their kernels overlap
+//      on the device.
 //   4. Completion is observed by polling each slot's event (a stream host-function
 //      callback is only delivered while some host thread is inside the runtime, and the
 //      server's threads sleep between events).
@@ -44,8 +45,9 @@ using Clock = std::chrono::steady_clock;
 namespace {
 
 constexpr uint32_t kMaxBatch = 1024;
-// M31 requests with at least this many table rows over all layers (sum of 2^n) take a
-// single slot: in one workgroup they would run for hundreds of milliseconds.
+// With the one-workgroup-per-proof batch engine, M31 requests with at least this many
+// table rows over all layers (sum of 2^n) take a single slot: in one workgroup they would
+// run for hundreds of milliseconds.
 constexpr uint64_t kBigRows = 1ull << 19;
 constexpr auto kPollPeriod = std::chrono::microseconds(100);
 
@@ -55,7 +57,7 @@ struct SrvReq {
     uint8_t *proof = nullptr;
     uint32_t *claims = nullptr;
     uint64_t bytes = 0;   // admission footprint
-    bool single = false;  // runs on a single slot (large M31, or BN254)
+    bool single = false;  // runs on a single slot (BN254, or M31 outside the batch engine)
     Clock::time_point t_submit;
     int state = 0;        // 0 queued, 1 on the GPU, 2 done
     int rc = 0;
@@ -168,6 +170,10 @@ int launch(gkrx_server *S, Slot &sl)
 {
     gkrx_ctx *ctx = S->ctx;
     const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
+    if (!sl.st) {
+        GKRX_HIP_CHECK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+        GKRX_HIP_CHECK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+    }
     if (sl.single) {
         SrvReq *r = sl.reqs[0];
         gkrx_circuit *c = r->c;
@@ -360,6 +366,12 @@ extern "C" int gkrx_server_create(gkrx_ctx *ctx, uint64_t hbm_budget_bytes, uint
         sl->bs.engine = ctx->batch.engine;  // the context's batch engine (gkrx_ctx_set_batch_engine)
         sl->bs.block = ctx->batch.block;
         sl->bs.chain_stream = false;  // slots already overlap; stay within the hardware queues
+        // an M31 single slot (circuits wider than the batch engines take) gets its stream on
+        // first use, so that an idle one holds no hardware queue
+        if (sl->single && !ctx->bn254) {
+            S->slots.push_back(std::move(sl));
+            continue;
+        }
         if (hipStreamCreateWithFlags(&sl->st, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl->ev, hipEventDisableTiming) != hipSuccess) {
             free_slot(*sl);
@@ -393,7 +405,10 @@ extern "C" int gkrx_server_submit(gkrx_server *S, gkrx_circuit *c, uint8_t *proo
     }
     uint64_t rows = 0;
     for (uint32_t l = 0; l < c->D; l++) rows += 1ull << c->n[l];
-    const bool single = ctx->bn254 || rows >= kBigRows || !dev_batch_supported(c->dev);
+    // the lockstep engine gives a large proof as many workgroups as its tables need, so it
+    // joins a batch; the one-workgroup-per-proof engine sends it to a single slot
+    const bool single = ctx->bn254 || !dev_batch_supported(c->dev) ||
+                        (ctx->batch.engine == kBatchPerProof && rows >= kBigRows);
     const uint64_t bytes = footprint(ctx, c);
     if (bytes > S->budget) {
         set_error("request needs " + std::to_string(bytes) + " bytes of HBM, above the server budget");
