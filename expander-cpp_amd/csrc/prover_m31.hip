@@ -667,11 +667,12 @@ struct TailRepParams {
     uint32_t len0, hs0;  // transcript length and hash start (bytes of the tail) at round k0
 };
 
-// The body of k_tail_rep for repetition `rep`; blockDim is kRepThreads.
-template <int L, bool PH2>
+// The body of k_tail_rep for repetition `rep`; blockDim is NT (a multiple of 64, >= 3L).
+template <int L, bool PH2, uint32_t NT = kRepThreads>
 __device__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
 {
-    constexpr uint32_t LG = L / 4, Q = 3 * L, W = kRepThreads / 64;
+    constexpr uint32_t LG = L / 4, Q = 3 * L, W = NT / 64;
+    static_assert(NT % 64 == 0 && NT >= Q, "tail workgroup shape");
     const TailParams &p = q.t;
     __shared__ uint32_t red[W][3 * L];
     __shared__ uint32_t sums[Q], rr, hb[L + Q];
@@ -690,7 +691,7 @@ __device__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
         uint32_t *hd = ((k & 1) ? p.hA : p.hB) + rep * ((k & 1) ? p.hA_rs : p.hB_rs);
         const uint32_t r = rr;
         uint64_t acc[3][4] = {};
-        for (uint64_t w = t; w < items * LG; w += kRepThreads) {
+        for (uint64_t w = t; w < items * LG; w += NT) {
             if (k == 0) round_item<L, PH2, false>(fs, nullptr, hs, nullptr, w / LG, g, 0u, acc);
             else round_item<L, PH2, true>(fs, fd, hs, hd, w / LG, g, r, acc);
         }
@@ -1265,15 +1266,54 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restri
     }
 }
 
+// The last round of a phase (one pair per repetition, so no reduction across threads)
+// and its transcript step: the ordered x-claim hashing of phase 1, or the layer end of
+// phase 2.  The same work as phase_rounds from k0 = n - 1, for any blockDim of at least
+// 64 R threads.
+template <int L, bool PH2>
+__device__ void last_round_body(const TailParams &p)
+{
+    constexpr uint32_t LG = L / 4, Q = 3 * L;
+    __shared__ uint32_t sums[3 * Q], rr[3];
+    const uint32_t R = p.ts.R, k = p.n - 1, t = threadIdx.x;
+    if (t < R) rr[t] = k > 0 ? p.ts.chal[t * kChalStride + (PH2 ? 32u : 0u) + k - 1] : 0u;
+    __syncthreads();
+    // round k's sources (table k - 1; table 0 is the layer input and h0) and destinations
+    const uint32_t *fs = p.f0, *hs = p.h0;
+    const uint64_t fs_rs = p.f0_rs, hs_rs = p.h0_rs;
+    uint32_t *fd = (k & 1) ? p.fA : p.fB, *hd = (k & 1) ? p.hA : p.hB;
+    const uint64_t fd_rs = (k & 1) ? p.fA_rs : p.fB_rs, hd_rs = (k & 1) ? p.hA_rs : p.hB_rs;
+    if (t < R * LG) {
+        const uint32_t rep = t / LG, g = t % LG;
+        uint64_t acc[3][4] = {};
+        if (k == 0) round_item<L, PH2, false>(fs + rep * fs_rs, nullptr, hs + rep * hs_rs, nullptr, 0, g, 0u, acc);
+        else
+            round_item<L, PH2, true>(fs + rep * fs_rs, fd + rep * fd_rs, hs + rep * hs_rs, hd + rep * hd_rs, 0, g,
+                                     rr[rep], acc);
+#pragma unroll
+        for (int pk = 0; pk < 3; pk++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) sums[rep * Q + pk * L + 4 * g + c] = m31_reduce64(acc[pk][c]);
+    }
+    __syncthreads();
+    TsArgs a = p.ts;
+    a.k = k;
+    a.last = 1;
+    a.fin = k == 0 ? fs : fd;
+    a.fin_rs = k == 0 ? fs_rs : fd_rs;
+    transcript_step<L>(sums, a, rr);
+}
+
 // The serial chains of a step: the small-table rounds of a phase (one workgroup per
 // repetition) and the last round of a phase.  They run on a second stream, beside the
-// wide ops of the next step of the other proofs.
-static_assert(kFinThreads == kRepThreads, "k_ls_chain runs tail_rep_body");
+// wide ops of the next step of the other proofs; small workgroups let several chains share
+// a CU.
+constexpr uint32_t kChainThreads = 256;
 template <int L>
-__global__ void __launch_bounds__(kFinThreads) k_ls_chain(const BatchJob *__restrict__ jobs,
-                                                          const LsTask *__restrict__ tasks,
-                                                          const uint32_t *__restrict__ start, uint32_t ntasks,
-                                                          uint32_t R, int hash_kind)
+__global__ void __launch_bounds__(kChainThreads) k_ls_chain(const BatchJob *__restrict__ jobs,
+                                                            const LsTask *__restrict__ tasks,
+                                                            const uint32_t *__restrict__ start, uint32_t ntasks,
+                                                            uint32_t R, int hash_kind)
 {
     const uint2 f = ls_find(start, ntasks, blockIdx.x);
     const LsTask &T = tasks[f.x];
@@ -1284,12 +1324,12 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_chain(const BatchJob *__rest
         q.kend = T.kend;
         q.len0 = T.len0;
         q.hs0 = T.hs0;
-        if (T.phase == 1) tail_rep_body<L, false>(q, f.y);
-        else tail_rep_body<L, true>(q, f.y);
+        if (T.phase == 1) tail_rep_body<L, false, kChainThreads>(q, f.y);
+        else tail_rep_body<L, true, kChainThreads>(q, f.y);
     } else {
         const TailParams tp = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, J.layers[T.l].n - 1);
-        if (T.phase == 1) phase_rounds<L, false>(tp);
-        else phase_rounds<L, true>(tp);
+        if (T.phase == 1) last_round_body<L, false>(tp);
+        else last_round_body<L, true>(tp);
     }
 }
 
@@ -2140,10 +2180,10 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                 if (cs != st) GKRX_HIP_CHECK(hipStreamWaitEvent(cs, bs.ev_main, 0));
                 timed(pf, kKLsChain, cs, [&] {
                     if (L == 8)
-                        hipLaunchKernelGGL(k_ls_chain<8>, dim3(S.c_blocks), dim3(kFinThreads), 0, cs, djobs,
+                        hipLaunchKernelGGL(k_ls_chain<8>, dim3(S.c_blocks), dim3(kChainThreads), 0, cs, djobs,
                                            dtasks + S.c_task, dstarts + S.c_start, S.c_n, R, hash_kind);
                     else
-                        hipLaunchKernelGGL(k_ls_chain<16>, dim3(S.c_blocks), dim3(kFinThreads), 0, cs, djobs,
+                        hipLaunchKernelGGL(k_ls_chain<16>, dim3(S.c_blocks), dim3(kChainThreads), 0, cs, djobs,
                                            dtasks + S.c_task, dstarts + S.c_start, S.c_n, R, hash_kind);
                 });
             }
