@@ -28,6 +28,9 @@ void set_error(const std::string &msg);
     } while (0)
 
 constexpr uint32_t kAddMark = 0xffffffffu;  // "no second input" in a CSR entry
+// Words per CSR entry.  The g_x and h_y entries carry their own row, so that a tile of rows
+// can spread its entries evenly over a workgroup; 16 bytes load in one instruction.
+constexpr uint32_t kFwWords = 3, kGxWords = 4, kHyWords = 4;
 
 // Runs fn(i) for i < n on up to max_threads host threads (the caller is one of them).
 template <typename F>
@@ -52,11 +55,11 @@ struct LayerDev {
     uint32_t n = 0, m = 0;       // nb_input_vars, nb_output_vars
     uint64_t v_off = 0;          // u32 offset of the layer input values in the value buffer
     uint64_t out_off = 0;        // u32 offset of the layer output values
-    // forward CSR by output index: entry {in0, in1 or kAddMark, coef}
+    // forward CSR by output index: entry {in0, in1 or kAddMark, coef} (kFwWords)
     uint32_t *fw_ptr = nullptr, *fw_ent = nullptr;
-    // g_x CSR by first input x: entry {in1 or kAddMark, z, coef}
+    // g_x CSR by first input x: entry {in1 or kAddMark, z, coef, x} (kGxWords)
     uint32_t *gx_ptr = nullptr, *gx_ent = nullptr;
-    // h_y CSR by second input y (mul gates only): entry {x, z, coef}
+    // h_y CSR by second input y (mul gates only): entry {x, z, coef, y} (kHyWords)
     uint32_t *hy_ptr = nullptr, *hy_ent = nullptr;
 };
 

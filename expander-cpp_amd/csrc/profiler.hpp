@@ -26,6 +26,11 @@ enum KernelClass : int {
     kKLsWork,
     kKLsFin,
     kKLsChain,
+    kKLsFwd,
+    kKLsGx,
+    kKLsHy,
+    kKLsEq,
+    kKLsRound,
     kKClassCount
 };
 
@@ -35,7 +40,9 @@ inline const char *kernel_class_name(int c)
                                               "k_dot",          "k_round_ph1_first", "k_round_ph1",
                                               "k_round_ph2_first", "k_round_ph2", "k_tail_ph1",
                                               "k_tail_ph2",     "k_prove_batch", "k_round_fin",
-                                              "k_ls_work",      "k_ls_fin",  "k_ls_chain"};
+                                              "k_ls_work",      "k_ls_fin",  "k_ls_chain",
+                                              "k_ls_work:forward", "k_ls_work:g_x", "k_ls_work:h_y",
+                                              "k_ls_work:eq",   "k_ls_work:round"};
     return (c >= 0 && c < kKClassCount) ? names[c] : "";
 }
 

@@ -166,7 +166,8 @@ __global__ void __launch_bounds__(kBnThreads) k_bn_gx(uint32_t *__restrict__ hg,
         Fr acc = fr_zero();
         const uint32_t e1 = ptr[x + 1];
         for (uint32_t e = ptr[x]; e < e1; e++) {
-            const uint32_t b = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+            const uint32_t b = ent[kGxWords * (uint64_t)e], z = ent[kGxWords * (uint64_t)e + 1],
+                           c = ent[kGxWords * (uint64_t)e + 2];
             Fr term = ld_fr(eqz + 8 * (uint64_t)z);
             if (b != kAddMark) term = fr_mont_mul(term, ld_fr(v + 8 * (uint64_t)b));
             acc = fr_add(acc, coef_mul(term, c));
@@ -185,7 +186,8 @@ __global__ void __launch_bounds__(kBnThreads) k_bn_hy(uint32_t *__restrict__ s, 
         Fr acc = fr_zero();
         const uint32_t e1 = ptr[y + 1];
         for (uint32_t e = ptr[y]; e < e1; e++) {
-            const uint32_t x = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+            const uint32_t x = ent[kHyWords * (uint64_t)e], z = ent[kHyWords * (uint64_t)e + 1],
+                           c = ent[kHyWords * (uint64_t)e + 2];
             acc = fr_add(acc, coef_mul(fr_mont_mul(ld_fr(eqz + 8 * (uint64_t)z), ld_fr(eqx + 8 * (uint64_t)x)), c));
         }
         st_fr(s + 8 * y, acc);

@@ -158,37 +158,60 @@ __global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, uin
 }
 
 // ------------------------------------------------------------------ g_x and h_y
+// g_x and h_y as segmented reductions over tiles of rows.  The wiring entries of a tile
+// are contiguous in the CSR and each carries its row, so they are spread evenly over the
+// workgroup whatever the fan-in of each row; every entry's contributions are added into
+// u64 LDS accumulators (sums of canonical values: exact, and order-free, so the result
+// does not depend on the schedule), then the workgroup writes every row of the tile (zero
+// where no gate has that input).
+//
 // hg_j[x] = sum over the gates with first input x of v[y] * (c * eqZ_j[z]) (mul) or
 // c * eqZ_j[z] (add, broadcast to every lane), for every rep j at once: each wiring entry
-// and each gathered v[y] row is read once for all R repetitions.  Work item w: x = w / LG,
-// lanes 4g .. 4g+3.
-template <int L>
-__device__ __forceinline__ void gx_item(uint32_t *__restrict__ hg, uint64_t hg_rs, const uint32_t *__restrict__ v,
-                                        const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
-                                        const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
-                                        uint32_t R, uint64_t w)
+// and each gathered v[y] row is read once for all R repetitions.  A tile for a workgroup
+// of BS threads is tile_lanes(BS) lane-rows (4 per thread, at most 1024: 64 rows of 16
+// lanes); its u64 accumulators are [rep][row][lane].
+constexpr uint32_t tile_lanes(uint32_t bs) { return 4 * bs < 1024 ? 4 * bs : 1024; }
+template <int L, int BS>
+constexpr uint32_t gx_rows() { return tile_lanes(BS) / L; }
+constexpr uint32_t kGxAccWords = 3 * 1024;
+
+template <int L, int BS>
+__device__ void gx_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ hg, uint64_t hg_rs,
+                        const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
+                        const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N, uint32_t R,
+                        uint32_t x0)
 {
-    constexpr uint32_t LG = L / 4;
-    const uint32_t x = (uint32_t)(w / LG), g = (uint32_t)(w % LG);
-    uint4 acc[3] = {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
-    const uint32_t e1 = ptr[x + 1];
-    for (uint32_t e = ptr[x]; e < e1; e++) {
-        const uint32_t b = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
-        uint32_t ce[3];
+    constexpr uint32_t LG = L / 4, ROWS = gx_rows<L, BS>();
+    const uint32_t t = threadIdx.x;
+    const uint32_t rows = N - x0 < ROWS ? N - x0 : ROWS;
+    for (uint32_t i = t; i < R * ROWS * L; i += BS) acc[i] = 0;
+    __syncthreads();
+    const uint32_t e0 = ptr[x0], e1 = ptr[x0 + rows];
+    for (uint64_t w = t; w < (uint64_t)(e1 - e0) * LG; w += BS) {
+        const uint64_t e = e0 + w / LG;
+        const uint32_t g = (uint32_t)(w % LG);
+        const uint4 en = ld4(ent + kGxWords * e);  // {in1 or kAddMark, z, coef, x}
+        const uint4 vb = en.x != kAddMark ? ld4(v + (uint64_t)en.x * L + 4 * g) : make_uint4(1u, 1u, 1u, 1u);
+        unsigned long long *a = acc + (uint64_t)(en.w - x0) * L + 4 * g;
 #pragma unroll
-        for (uint32_t j = 0; j < 3; j++) ce[j] = j < R ? m31_mul(c, eqz[j * eqz_rs + z]) : 0u;
-        if (b != kAddMark) {
-            const uint4 vb = ld4(v + (uint64_t)b * L + 4 * g);
-#pragma unroll
-            for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], mul4s(vb, ce[j]));
-        } else {
-#pragma unroll
-            for (uint32_t j = 0; j < 3; j++) acc[j] = add4(acc[j], make_uint4(ce[j], ce[j], ce[j], ce[j]));
-        }
+        for (uint32_t j = 0; j < 3; j++)
+            if (j < R) {
+                const uint4 p = mul4s(vb, m31_mul(en.z, eqz[j * eqz_rs + en.y]));
+                unsigned long long *aj = a + j * ROWS * L;
+                atomicAdd(aj, (unsigned long long)p.x);
+                atomicAdd(aj + 1, (unsigned long long)p.y);
+                atomicAdd(aj + 2, (unsigned long long)p.z);
+                atomicAdd(aj + 3, (unsigned long long)p.w);
+            }
     }
-#pragma unroll
-    for (uint32_t j = 0; j < 3; j++)
-        if (j < R) st4(hg + j * hg_rs + (uint64_t)x * L + 4 * g, acc[j]);
+    __syncthreads();
+    for (uint32_t i = t; i < R * rows * LG; i += BS) {
+        const uint32_t j = i / (rows * LG), rem = i % (rows * LG), r = rem / LG, g = rem % LG;
+        const unsigned long long *aj = acc + (j * ROWS + r) * L + 4 * g;
+        st4(hg + j * hg_rs + (uint64_t)(x0 + r) * L + 4 * g,
+            make_uint4(m31_reduce64(aj[0]), m31_reduce64(aj[1]), m31_reduce64(aj[2]), m31_reduce64(aj[3])));
+    }
+    __syncthreads();
 }
 
 template <int L>
@@ -197,31 +220,43 @@ __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint
                                                  uint64_t eqz_rs, const uint32_t *__restrict__ ptr,
                                                  const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
 {
-    constexpr uint32_t LG = L / 4;
-    const uint64_t total = (uint64_t)N * LG;
-    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    for (uint64_t w = tile * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
-        gx_item<L>(hg, hg_rs, v, eqz, eqz_rs, ptr, ent, R, w);
+    __shared__ unsigned long long acc[kGxAccWords];
+    constexpr uint32_t ROWS = gx_rows<L, kThreads>();
+    const uint64_t tiles = (N + ROWS - 1) / ROWS;
+    for (uint64_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x)
+        gx_tile<L, kThreads>(acc, hg, hg_rs, v, eqz, eqz_rs, ptr, ent, N, R, (uint32_t)(tile * ROWS));
 }
 
 // s_j[y] = sum over the mul gates with second input y of eqZ_j[z] * eqX_j[x] * c, for every
-// rep j at once (phase-2 hg_j = vx_j * s_j).
-__device__ __forceinline__ void hy_item(uint32_t *__restrict__ s, uint64_t s_rs, const uint32_t *__restrict__ eqz,
-                                        uint64_t eqz_rs, const uint32_t *__restrict__ eqx, uint64_t eqx_rs,
-                                        const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
-                                        uint32_t R, uint64_t y)
+// rep j at once (phase-2 hg_j = vx_j * s_j); a tile is tile_lanes(BS) rows.
+constexpr uint32_t kHyAccWords = 3 * 1024;
+
+template <int BS>
+__device__ void hy_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ s, uint64_t s_rs,
+                        const uint32_t *__restrict__ eqz, uint64_t eqz_rs, const uint32_t *__restrict__ eqx,
+                        uint64_t eqx_rs, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
+                        uint32_t N, uint32_t R, uint32_t y0)
 {
-    uint32_t acc[3] = {0, 0, 0};
-    const uint32_t e1 = ptr[y + 1];
-    for (uint32_t e = ptr[y]; e < e1; e++) {
-        const uint32_t x = ent[3 * (uint64_t)e], z = ent[3 * (uint64_t)e + 1], c = ent[3 * (uint64_t)e + 2];
+    constexpr uint32_t kHyRows = tile_lanes(BS);
+    const uint32_t t = threadIdx.x;
+    const uint32_t rows = N - y0 < kHyRows ? N - y0 : kHyRows;
+    for (uint32_t i = t; i < R * kHyRows; i += BS) acc[i] = 0;
+    __syncthreads();
+    const uint32_t e0 = ptr[y0], e1 = ptr[y0 + rows];
+    for (uint32_t w = t; w < e1 - e0; w += BS) {
+        const uint4 en = ld4(ent + kHyWords * (uint64_t)(e0 + w));  // {x, z, coef, y}
 #pragma unroll
         for (uint32_t j = 0; j < 3; j++)
-            if (j < R) acc[j] = m31_add(acc[j], m31_mul(m31_mul(eqz[j * eqz_rs + z], eqx[j * eqx_rs + x]), c));
+            if (j < R)
+                atomicAdd(acc + j * kHyRows + (en.w - y0),
+                          (unsigned long long)m31_mul(m31_mul(eqz[j * eqz_rs + en.y], eqx[j * eqx_rs + en.x]), en.z));
     }
-#pragma unroll
-    for (uint32_t j = 0; j < 3; j++)
-        if (j < R) s[j * s_rs + y] = acc[j];
+    __syncthreads();
+    for (uint32_t i = t; i < R * rows; i += BS) {
+        const uint32_t j = i / rows, r = i % rows;
+        s[j * s_rs + y0 + r] = m31_reduce64(acc[j * kHyRows + r]);
+    }
+    __syncthreads();
 }
 
 __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint64_t s_rs,
@@ -230,9 +265,11 @@ __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint6
                                                  const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
                                                  uint32_t N, uint32_t R)
 {
-    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    for (uint64_t y = tile * kThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kThreads)
-        hy_item(s, s_rs, eqz, eqz_rs, eqx, eqx_rs, ptr, ent, R, y);
+    __shared__ unsigned long long acc[kHyAccWords];
+    constexpr uint32_t ROWS = tile_lanes(kThreads);
+    const uint64_t tiles = (N + ROWS - 1) / ROWS;
+    for (uint64_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x)
+        hy_tile<kThreads>(acc, s, s_rs, eqz, eqz_rs, eqx, eqx_rs, ptr, ent, N, R, (uint32_t)(tile * ROWS));
 }
 
 // ------------------------------------------------------------------ reductions
@@ -888,28 +925,15 @@ __device__ void block_claims(const uint32_t *__restrict__ vals, const uint32_t *
     __syncthreads();
 }
 
-template <int L>
-__device__ void block_gx(uint32_t *__restrict__ hg, uint64_t hg_rs, const uint32_t *__restrict__ v,
-                         const uint32_t *__restrict__ eqz, uint64_t eqz_rs, const uint32_t *__restrict__ ptr,
-                         const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
-{
-    constexpr uint32_t LG = L / 4;
-    for (uint64_t w = threadIdx.x; w < (uint64_t)N * LG; w += blockDim.x)
-        gx_item<L>(hg, hg_rs, v, eqz, eqz_rs, ptr, ent, R, w);
-}
-
-__device__ void block_hy(uint32_t *__restrict__ s, uint64_t s_rs, const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
-                         const uint32_t *__restrict__ eqx, uint64_t eqx_rs, const uint32_t *__restrict__ ptr,
-                         const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
-{
-    for (uint32_t y = threadIdx.x; y < N; y += blockDim.x) hy_item(s, s_rs, eqz, eqz_rs, eqx, eqx_rs, ptr, ent, R, y);
-}
+// LDS accumulators shared by the g_x and h_y tiles of one workgroup
+constexpr uint32_t kTileAccWords = kGxAccWords > kHyAccWords ? kGxAccWords : kHyAccWords;
 
 template <int L>
 __global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *__restrict__ jobs,
                                                                const uint32_t *__restrict__ order, uint32_t R,
                                                                int hash_kind)
 {
+    __shared__ unsigned long long tacc[kTileAccWords];
     const BatchJob J = jobs[order[blockIdx.x]];
     const uint32_t D = J.D;
     auto chal_slot = [&](uint32_t l) { return J.chal + (uint64_t)l * R * kChalStride; };
@@ -935,7 +959,8 @@ __global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *_
         uint32_t *chal_l = chal_slot(l);
         block_eq(J.eqz, J.eqz_rs, chal_slot(l + 1), 0u, 32u, ab_slot(l + 1), m, R);
         if (l == (int)D - 1) block_claims<L>(J.values + ly.out_off, J.eqz, J.eqz_rs, 1u << m, R, J.claims);
-        block_gx<L>(J.hg0, J.hg0_rs, V, J.eqz, J.eqz_rs, ly.gx_ptr, ly.gx_ent, 1u << n, R);
+        for (uint32_t x0 = 0; x0 < (1u << n); x0 += gx_rows<L, kBatchThreads>())
+            gx_tile<L, kBatchThreads>(tacc, J.hg0, J.hg0_rs, V, J.eqz, J.eqz_rs, ly.gx_ptr, ly.gx_ent, 1u << n, R, x0);
         __syncthreads();
         TsArgs ts{};
         ts.R = R; ts.chal = chal_l; ts.vx = J.vxy; ts.vy = J.vxy + R * L;
@@ -948,7 +973,9 @@ __global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *_
         tp.ts.phase = 1;
         phase_rounds<L, false>(tp);
         block_eq(J.eqx, J.eqx_rs, chal_l, 0u, 0u, nullptr, n, R);
-        block_hy(J.s0, J.s0_rs, J.eqz, J.eqz_rs, J.eqx, J.eqx_rs, ly.hy_ptr, ly.hy_ent, 1u << n, R);
+        for (uint32_t y0 = 0; y0 < (1u << n); y0 += tile_lanes(kBatchThreads))
+            hy_tile<kBatchThreads>(tacc, J.s0, J.s0_rs, J.eqz, J.eqz_rs, J.eqx, J.eqx_rs, ly.hy_ptr, ly.hy_ent, 1u << n,
+                                   R, y0);
         __syncthreads();
         tp.h0 = J.s0; tp.h0_rs = J.s0_rs; tp.hA = J.sA; tp.hB = J.sB; tp.hA_rs = J.sA_rs; tp.hB_rs = J.sB_rs;
         tp.ts.phase = 2;
@@ -1120,10 +1147,14 @@ __device__ void ls_round(const RoundParams &p, uint32_t nb, uint32_t lb)
 
 template <int L, int BS>
 __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ jobs, const LsTask *__restrict__ tasks,
-                                                const uint32_t *__restrict__ start, uint32_t ntasks, uint32_t R)
+                                                const uint32_t *__restrict__ start, uint32_t ntasks, uint32_t R,
+                                                uint32_t block_base)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
-    const uint2 f = ls_find(start, ntasks, blockIdx.x);
+    __shared__ unsigned long long tacc[3 * tile_lanes(BS)];
+    // block_base: the launch covers the step's workgroups from block_base on (a multiple of
+    // 8, so that blockIdx.x and the step's workgroup index sit on the same XCD)
+    const uint2 f = ls_find(start, ntasks, blockIdx.x + block_base);
     const LsTask &T = tasks[f.x];
     const uint32_t lb = f.y, t = threadIdx.x;
     switch (T.op) {
@@ -1157,16 +1188,18 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
     }
     case kOpGx: {
         const LsGx &a = T.gx;
-        const uint64_t total = (uint64_t)a.N * LG, tile = xcd_tile(lb, T.nb);
-        for (uint64_t w = tile * BS + t; w < total; w += (uint64_t)T.nb * BS)
-            gx_item<L>(a.hg, a.hg_rs, a.v, a.eqz, a.eqz_rs, a.ptr, a.ent, R, w);
+        constexpr uint32_t ROWS = gx_rows<L, BS>();
+        const uint32_t tiles = (a.N + ROWS - 1) / ROWS;
+        for (uint32_t tile = (uint32_t)xcd_tile(lb, T.nb); tile < tiles; tile += T.nb)
+            gx_tile<L, BS>(tacc, a.hg, a.hg_rs, a.v, a.eqz, a.eqz_rs, a.ptr, a.ent, a.N, R, tile * ROWS);
         break;
     }
     case kOpHy: {
         const LsHy &a = T.hy;
-        const uint64_t tile = xcd_tile(lb, T.nb);
-        for (uint64_t y = tile * BS + t; y < a.N; y += (uint64_t)T.nb * BS)
-            hy_item(a.s, a.s_rs, a.eqz, a.eqz_rs, a.eqx, a.eqx_rs, a.ptr, a.ent, R, y);
+        constexpr uint32_t ROWS = tile_lanes(BS);
+        const uint32_t tiles = (a.N + ROWS - 1) / ROWS;
+        for (uint32_t tile = (uint32_t)xcd_tile(lb, T.nb); tile < tiles; tile += T.nb)
+            hy_tile<BS>(tacc, a.s, a.s_rs, a.eqz, a.eqz_rs, a.eqx, a.eqx_rs, a.ptr, a.ent, a.N, R, tile * ROWS);
         break;
     }
     case kOpDot: {
@@ -1418,7 +1451,7 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
         }
         {
             // one item per thread: the CSR walk is a chain of dependent gathers, hidden by occupancy
-            dim3 grid(grid_cover(N * LG, 65536));
+            dim3 grid((uint32_t)std::min<uint64_t>((N + gx_rows<L, kThreads>() - 1) / gx_rows<L, kThreads>(), 65536));
             timed(pf, kKGx, st, [&] {
                 hipLaunchKernelGGL(k_gx<L>, grid, dim3(kThreads), 0, st, s.hg0, hg0_rs, V, s.eqz, eqz_rs, ly.gx_ptr,
                                    ly.gx_ent, (uint32_t)N, R);
@@ -1431,7 +1464,7 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
                     hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqx, eqx_rs, (const uint32_t *)chal_l,
                                        0u, 0u, (const uint32_t *)nullptr, n);
                 });
-                dim3 g2(grid_cover(N, 65536));
+                dim3 g2((uint32_t)std::min<uint64_t>((N + tile_lanes(kThreads) - 1) / tile_lanes(kThreads), 65536));
                 timed(pf, kKHy, st, [&] {
                     hipLaunchKernelGGL(k_hy, g2, dim3(kThreads), 0, st, s.s0, s0_rs, s.eqz, eqz_rs, s.eqx, eqx_rs,
                                        ly.hy_ptr, ly.hy_ent, (uint32_t)N, R);
@@ -1584,19 +1617,17 @@ int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, co
 }
 
 // CSR for one key: counting sort of entries by key.
-static void build_csr(uint32_t nrows, const std::vector<uint32_t> &keys, const std::vector<uint32_t> &ent3,
+static void build_csr(uint32_t nrows, const std::vector<uint32_t> &keys, const std::vector<uint32_t> &ent, uint32_t W,
                       std::vector<uint32_t> &ptr, std::vector<uint32_t> &out)
 {
     ptr.assign((size_t)nrows + 1, 0);
     for (uint32_t k : keys) ptr[k + 1]++;
     for (uint32_t i = 0; i < nrows; i++) ptr[i + 1] += ptr[i];
-    out.resize(ent3.size());
+    out.resize(ent.size());
     std::vector<uint32_t> pos(ptr.begin(), ptr.end() - 1);
     for (size_t e = 0; e < keys.size(); e++) {
-        uint32_t at = pos[keys[e]]++;
-        out[3 * (size_t)at] = ent3[3 * e];
-        out[3 * (size_t)at + 1] = ent3[3 * e + 1];
-        out[3 * (size_t)at + 2] = ent3[3 * e + 2];
+        const uint32_t at = pos[keys[e]]++;
+        for (uint32_t w = 0; w < W; w++) out[W * (size_t)at + w] = ent[W * e + w];
     }
 }
 
@@ -1620,23 +1651,23 @@ int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &
         off += (1ull << ly.n) * L;
         const size_t U = mul[l].size() / 4, A = add[l].size() / 3;
         std::vector<uint32_t> kf, ef, kg, eg, kh, eh;
-        kf.reserve(U + A); ef.reserve(3 * (U + A));
-        kg.reserve(U + A); eg.reserve(3 * (U + A));
-        kh.reserve(U); eh.reserve(3 * U);
+        kf.reserve(U + A); ef.reserve(kFwWords * (U + A));
+        kg.reserve(U + A); eg.reserve(kGxWords * (U + A));
+        kh.reserve(U); eh.reserve(kHyWords * U);
         for (size_t g = 0; g < U; g++) {
-            const uint32_t *q = &mul[l][4 * g];
+            const uint32_t *q = &mul[l][4 * g];  // in0, in1, out, coef
             kf.push_back(q[2]); ef.insert(ef.end(), {q[0], q[1], q[3]});
-            kg.push_back(q[0]); eg.insert(eg.end(), {q[1], q[2], q[3]});
-            kh.push_back(q[1]); eh.insert(eh.end(), {q[0], q[2], q[3]});
+            kg.push_back(q[0]); eg.insert(eg.end(), {q[1], q[2], q[3], q[0]});
+            kh.push_back(q[1]); eh.insert(eh.end(), {q[0], q[2], q[3], q[1]});
         }
         for (size_t g = 0; g < A; g++) {
-            const uint32_t *q = &add[l][3 * g];
+            const uint32_t *q = &add[l][3 * g];  // in0, out, coef
             kf.push_back(q[1]); ef.insert(ef.end(), {q[0], kAddMark, q[2]});
-            kg.push_back(q[0]); eg.insert(eg.end(), {kAddMark, q[1], q[2]});
+            kg.push_back(q[0]); eg.insert(eg.end(), {kAddMark, q[1], q[2], q[0]});
         }
-        build_csr(1u << ly.m, kf, ef, host[6 * l + 0], host[6 * l + 1]);
-        build_csr(1u << ly.n, kg, eg, host[6 * l + 2], host[6 * l + 3]);
-        build_csr(1u << ly.n, kh, eh, host[6 * l + 4], host[6 * l + 5]);
+        build_csr(1u << ly.m, kf, ef, kFwWords, host[6 * l + 0], host[6 * l + 1]);
+        build_csr(1u << ly.n, kg, eg, kGxWords, host[6 * l + 2], host[6 * l + 3]);
+        build_csr(1u << ly.n, kh, eh, kHyWords, host[6 * l + 4], host[6 * l + 5]);
     }
     dc.layers[dc.D - 1].out_off = off;
     for (uint32_t l = 0; l + 1 < dc.D; l++) dc.layers[l].out_off = dc.layers[l + 1].v_off;
@@ -1750,7 +1781,12 @@ uint32_t ls_gather_blocks(uint64_t work, uint32_t bs)
     return (uint32_t)((b + 7) & ~7ull);
 }
 
-bool ls_is_gather(uint32_t op) { return op == kOpFwd || op == kOpGx || op == kOpHy; }
+// Blocks of a tiled gather task (g_x, h_y): one per tile up to 65536, a multiple of 8.
+uint32_t ls_tile_blocks(uint64_t tiles)
+{
+    const uint64_t b = std::max<uint64_t>(1, std::min<uint64_t>(tiles, 65536));
+    return (uint32_t)((b + 7) & ~7ull);
+}
 
 uint32_t ls_work_blocks(const LsTask &t, uint32_t R)
 {
@@ -1829,7 +1865,7 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
             LsEntry e;
             LsTask t = ls_task(kOpGx, job, l);
             t.gx = LsGx{J.hg0, J.hg0_rs, V, J.eqz, J.eqz_rs, ly.gx_ptr, ly.gx_ent, (uint32_t)N};
-            t.nb = ls_gather_blocks(N * LG, BS);
+            t.nb = ls_tile_blocks((N * L + tile_lanes(BS) - 1) / tile_lanes(BS));
             e.w[0] = t;
             e.nw = 1;
             if (l == D - 1) {
@@ -1854,7 +1890,7 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
                 push_w(t);
                 LsTask h = ls_task(kOpHy, job, l);
                 h.hy = LsHy{J.s0, J.s0_rs, J.eqz, J.eqz_rs, J.eqx, J.eqx_rs, ly.hy_ptr, ly.hy_ent, (uint32_t)N};
-                h.nb = ls_gather_blocks(N, BS);
+                h.nb = ls_tile_blocks((N + tile_lanes(BS) - 1) / tile_lanes(BS));
                 push_w(h);
             }
             uint32_t *h0 = p2 ? J.s0 : J.hg0, *hA = p2 ? J.sA : J.hgA, *hB = p2 ? J.sB : J.hgB;
@@ -1923,13 +1959,32 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
 
 template <int L>
 void ls_launch_work(uint32_t BS, uint32_t blocks, hipStream_t st, const BatchJob *jobs, const LsTask *tasks,
-                    const uint32_t *start, uint32_t n, uint32_t R)
+                    const uint32_t *start, uint32_t n, uint32_t R, uint32_t base)
 {
+    const dim3 g(blocks);
     switch (BS) {
-    case 64: hipLaunchKernelGGL((k_ls_work<L, 64>), dim3(blocks), dim3(64), 0, st, jobs, tasks, start, n, R); break;
-    case 128: hipLaunchKernelGGL((k_ls_work<L, 128>), dim3(blocks), dim3(128), 0, st, jobs, tasks, start, n, R); break;
-    case 512: hipLaunchKernelGGL((k_ls_work<L, 512>), dim3(blocks), dim3(512), 0, st, jobs, tasks, start, n, R); break;
-    default: hipLaunchKernelGGL((k_ls_work<L, 256>), dim3(blocks), dim3(256), 0, st, jobs, tasks, start, n, R); break;
+    case 64: hipLaunchKernelGGL((k_ls_work<L, 64>), g, dim3(64), 0, st, jobs, tasks, start, n, R, base); break;
+    case 128: hipLaunchKernelGGL((k_ls_work<L, 128>), g, dim3(128), 0, st, jobs, tasks, start, n, R, base); break;
+    case 512: hipLaunchKernelGGL((k_ls_work<L, 512>), g, dim3(512), 0, st, jobs, tasks, start, n, R, base); break;
+    default: hipLaunchKernelGGL((k_ls_work<L, 256>), g, dim3(256), 0, st, jobs, tasks, start, n, R, base); break;
+    }
+}
+
+// Work ops in the order they take within a step: the gathers first (their block counts
+// are multiples of 8), then the rest.  With profiling on, each op's range of a step is
+// launched (and timed) on its own.
+constexpr uint32_t kLsWorkOrder[] = {kOpFwd, kOpGx, kOpHy, kOpSeed, kOpEqz, kOpEqx, kOpDot, kOpRound};
+
+int ls_op_class(uint32_t op)
+{
+    switch (op) {
+    case kOpFwd: return kKLsFwd;
+    case kOpGx: return kKLsGx;
+    case kOpHy: return kKLsHy;
+    case kOpEqz:
+    case kOpEqx: return kKLsEq;
+    case kOpRound: return kKLsRound;
+    default: return kKLsWork;
     }
 }
 
@@ -2079,12 +2134,12 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                 S.w_task = (uint32_t)tasks.size();
                 S.w_start = (uint32_t)starts.size();
                 uint32_t blocks = 0;
-                for (int pass = 0; pass < 2; pass++)  // gather tasks first (multiples of 8 blocks)
+                for (uint32_t op : kLsWorkOrder)
                     for (size_t i = 0; i < nj; i++) {
                         if (s >= ops[i].size()) continue;
                         const LsEntry &e = ops[i][s];
                         for (uint32_t q = 0; q < e.nw; q++)
-                            if (ls_is_gather(e.w[q].op) == (pass == 0)) {
+                            if (e.w[q].op == op) {
                                 starts.push_back(blocks);
                                 tasks.push_back(e.w[q]);
                                 blocks += ls_work_blocks(e.w[q], R);
@@ -2188,11 +2243,25 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                 });
             }
             if (chains_pending) GKRX_HIP_CHECK(hipStreamWaitEvent(st, bs.ev_aux, 0));
-            if (S.w_n)
-                timed(pf, kKLsWork, st, [&] {
-                    if (L == 8) ls_launch_work<8>(BS, S.w_blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R);
-                    else ls_launch_work<16>(BS, S.w_blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R);
-                });
+            auto work = [&](uint32_t blocks, uint32_t base) {
+                if (L == 8)
+                    ls_launch_work<8>(BS, blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R, base);
+                else
+                    ls_launch_work<16>(BS, blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R, base);
+            };
+            if (S.w_n && !pf) {
+                work(S.w_blocks, 0);
+            } else if (S.w_n) {
+                // profiling: one launch per run of tasks of one op class
+                const LsTask *ht = reinterpret_cast<const LsTask *>(bs.host_meta + bs.o_tasks) + S.w_task;
+                const uint32_t *hs = reinterpret_cast<const uint32_t *>(bs.host_meta + bs.o_starts) + S.w_start;
+                for (uint32_t a = 0; a < S.w_n;) {
+                    uint32_t b = a + 1;
+                    while (b < S.w_n && ls_op_class(ht[b].op) == ls_op_class(ht[a].op)) b++;
+                    timed(pf, ls_op_class(ht[a].op), st, [&] { work(hs[b] - hs[a], hs[a]); });
+                    a = b;
+                }
+            }
             if (S.f_n)
                 timed(pf, kKLsFin, st, [&] {
                     if (L == 8)
