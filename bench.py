@@ -26,11 +26,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per dispatch of the phase-1 fused k_round on the 24 x 2^20 M31x16 workload,
-# measured with two rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, per
-# dispatch) and scaled by what the same counters read for a 256 MiB device-to-device copy
+# HBM bytes per dispatch of the roofline kernel on the 24 x 2^20 M31x16 workload, measured
+# with two rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, per dispatch)
+# and scaled by what the same counters read for a 256 MiB device-to-device copy
 # (--calibration-copy-mb 256): bytes read = 2 x FETCH_SIZE KiB, written = 1 x WRITE_SIZE KiB.
-K_ROUND_TRAFFIC_BYTES = 145104239
+# None until measured for the current kernel.
+ROOFLINE_TRAFFIC_BYTES = None
 
 
 def log(msg):
@@ -46,18 +47,20 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-TAIL_ROWS = 2048  # rounds whose tables have at most this many rows run in k_tail
+TAIL_ROWS = 2048  # tables with at most this many rows end a phase in the small-table kernels
 
 
-def round_kernel_bytes(n, R, S):
-    """Algorithmic HBM bytes of the phase-1 fused round kernel (k_round) launches of one
-    layer with n input variables.  Round k >= 1 reads f_{k-1} and hg_{k-1} (2^(n-k+1)
-    entries each) and writes f_k and hg_k (2^(n-k) entries each): 3 * 2^(n-k+1) * S bytes
-    per rep; only rounds with more than TAIL_ROWS rows run in k_round."""
+def pair_kernel_bytes(n, R, S):
+    """Algorithmic HBM bytes of the phase-1 double-fold round-pair launches (k_pair, MODE 2)
+    of one layer with n input variables.  Rounds run in pairs (k, k+1) while the tables are
+    large: K is the first round whose table has at most TAIL_ROWS rows, rounded up to even.
+    The pass of pair k >= 2 reads f and hg of table k - 2 (2^(n-k+2) entries each) and writes
+    table k (2^(n-k) entries each): 10 * 2^(n-k) * S bytes per rep."""
     k0 = 0
     while k0 < n and (1 << (n - k0)) > TAIL_ROWS:
         k0 += 1
-    return sum(R * 3 * (1 << (n - k + 1)) * S for k in range(1, k0))
+    K = (k0 + 1) & ~1
+    return sum(R * 10 * (1 << (n - k)) * S for k in range(2, K, 2))
 
 
 def cpu_baseline(layers_all, L, fs, seed, n_layers):
@@ -408,9 +411,9 @@ def main():
     value = gates * args.steps / elapsed
 
     S = 4 * L
-    dom = "k_round_ph1"
+    dom = "k_pair_ph1"
     st = stats.get(dom, {"count": 0, "ms": 0.0})
-    bytes_per_proof = sum(round_kernel_bytes(n, R, S) for n in circ.nb_input_vars)
+    bytes_per_proof = sum(pair_kernel_bytes(n, R, S) for n in circ.nb_input_vars)
     roof = None
     if st["count"]:
         per_launch = bytes_per_proof * args.steps / st["count"]
@@ -418,9 +421,9 @@ def main():
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": K_ROUND_TRAFFIC_BYTES if (D, k, L) == (24, 20, 16) else None,
+                "traffic": ROOFLINE_TRAFFIC_BYTES if (D, k, L) == (24, 20, 16) else None,
                 "algorithmic_bytes_per_launch": int(per_launch),
-                "kernel": "k_round<16,phase1,fused>", "avg_launch_us": round(avg_ms * 1e3, 2),
+                "kernel": "k_pair<16,phase1,double-fold>", "avg_launch_us": round(avg_ms * 1e3, 2),
                 "launches": st["count"]}
     ctx.close()
     batch = serve = None

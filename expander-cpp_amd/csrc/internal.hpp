@@ -95,10 +95,11 @@ enum BatchEngine : int { kBatchPerProof = 1, kBatchLockstep = 2 };
 struct BatchScratch {
     int engine = kBatchLockstep;
     uint32_t block = 256;           // workgroup size of the lockstep work kernel: 64, 128, 256 or 512
-    // lockstep: run the serial chains on a second stream beside the next step's wide ops
+    // lockstep: run the serial chains and the round pairs on streams of their own, beside the
+    // other wide ops (false: everything on the caller's stream)
     bool chain_stream = true;
-    hipStream_t aux = nullptr;
-    hipEvent_t ev_main = nullptr, ev_aux = nullptr;
+    hipStream_t aux = nullptr, aux2 = nullptr;
+    hipEvent_t ev_main = nullptr, ev_aux = nullptr, ev_aux2 = nullptr;
     uint32_t *pool = nullptr;
     uint64_t pool_u32 = 0;
     uint8_t *meta = nullptr;        // job table, order or step schedule

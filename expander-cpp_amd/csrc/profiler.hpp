@@ -15,14 +15,14 @@ enum KernelClass : int {
     kKGx,
     kKHy,
     kKDot,
-    kKRoundPh1First,
-    kKRoundPh1,
-    kKRoundPh2First,
-    kKRoundPh2,
+    kKPairPh1First,
+    kKPairPh1,
+    kKPairPh2First,
+    kKPairPh2,
     kKTailPh1,
     kKTailPh2,
     kKBatch,
-    kKRoundFin,
+    kKPairFin,
     kKLsWork,
     kKLsFin,
     kKLsChain,
@@ -30,19 +30,26 @@ enum KernelClass : int {
     kKLsGx,
     kKLsHy,
     kKLsEq,
-    kKLsRound,
+    kKLsPair,
+    kKRoundPh1First,  // single rounds (BN254)
+    kKRoundPh1,
+    kKRoundPh2First,
+    kKRoundPh2,
+    kKRoundFin,
     kKClassCount
 };
 
 inline const char *kernel_class_name(int c)
 {
-    static const char *names[kKClassCount] = {"k_forward",      "k_eq",      "k_gx",           "k_hy",
-                                              "k_dot",          "k_round_ph1_first", "k_round_ph1",
-                                              "k_round_ph2_first", "k_round_ph2", "k_tail_ph1",
-                                              "k_tail_ph2",     "k_prove_batch", "k_round_fin",
-                                              "k_ls_work",      "k_ls_fin",  "k_ls_chain",
-                                              "k_ls_work:forward", "k_ls_work:g_x", "k_ls_work:h_y",
-                                              "k_ls_work:eq",   "k_ls_work:round"};
+    static const char *names[kKClassCount] = {"k_forward",         "k_eq",          "k_gx",
+                                              "k_hy",              "k_dot",         "k_pair_ph1_first",
+                                              "k_pair_ph1",        "k_pair_ph2_first", "k_pair_ph2",
+                                              "k_tail_ph1",        "k_tail_ph2",    "k_prove_batch",
+                                              "k_pair_fin",        "k_ls_work",     "k_ls_fin",
+                                              "k_ls_chain",        "k_ls_work:forward", "k_ls_work:g_x",
+                                              "k_ls_work:h_y",     "k_ls_work:eq",  "k_ls_work:pair",
+                                              "k_round_ph1_first", "k_round_ph1",   "k_round_ph2_first",
+                                              "k_round_ph2",       "k_round_fin"};
     return (c >= 0 && c < kKClassCount) ? names[c] : "";
 }
 

@@ -50,6 +50,10 @@ __device__ __forceinline__ uint4 mul4(uint4 a, uint4 b)
 {
     return make_uint4(m31_mul(a.x, b.x), m31_mul(a.y, b.y), m31_mul(a.z, b.z), m31_mul(a.w, b.w));
 }
+__device__ __forceinline__ uint4 sub4(uint4 a, uint4 b)
+{
+    return make_uint4(m31_sub(a.x, b.x), m31_sub(a.y, b.y), m31_sub(a.z, b.z), m31_sub(a.w, b.w));
+}
 __device__ __forceinline__ uint4 mul4s(uint4 a, uint32_t s)
 {
     return make_uint4(m31_mul(a.x, s), m31_mul(a.y, s), m31_mul(a.z, s), m31_mul(a.w, s));
@@ -366,6 +370,120 @@ __device__ __forceinline__ void round_item(const uint32_t *__restrict__ fs, uint
     acc[2][0] += e2.x; acc[2][1] += e2.y; acc[2][2] += e2.z; acc[2][3] += e2.w;
 }
 
+// Round k of a phase whose source is table k - 2 (the first small-table round after the
+// round pairs below): rows 2i, 2i+1 of table k are folds of rows 8i .. 8i+7 of table k - 2
+// by ra = r_{k-2} and then rb = r_{k-1}; they are stored and their round-k sums added.
+template <int L, bool PH2>
+__device__ __forceinline__ void round_item_dbl(const uint32_t *__restrict__ fs, uint32_t *__restrict__ fd,
+                                               const uint32_t *__restrict__ hs, uint32_t *__restrict__ hd, uint64_t i,
+                                               uint32_t g, uint32_t ra, uint32_t rb, uint64_t (&acc)[3][4])
+{
+    uint4 f[2], h[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const uint32_t *src = fs + (8 * i + 4 * r) * L + 4 * g;
+        f[r] = fold4(fold4(ld4(src), ld4(src + L), ra), fold4(ld4(src + 2 * L), ld4(src + 3 * L), ra), rb);
+        st4(fd + (2 * i + r) * L + 4 * g, f[r]);
+        if (PH2) {
+            const uint4 b = *reinterpret_cast<const uint4 *>(hs + 8 * i + 4 * r);
+            const uint32_t sv = fold1(fold1(b.x, b.y, ra), fold1(b.z, b.w, ra), rb);
+            if (g == 0) hd[2 * i + r] = sv;
+            h[r] = make_uint4(sv, sv, sv, sv);
+        } else {
+            const uint32_t *hsrc = hs + (8 * i + 4 * r) * L + 4 * g;
+            h[r] = fold4(fold4(ld4(hsrc), ld4(hsrc + L), ra), fold4(ld4(hsrc + 2 * L), ld4(hsrc + 3 * L), ra), rb);
+            st4(hd + (2 * i + r) * L + 4 * g, h[r]);
+        }
+    }
+    const uint4 e0 = mul4(f[0], h[0]), e1 = mul4(f[1], h[1]), e2 = mul4(add4(f[0], f[1]), add4(h[0], h[1]));
+    acc[0][0] += e0.x; acc[0][1] += e0.y; acc[0][2] += e0.z; acc[0][3] += e0.w;
+    acc[1][0] += e1.x; acc[1][1] += e1.y; acc[1][2] += e1.z; acc[1][3] += e1.w;
+    acc[2][0] += e2.x; acc[2][1] += e2.y; acc[2][2] += e2.z; acc[2][3] += e2.w;
+}
+
+// ------------------------------------------------------------------ round pairs
+// Rounds k and k+1 of a phase in one pass over the round-k tables T (f, and h: hg in phase
+// 1, the scalar s in phase 2).  Round k+1's pairs are folds of rows (4i, 4i+1) and
+// (4i+2, 4i+3) by the challenge r_k that is not drawn yet, so each of its three sums is a
+// polynomial of degree 2 in r_k.  With x0..x3 = T[4i .. 4i+3] (of f and of h) those are
+//   s0(r) = sum (x0 + r (x1 - x0))_f (..)_h,   s1(r) = the same of x2, x3,
+//   s2(r) = sum ((x0 + x2) + r ((x1 + x3) - (x0 + x2)))_f (..)_h,
+// and with u = x0, x1, x1 - x0 (and v, w likewise for s1, s2) every coefficient, and every
+// sum of round k, is a linear combination of nine products per lane:
+//   P0 = f0 h0, P1 = f1 h1, P2 = (f1 - f0)(h1 - h0), P3..P5 the same of rows 2, 3,
+//   P6 = (f0 + f2)(h0 + h2), P7 = (f1 + f3)(h1 + h3), P8 = (f1 + f3 - f0 - f2)(h1 + h3 - h0 - h2);
+//   s0(r) = P0 + r (P1 - P0 - P2) + r^2 P2 (likewise s1 from P3..P5, s2 from P6..P8), and round
+//   k: p0 = P0 + P3, p1 = P1 + P4, sum (f0 + f1)(h0 + h1) = 2 P0 + 2 P1 - P2 (+ rows 2, 3).
+// Once r_k is drawn the transcript kernel evaluates them: the same field values as a pass
+// over the folded round-(k+1) tables, which are never stored.  T is the layer input (MODE 0:
+// rounds 0 and 1) or is made here by folding rows 16i .. 16i+15 of table k - 2 by
+// ra = r_{k-2} and then rb = r_{k-1} (MODE 2), and stored for the next pair.  Sums are
+// canonical (u32 accumulators keep the register count, and with it the occupancy, up).
+constexpr uint32_t kPairVals = 9;
+
+template <int L, bool PH2, int MODE>
+__device__ __forceinline__ void pair_item(const uint32_t *__restrict__ fs, uint32_t *__restrict__ fd,
+                                          const uint32_t *__restrict__ hs, uint32_t *__restrict__ hd, uint64_t i,
+                                          uint32_t g, uint32_t ra, uint32_t rb, uint32_t (&acc)[kPairVals][4])
+{
+    uint4 f[4];
+    uint4 h[4];       // phase 1
+    uint32_t sv[4];   // phase 2: the scalar h of each row
+    if (MODE == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) f[r] = ld4(fs + (4 * i + r) * L + 4 * g);
+        if (PH2) {
+            const uint4 b = *reinterpret_cast<const uint4 *>(hs + 4 * i);
+            sv[0] = b.x; sv[1] = b.y; sv[2] = b.z; sv[3] = b.w;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; r++) h[r] = ld4(hs + (4 * i + r) * L + 4 * g);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t *src = fs + (16 * i + 4 * r) * L + 4 * g;
+            f[r] = fold4(fold4(ld4(src), ld4(src + L), ra), fold4(ld4(src + 2 * L), ld4(src + 3 * L), ra), rb);
+            st4(fd + (4 * i + r) * L + 4 * g, f[r]);
+            if (PH2) {
+                const uint4 b = *reinterpret_cast<const uint4 *>(hs + 16 * i + 4 * r);
+                sv[r] = fold1(fold1(b.x, b.y, ra), fold1(b.z, b.w, ra), rb);
+            } else {
+                const uint32_t *hsrc = hs + (16 * i + 4 * r) * L + 4 * g;
+                h[r] = fold4(fold4(ld4(hsrc), ld4(hsrc + L), ra), fold4(ld4(hsrc + 2 * L), ld4(hsrc + 3 * L), ra),
+                             rb);
+                st4(hd + (4 * i + r) * L + 4 * g, h[r]);
+            }
+        }
+        if (PH2 && g == 0) *reinterpret_cast<uint4 *>(hd + 4 * i) = make_uint4(sv[0], sv[1], sv[2], sv[3]);
+    }
+    // the nine factors of f (lanes) and of h (lanes, or one scalar in phase 2)
+    const uint4 f02 = add4(f[0], f[2]), f13 = add4(f[1], f[3]);
+    const uint4 ff[9] = {f[0], f[1], sub4(f[1], f[0]), f[2], f[3], sub4(f[3], f[2]), f02, f13, sub4(f13, f02)};
+    if (PH2) {
+        const uint32_t s02 = m31_add(sv[0], sv[2]), s13 = m31_add(sv[1], sv[3]);
+        const uint32_t hh[9] = {sv[0], sv[1], m31_sub(sv[1], sv[0]), sv[2], sv[3], m31_sub(sv[3], sv[2]),
+                                s02, s13, m31_sub(s13, s02)};
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            acc[q][0] = m31_add(acc[q][0], m31_mul(ff[q].x, hh[q]));
+            acc[q][1] = m31_add(acc[q][1], m31_mul(ff[q].y, hh[q]));
+            acc[q][2] = m31_add(acc[q][2], m31_mul(ff[q].z, hh[q]));
+            acc[q][3] = m31_add(acc[q][3], m31_mul(ff[q].w, hh[q]));
+        }
+    } else {
+        const uint4 h02 = add4(h[0], h[2]), h13 = add4(h[1], h[3]);
+        const uint4 hh[9] = {h[0], h[1], sub4(h[1], h[0]), h[2], h[3], sub4(h[3], h[2]), h02, h13, sub4(h13, h02)};
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            acc[q][0] = m31_add(acc[q][0], m31_mul(ff[q].x, hh[q].x));
+            acc[q][1] = m31_add(acc[q][1], m31_mul(ff[q].y, hh[q].y));
+            acc[q][2] = m31_add(acc[q][2], m31_mul(ff[q].z, hh[q].z));
+            acc[q][3] = m31_add(acc[q][3], m31_mul(ff[q].w, hh[q].w));
+        }
+    }
+}
+
 // ------------------------------------------------------------------ transcript
 // The transcript (include/fiat_shamir/transcript.hpp:87-175) continues on the device over
 // the proof bytes after the commitment.  tstate: [0] length in bytes, [1] start of the
@@ -537,89 +655,151 @@ __device__ void transcript_step(const uint32_t *sums, const TsArgs &a, uint32_t 
 }
 
 struct RoundParams {
-    const uint32_t *fsrc;  // f_{k-1} (or f_0 when k == 0)
-    uint64_t fsrc_rs;      // rep stride in u32 (0 when the source is the shared layer input)
-    uint32_t *fdst;        // f_k (unused when k == 0)
+    const uint32_t *fsrc;  // table k - 2 of f, or the layer input (rep stride 0) for the first two pairs
+    uint64_t fsrc_rs;      // rep stride in u32
+    uint32_t *fdst;        // table k of f (unused for the pair of rounds 0 and 1)
     uint64_t fdst_rs;
     const uint32_t *hsrc;  // phase 1: lane table hg; phase 2: scalar table s
     uint64_t hsrc_rs;
     uint32_t *hdst;
     uint64_t hdst_rs;
-    const uint32_t *rprev; // challenge of round k-1 for rep 0; rep stride kChalStride
-    uint32_t *partials;    // value-major: [rep][3L][gridDim.x]
-    uint32_t items;        // pairs in round k = N_k / 2
+    const uint32_t *rprev; // r_{k-2} of rep 0 (r_{k-1} follows it); rep stride kChalStride
+    uint32_t *partials;    // value-major: [rep][kPairVals L][blocks per rep]
+    uint32_t items;        // items of the pair: N_k / 4 (four rows of table k each)
 };
 
-// Round k of one sumcheck phase for every repetition (blockIdx.y = rep), on many blocks:
-// the fold of round k-1 and the three sums of round k.  Each block leaves its partial
-// sums; k_fin adds them and runs the transcript.  The launch boundary between the two
-// orders the partials for k_fin, so no device-scope fence is needed here (on a multi-XCD
-// part a release fence writes back the whole L2, once per block).
-template <int L, bool PH2, bool FUSED>
-__global__ void __launch_bounds__(kThreads) k_round(RoundParams p)
+// Adds the NV x 4 canonical lane sums of a block of BS threads (lanes sharing t % LG) into
+// out[v * ostride], v = q L + lane < NV L.  Every thread must call it.
+template <int L, int BS, int NV>
+__device__ __forceinline__ void block_reduce_vals(uint32_t (&acc)[NV][4], uint32_t *out, uint32_t ostride)
 {
-    constexpr uint32_t LG = L / 4, Q = 3 * L;
-    __shared__ uint32_t red[4][3 * L];
-    const uint32_t rep = blockIdx.y;
-    const uint32_t g = threadIdx.x % LG;
-    const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
-    uint32_t *fd = p.fdst + rep * p.fdst_rs;
-    const uint32_t *hs = p.hsrc + rep * p.hsrc_rs;
-    uint32_t *hd = p.hdst + rep * p.hdst_rs;
-    const uint32_t r = FUSED ? p.rprev[(uint64_t)rep * kChalStride] : 0u;
-    uint64_t acc[3][4] = {};
-    const uint64_t total = (uint64_t)p.items * LG;
-    for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
-        round_item<L, PH2, FUSED>(fs, fd, hs, hd, w / LG, g, r, acc);
-    group_reduce<L>(acc, red, p.partials + (uint64_t)rep * Q * gridDim.x + blockIdx.x, gridDim.x);
+    constexpr uint32_t LG = L / 4, W = BS / 64;
+    __shared__ uint32_t red[W][NV * L];
+#pragma unroll
+    for (uint32_t off = LG; off < 64; off <<= 1) {
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[q][c] = m31_add(acc[q][c], __shfl_xor(acc[q][c], off, 64));
+    }
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < LG) {
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) red[wave][q * L + 4 * lane + c] = acc[q][c];
+    }
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < NV * L; v += BS) {
+        uint32_t x = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < W; w++) x = m31_add(x, red[w][v]);
+        out[v * ostride] = x;
+    }
 }
 
-// Adds the value-major block partials [R][3L][nb] of k_round (or k_dot) into sums[R][3L]
-// (LDS): wave w takes values w, w + 16, ... (at most kFinVals of them); lanes stride over
-// the nb <= kMaxBlocksPerRep blocks.  Every load is issued before the first add, so the
-// reduction costs one memory latency rather than one per value.  blockDim is kFinThreads.
+// Rounds k and k+1 of one phase for every repetition (blockIdx.y = rep), on many blocks
+// (pair_item).  Each block leaves its partial sums; k_pair_fin adds them and runs the two
+// transcript steps.  The launch boundary between the two orders the partials, so no
+// device-scope fence is needed (on a multi-XCD part a release fence writes back the whole
+// L2, once per block).
+template <int L, bool PH2, int MODE>
+__global__ void __launch_bounds__(kThreads) k_pair(RoundParams p)
+{
+    constexpr uint32_t LG = L / 4;
+    const uint32_t rep = blockIdx.y, g = threadIdx.x % LG;
+    const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
+    const uint32_t *hs = p.hsrc + rep * p.hsrc_rs;
+    uint32_t *fd = MODE ? p.fdst + rep * p.fdst_rs : nullptr;
+    uint32_t *hd = MODE ? p.hdst + rep * p.hdst_rs : nullptr;
+    const uint32_t ra = MODE ? p.rprev[(uint64_t)rep * kChalStride] : 0u;
+    const uint32_t rb = MODE ? p.rprev[(uint64_t)rep * kChalStride + 1] : 0u;
+    uint32_t acc[kPairVals][4] = {};
+    const uint64_t total = (uint64_t)p.items * LG;
+    for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
+        pair_item<L, PH2, MODE>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
+    block_reduce_vals<L, kThreads, kPairVals>(acc, p.partials + (uint64_t)rep * kPairVals * L * gridDim.x + blockIdx.x,
+                                              gridDim.x);
+}
+
+// Adds the value-major block partials [R][V][nb] of k_pair (V = kPairVals L) or k_dot
+// (V = 3L) into sums[R][V] (LDS): wave w takes values w, w + 16, ..., kFinVals of them per
+// pass; lanes stride over the nb <= kMaxBlocksPerRep blocks.  Every load of a pass is issued
+// before the first add, so a pass costs one memory latency rather than one per value.
+// blockDim is kFinThreads.
 constexpr uint32_t kFinThreads = 1024;
 template <int L>
 __device__ __forceinline__ void reduce_partials(const uint32_t *__restrict__ partials, uint32_t nb, uint32_t nvals,
                                                 uint32_t *sums)
 {
     constexpr uint32_t W = kFinThreads / 64;
-    constexpr uint32_t kFinVals = (3 * 3 * L + W - 1) / W;  // R <= 3
+    constexpr uint32_t kFinVals = (3 * 3 * L + W - 1) / W;
     constexpr uint32_t kBlk = kMaxBlocksPerRep / 64;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint32_t x[kFinVals][kBlk];
+    for (uint32_t base = 0; base < nvals; base += kFinVals * W) {
+        uint32_t x[kFinVals][kBlk];
 #pragma unroll
-    for (uint32_t i = 0; i < kFinVals; i++) {
-        const uint32_t v = wave + i * W;
+        for (uint32_t i = 0; i < kFinVals; i++) {
+            const uint32_t v = base + wave + i * W;
 #pragma unroll
-        for (uint32_t c = 0; c < kBlk; c++) {
-            const uint32_t b = lane + 64 * c;
-            x[i][c] = (v < nvals && b < nb) ? partials[(uint64_t)v * nb + b] : 0u;
+            for (uint32_t c = 0; c < kBlk; c++) {
+                const uint32_t b = lane + 64 * c;
+                x[i][c] = (v < nvals && b < nb) ? partials[(uint64_t)v * nb + b] : 0u;
+            }
         }
-    }
 #pragma unroll
-    for (uint32_t i = 0; i < kFinVals; i++) {
-        uint64_t a = 0;
+        for (uint32_t i = 0; i < kFinVals; i++) {
+            uint64_t a = 0;
 #pragma unroll
-        for (uint32_t c = 0; c < kBlk; c++) a += x[i][c];
-        uint32_t y = m31_reduce64(a);
+            for (uint32_t c = 0; c < kBlk; c++) a += x[i][c];
+            uint32_t y = m31_reduce64(a);
 #pragma unroll
-        for (uint32_t off = 1; off < 64; off <<= 1) y = m31_add(y, __shfl_xor(y, off, 64));
-        const uint32_t v = wave + i * W;
-        if (lane == 0 && v < nvals) sums[v] = y;
+            for (uint32_t off = 1; off < 64; off <<= 1) y = m31_add(y, __shfl_xor(y, off, 64));
+            const uint32_t v = base + wave + i * W;
+            if (lane == 0 && v < nvals) sums[v] = y;
+        }
     }
     __syncthreads();
 }
 
-// The end of a grid round: adds the partials of k_round and runs the transcript step
-// (one workgroup of kFinThreads).
+// The end of a pair of grid rounds (one workgroup of kFinThreads): adds the partials (the
+// nine products P0..P8 of pair_item per lane), forms round k's sums and runs its transcript
+// step, evaluates round k+1's sums at the r_k just drawn and runs its transcript step.
+// ts.k is round k.
 template <int L>
-__global__ void __launch_bounds__(kFinThreads) k_fin(const uint32_t *__restrict__ partials, uint32_t nb, TsArgs ts)
+__device__ void pair_fin_body(const uint32_t *__restrict__ partials, uint32_t nb, TsArgs ts)
 {
-    constexpr uint32_t Q = 3 * L;
-    __shared__ uint32_t sums[3 * Q], rr[3];
-    reduce_partials<L>(partials, nb, ts.R * Q, sums);
+    constexpr uint32_t Q = 3 * L, V = kPairVals * L;
+    __shared__ uint32_t vals[3 * V], sums[3 * Q], rr[3];
+    const uint32_t R = ts.R;
+    reduce_partials<L>(partials, nb, R * V, vals);
+    for (uint32_t t = threadIdx.x; t < R * L; t += blockDim.x) {
+        const uint32_t j = t / L, c = t % L;
+        const uint32_t *P = vals + j * V + c;  // P[q L] = product q of lane c
+        const uint32_t u = m31_add(P[0], P[3 * L]), v = m31_add(P[L], P[4 * L]), w = m31_add(P[2 * L], P[5 * L]);
+        sums[j * Q + c] = u;                                               // sum f0 h0
+        sums[j * Q + L + c] = v;                                           // sum f1 h1
+        sums[j * Q + 2 * L + c] = m31_sub(m31_add(m31_add(u, u), m31_add(v, v)), w);  // sum (f0 + f1)(h0 + h1)
+    }
+    __syncthreads();
     transcript_step<L>(sums, ts, rr);
+    for (uint32_t t = threadIdx.x; t < R * Q; t += blockDim.x) {
+        const uint32_t j = t / Q, q = t % Q, s = q / L, c = q % L;
+        const uint32_t *P = vals + j * V + 3 * s * L + c;  // the three products of s_s
+        const uint32_t r = rr[j];
+        const uint32_t c1 = m31_sub(P[L], m31_add(P[0], P[2 * L]));
+        sums[t] = m31_add(P[0], m31_mul(r, m31_add(c1, m31_mul(r, P[2 * L]))));
+    }
+    __syncthreads();
+    ts.k += 1;
+    transcript_step<L>(sums, ts, rr);
+}
+
+template <int L>
+__global__ void __launch_bounds__(kFinThreads) k_pair_fin(const uint32_t *__restrict__ partials, uint32_t nb,
+                                                          TsArgs ts)
+{
+    pair_fin_body<L>(partials, nb, ts);
 }
 
 // Rounds k0 .. n-1 of one phase in a single workgroup of 768 threads: the tables are
@@ -635,6 +815,7 @@ struct TailParams {
     uint32_t *fA, *fB, *hA, *hB;
     uint64_t fA_rs, fB_rs, hA_rs, hB_rs;
     uint32_t k0, n;
+    uint32_t dbl;             // f0 / h0 are table k0 - 2 (after round pairs): round k0 folds twice
     TsArgs ts;
 };
 
@@ -728,9 +909,14 @@ __device__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
         uint32_t *hd = ((k & 1) ? p.hA : p.hB) + rep * ((k & 1) ? p.hA_rs : p.hB_rs);
         const uint32_t r = rr;
         uint64_t acc[3][4] = {};
-        for (uint64_t w = t; w < items * LG; w += NT) {
-            if (k == 0) round_item<L, PH2, false>(fs, nullptr, hs, nullptr, w / LG, g, 0u, acc);
-            else round_item<L, PH2, true>(fs, fd, hs, hd, w / LG, g, r, acc);
+        if (k == p.k0 && p.dbl) {
+            const uint32_t ra = chal[k - 2];
+            for (uint64_t w = t; w < items * LG; w += NT) round_item_dbl<L, PH2>(fs, fd, hs, hd, w / LG, g, ra, r, acc);
+        } else {
+            for (uint64_t w = t; w < items * LG; w += NT) {
+                if (k == 0) round_item<L, PH2, false>(fs, nullptr, hs, nullptr, w / LG, g, 0u, acc);
+                else round_item<L, PH2, true>(fs, fd, hs, hd, w / LG, g, r, acc);
+            }
         }
         // block reduce: lanes sharing t % LG within the wave, then the W waves
         uint32_t v[3][4];
@@ -847,7 +1033,7 @@ struct BatchJob {
     uint32_t *tail;          // proof bytes after the commitment
     uint32_t *claims;        // R L
     const uint32_t *seed;    // rz1 in slot layout [R * kChalStride], tstate [10], grind bytes [8]
-    uint32_t *partials;      // R * 3L * kMaxBlocksPerRep block partials (lockstep engine)
+    uint32_t *partials;      // R * kPairVals L * kMaxBlocksPerRep block partials (lockstep engine)
 };
 
 template <int L>
@@ -1000,10 +1186,10 @@ enum LsOp : uint32_t {
     kOpEqz,       // work, R x nb blocks: eqZ of layer l
     kOpGx,        // work: g_x of layer l
     kOpDot,       // work, R x nb blocks: claimed_v partials
-    kOpRound,     // work, R x nb blocks: grid round k of a phase
+    kOpPair,      // work, R x nb blocks: grid rounds k, k+1 of a phase
     kOpEqx,       // work, R x nb blocks: eqX of layer l
     kOpHy,        // work: h_y of layer l
-    kOpRoundFin,  // fin, 1 block: partials and transcript step of grid round k
+    kOpPairFin,   // fin, 1 block: partials and transcript steps of grid rounds k, k+1
     kOpDotFin,    // fin, 1 block: claimed_v
     kOpTailRep,   // chain, R blocks: rounds k .. kend - 1 of a phase, one block per rep
     kOpLast,      // chain, 1 block: the last round of a phase
@@ -1051,9 +1237,9 @@ struct LsDot {
 struct LsTask {
     uint32_t op, nb;     // nb: blocks per repetition (R x nb ops) or in total
     uint32_t job, l, phase;
-    uint32_t k, kend;    // round (kOpTailRep: rounds k .. kend - 1)
+    uint32_t k, kend;    // round (kOpPair: rounds k, k+1; kOpTailRep: rounds k .. kend - 1)
     uint32_t len0, hs0;  // kOpTailRep: transcript length and hash start at round k
-    uint32_t pad;
+    uint32_t dbl;        // kOpTailRep: the source is table k - 2 (round pairs ran before)
     union {
         LsFwd fwd;
         LsGx gx;
@@ -1127,22 +1313,23 @@ __device__ __forceinline__ void block_reduce_to(uint64_t (&acc)[3][4], uint32_t 
     }
 }
 
-// Workgroup lb of a grid round task (rep = lb / nb): k_round on a block range.
-template <int L, int BS, bool PH2, bool FUSED>
-__device__ void ls_round(const RoundParams &p, uint32_t nb, uint32_t lb)
+// Workgroup lb of a round-pair task (rep = lb / nb): k_pair on a block range.
+template <int L, int BS, bool PH2, int MODE>
+__device__ void ls_pair(const RoundParams &p, uint32_t nb, uint32_t lb)
 {
-    constexpr uint32_t LG = L / 4, Q = 3 * L;
+    constexpr uint32_t LG = L / 4;
     const uint32_t rep = lb / nb, bx = lb % nb, g = threadIdx.x % LG;
     const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
-    uint32_t *fd = FUSED ? p.fdst + rep * p.fdst_rs : nullptr;
     const uint32_t *hs = p.hsrc + rep * p.hsrc_rs;
-    uint32_t *hd = FUSED ? p.hdst + rep * p.hdst_rs : nullptr;
-    const uint32_t r = FUSED ? p.rprev[(uint64_t)rep * kChalStride] : 0u;
-    uint64_t acc[3][4] = {};
+    uint32_t *fd = MODE ? p.fdst + rep * p.fdst_rs : nullptr;
+    uint32_t *hd = MODE ? p.hdst + rep * p.hdst_rs : nullptr;
+    const uint32_t ra = MODE ? p.rprev[(uint64_t)rep * kChalStride] : 0u;
+    const uint32_t rb = MODE ? p.rprev[(uint64_t)rep * kChalStride + 1] : 0u;
+    uint32_t acc[kPairVals][4] = {};
     const uint64_t total = (uint64_t)p.items * LG;
     for (uint64_t w = (uint64_t)bx * BS + threadIdx.x; w < total; w += (uint64_t)nb * BS)
-        round_item<L, PH2, FUSED>(fs, fd, hs, hd, w / LG, g, r, acc);
-    block_reduce_to<L, BS>(acc, p.partials + (uint64_t)rep * Q * nb + bx, nb);
+        pair_item<L, PH2, MODE>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
+    block_reduce_vals<L, BS, kPairVals>(acc, p.partials + (uint64_t)rep * kPairVals * L * nb + bx, nb);
 }
 
 template <int L, int BS>
@@ -1215,18 +1402,25 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
         block_reduce_to<L, BS>(acc, a.partials + (uint64_t)rep * Q * T.nb + bx, T.nb);
         break;
     }
-    case kOpRound: {
-        if (T.phase == 1) {
-            if (T.k == 0) ls_round<L, BS, false, false>(T.rp, T.nb, lb);
-            else ls_round<L, BS, false, true>(T.rp, T.nb, lb);
-        } else {
-            if (T.k == 0) ls_round<L, BS, true, false>(T.rp, T.nb, lb);
-            else ls_round<L, BS, true, true>(T.rp, T.nb, lb);
-        }
-        break;
-    }
     default:
         break;
+    }
+}
+
+// The round pairs of a step (their own kernel and stream: a pair item holds many more
+// registers than the other ops, which would lower their occupancy).
+template <int L, int BS>
+__global__ void __launch_bounds__(BS) k_ls_pair(const LsTask *__restrict__ tasks, const uint32_t *__restrict__ start,
+                                                uint32_t ntasks)
+{
+    const uint2 f = ls_find(start, ntasks, blockIdx.x);
+    const LsTask &T = tasks[f.x];
+    if (T.phase == 1) {
+        if (T.k == 0) ls_pair<L, BS, false, 0>(T.rp, T.nb, f.y);
+        else ls_pair<L, BS, false, 2>(T.rp, T.nb, f.y);
+    } else {
+        if (T.k == 0) ls_pair<L, BS, true, 0>(T.rp, T.nb, f.y);
+        else ls_pair<L, BS, true, 2>(T.rp, T.nb, f.y);
     }
 }
 
@@ -1250,7 +1444,7 @@ __device__ TsArgs ls_ts(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t 
 // single-proof path: table k - 1 feeds round k, table 0 is the layer input and hg0 / s0).
 template <int L>
 __device__ TailParams ls_tail_params(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t R, int hash_kind,
-                                     uint32_t k0)
+                                     uint32_t k0, uint32_t dbl)
 {
     const LayerDev &ly = J.layers[l];
     const bool p2 = phase == 2;
@@ -1261,7 +1455,14 @@ __device__ TailParams ls_tail_params(const BatchJob &J, uint32_t l, uint32_t pha
     tp.fA_rs = J.fA_rs; tp.fB_rs = J.fB_rs; tp.hA_rs = hA_rs; tp.hB_rs = hB_rs;
     tp.n = ly.n;
     tp.k0 = k0;
-    if (k0 <= 1) {
+    tp.dbl = dbl;
+    if (dbl) {
+        // table k0 - 2: the layer input, or where the last round pair stored it (the A tables)
+        tp.f0 = k0 == 2 ? J.values + ly.v_off : J.fA;
+        tp.f0_rs = k0 == 2 ? 0 : J.fA_rs;
+        tp.h0 = k0 == 2 ? (p2 ? J.s0 : J.hg0) : hA;
+        tp.h0_rs = k0 == 2 ? (p2 ? J.s0_rs : J.hg0_rs) : hA_rs;
+    } else if (k0 <= 1) {
         tp.f0 = J.values + ly.v_off;
         tp.f0_rs = 0;
         tp.h0 = p2 ? J.s0 : J.hg0;
@@ -1276,8 +1477,8 @@ __device__ TailParams ls_tail_params(const BatchJob &J, uint32_t l, uint32_t pha
     return tp;
 }
 
-// The short ends of a step (one workgroup each): the partials and transcript step after a
-// grid round, and the claims.
+// The short ends of a step (one workgroup each): the partials and the two transcript
+// steps after a pair of grid rounds, and the claims.
 template <int L>
 __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restrict__ jobs,
                                                         const LsTask *__restrict__ tasks,
@@ -1285,17 +1486,17 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restri
                                                         uint32_t R, int hash_kind)
 {
     constexpr uint32_t Q = 3 * L;
-    __shared__ uint32_t sums[3 * Q], rr[3];
+    __shared__ uint32_t sums[3 * Q];
     const uint2 f = ls_find(start, ntasks, blockIdx.x);
     const LsTask &T = tasks[f.x];
     const BatchJob &J = jobs[T.job];
-    reduce_partials<L>(J.partials, T.nb, R * Q, sums);
-    if (T.op == kOpRoundFin) {
+    if (T.op == kOpPairFin) {
         TsArgs ts = ls_ts<L>(J, T.l, T.phase, R, hash_kind);
         ts.k = T.k;
-        transcript_step<L>(sums, ts, rr);
-    } else if (threadIdx.x < R * L) {
-        J.claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
+        pair_fin_body<L>(J.partials, T.nb, ts);
+    } else {
+        reduce_partials<L>(J.partials, T.nb, R * Q, sums);
+        if (threadIdx.x < R * L) J.claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
     }
 }
 
@@ -1353,14 +1554,14 @@ __global__ void __launch_bounds__(kChainThreads) k_ls_chain(const BatchJob *__re
     const BatchJob &J = jobs[T.job];
     if (T.op == kOpTailRep) {
         TailRepParams q{};
-        q.t = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, T.k);
+        q.t = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, T.k, T.dbl);
         q.kend = T.kend;
         q.len0 = T.len0;
         q.hs0 = T.hs0;
         if (T.phase == 1) tail_rep_body<L, false, kChainThreads>(q, f.y);
         else tail_rep_body<L, true, kChainThreads>(q, f.y);
     } else {
-        const TailParams tp = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, J.layers[T.l].n - 1);
+        const TailParams tp = ls_tail_params<L>(J, T.l, T.phase, R, hash_kind, J.layers[T.l].n - 1, 0u);
         if (T.phase == 1) last_round_body<L, false>(tp);
         else last_round_body<L, true>(tp);
     }
@@ -1477,50 +1678,56 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
             TsArgs ts{};
             ts.R = R; ts.phase = phase; ts.chal = chal_l; ts.vx = s.vxy; ts.vy = s.vxy + R * L;
             ts.tstate = s.tstate; ts.tail = s.tail; ts.ab_out = ab_slot(l); ts.hash_kind = hash_kind;
-            // rounds with more than kTailRows rows run on the whole GPU, the rest in k_tail
+            // rounds [0, K) run on the whole GPU in pairs (K: the first round with at most
+            // kTailRows rows, rounded up to even), the rest in k_tail_rep / k_tail.  Pair j stores table 2j (j >= 1); the tables
+            // alternate so that the last one, table K - 2, is in the A buffers, where the first
+            // small-table round (which folds it twice) reads it.
             uint32_t k0 = 0;
             while (k0 < n && (N >> k0) > kTailRows) k0++;
-            for (uint32_t k = 0; k < k0; k++) {
+            const uint32_t K = (k0 + 1) & ~1u, J = K / 2;
+            auto pair_in_A = [&](uint32_t j) { return ((J - 1 - j) & 1) == 0; };
+            for (uint32_t j = 0; j < J; j++) {
+                const uint32_t k = 2 * j;
                 RoundParams rp{};
-                rp.items = (uint32_t)(N >> (k + 1));
+                rp.items = (uint32_t)(N >> (k + 2));
                 rp.partials = s.partials;
                 TsArgs tk = ts;
                 tk.k = k;
-                tk.last = 0;  // the last round is always in k_tail
-                if (k == 0) {
+                tk.last = 0;
+                if (j <= 1) {
                     rp.fsrc = V; rp.fsrc_rs = 0;
                     rp.hsrc = h0; rp.hsrc_rs = h0_rs;
                 } else {
-                    rp.fsrc = (k == 1) ? V : ((k - 1) & 1 ? s.fA : s.fB);
-                    rp.fsrc_rs = (k == 1) ? 0 : ((k - 1) & 1 ? fA_rs : fB_rs);
-                    rp.fdst = (k & 1) ? s.fA : s.fB;
-                    rp.fdst_rs = (k & 1) ? fA_rs : fB_rs;
-                    rp.hsrc = (k == 1) ? h0 : ((k - 1) & 1 ? hA : hB);
-                    rp.hsrc_rs = (k == 1) ? h0_rs : ((k - 1) & 1 ? hA_rs : hB_rs);
-                    rp.hdst = (k & 1) ? hA : hB;
-                    rp.hdst_rs = (k & 1) ? hA_rs : hB_rs;
-                    rp.rprev = chal_l + (phase == 2 ? 32 : 0) + (k - 1);
+                    const bool a = pair_in_A(j - 1);
+                    rp.fsrc = a ? s.fA : s.fB; rp.fsrc_rs = a ? fA_rs : fB_rs;
+                    rp.hsrc = a ? hA : hB; rp.hsrc_rs = a ? hA_rs : hB_rs;
+                }
+                if (j >= 1) {
+                    const bool a = pair_in_A(j);
+                    rp.fdst = a ? s.fA : s.fB; rp.fdst_rs = a ? fA_rs : fB_rs;
+                    rp.hdst = a ? hA : hB; rp.hdst_rs = a ? hA_rs : hB_rs;
+                    rp.rprev = chal_l + (phase == 2 ? 32 : 0) + (k - 2);
                 }
                 const uint32_t nbx = grid_for((uint64_t)rp.items * LG);
                 dim3 grid(nbx, R);
-                const int cls = phase == 1 ? (k == 0 ? kKRoundPh1First : kKRoundPh1)
-                                           : (k == 0 ? kKRoundPh2First : kKRoundPh2);
+                const int cls = phase == 1 ? (j == 0 ? kKPairPh1First : kKPairPh1) : (j == 0 ? kKPairPh2First : kKPairPh2);
                 timed(pf, cls, st, [&] {
                     if (phase == 1) {
-                        if (k == 0) hipLaunchKernelGGL((k_round<L, false, false>), grid, dim3(kThreads), 0, st, rp);
-                        else hipLaunchKernelGGL((k_round<L, false, true>), grid, dim3(kThreads), 0, st, rp);
+                        if (j == 0) hipLaunchKernelGGL((k_pair<L, false, 0>), grid, dim3(kThreads), 0, st, rp);
+                        else hipLaunchKernelGGL((k_pair<L, false, 2>), grid, dim3(kThreads), 0, st, rp);
                     } else {
-                        if (k == 0) hipLaunchKernelGGL((k_round<L, true, false>), grid, dim3(kThreads), 0, st, rp);
-                        else hipLaunchKernelGGL((k_round<L, true, true>), grid, dim3(kThreads), 0, st, rp);
+                        if (j == 0) hipLaunchKernelGGL((k_pair<L, true, 0>), grid, dim3(kThreads), 0, st, rp);
+                        else hipLaunchKernelGGL((k_pair<L, true, 2>), grid, dim3(kThreads), 0, st, rp);
                     }
                 });
-                timed(pf, kKRoundFin, st, [&] {
-                    hipLaunchKernelGGL(k_fin<L>, dim3(1), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials, nbx,
-                                       tk);
+                timed(pf, kKPairFin, st, [&] {
+                    hipLaunchKernelGGL(k_pair_fin<L>, dim3(1), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials,
+                                       nbx, tk);
                 });
-                ts_len += (uint64_t)R * 4 * Q;
+                ts_len += 2ull * R * 4 * Q;
                 ts_hs = ts_len;
             }
+            k0 = K;
             // round kk's sources: table kk - 1 (table 0 = the layer input V and h0)
             TailParams tp{};
             tp.fA = s.fA; tp.fB = s.fB; tp.hA = hA; tp.hB = hB;
@@ -1541,6 +1748,14 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
             if (k0 < klast) {
                 TailRepParams q{};
                 set_src(q.t, k0);
+                if (K >= 2) {
+                    // table K - 2: the layer input, or the A buffers of the last pair
+                    q.t.dbl = 1;
+                    q.t.f0 = K == 2 ? V : s.fA;
+                    q.t.f0_rs = K == 2 ? 0 : fA_rs;
+                    q.t.h0 = K == 2 ? h0 : hA;
+                    q.t.h0_rs = K == 2 ? h0_rs : hA_rs;
+                }
                 q.t.fA = s.fA; q.t.fB = s.fB; q.t.hA = hA; q.t.hB = hB;
                 q.t.fA_rs = fA_rs; q.t.fB_rs = fB_rs; q.t.hA_rs = hA_rs; q.t.hB_rs = hB_rs;
                 q.t.n = n; q.t.ts = ts;
@@ -1717,7 +1932,7 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
     Piece pieces[] = {
         {&s.eqz, R * maxM}, {&s.eqx, R * maxN}, {&s.hg0, R * maxN * L}, {&s.hgA, R * N2 * L},
         {&s.hgB, R * N4 * L}, {&s.fA, R * N2 * L}, {&s.fB, R * N4 * L}, {&s.s0, R * N1},
-        {&s.sA, R * N2}, {&s.sB, R * N4}, {&s.partials, (uint64_t)R * kMaxBlocksPerRep * 3 * L},
+        {&s.sA, R * N2}, {&s.sB, R * N4}, {&s.partials, (uint64_t)R * kMaxBlocksPerRep * kPairVals * L},
         {&s.chal, (uint64_t)(dc.D + 1) * R * kChalStride}, {&s.ab, 2ull * (dc.D + 1)}, {&s.claims, R * L},
         {&s.vxy, 2ull * R * L}, {&s.tstate, 16}, {&s.tail, (tail_bytes + 3) / 4 + 64}};
     uint64_t total = 0;
@@ -1759,7 +1974,7 @@ uint64_t batch_tables_u32(const DevCircuit &dc, uint32_t R)
     auto add = [&](uint64_t n) { u += (n + 63) & ~63ull; };
     add(R * maxM); add(R * maxN); add(R * maxN * L); add(R * N2 * L); add(R * N4 * L); add(R * N2 * L);
     add(R * N4 * L); add(R * N1); add(R * N2); add(R * N4); add((uint64_t)(dc.D + 1) * R * kChalStride);
-    add(2ull * (dc.D + 1)); add(2ull * R * L); add(16); add((uint64_t)R * 3 * L * kMaxBlocksPerRep);
+    add(2ull * (dc.D + 1)); add(2ull * R * L); add(16); add((uint64_t)R * kPairVals * L * kMaxBlocksPerRep);
     return u;
 }
 
@@ -1790,7 +2005,7 @@ uint32_t ls_tile_blocks(uint64_t tiles)
 
 uint32_t ls_work_blocks(const LsTask &t, uint32_t R)
 {
-    return (t.op == kOpEqz || t.op == kOpEqx || t.op == kOpDot || t.op == kOpRound) ? R * t.nb : t.nb;
+    return (t.op == kOpEqz || t.op == kOpEqx || t.op == kOpDot || t.op == kOpPair) ? R * t.nb : t.nb;
 }
 
 uint32_t ls_fin_blocks(const LsTask &t, uint32_t R) { return t.op == kOpTailRep ? R : 1u; }
@@ -1896,44 +2111,49 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
             uint32_t *h0 = p2 ? J.s0 : J.hg0, *hA = p2 ? J.sA : J.hgA, *hB = p2 ? J.sB : J.hgB;
             const uint64_t h0_rs = p2 ? J.s0_rs : J.hg0_rs, hA_rs = p2 ? J.sA_rs : J.hgA_rs,
                            hB_rs = p2 ? J.sB_rs : J.hgB_rs;
+            // rounds [0, K) in pairs on the whole GPU, the rest in the chains (as prove_impl)
             uint32_t k0 = 0;
             while (k0 < n && (N >> k0) > kTailRows) k0++;
-            for (uint32_t k = 0; k < k0; k++) {
+            const uint32_t K = (k0 + 1) & ~1u, Jp = K / 2;
+            auto pair_in_A = [&](uint32_t j) { return ((Jp - 1 - j) & 1) == 0; };
+            for (uint32_t j = 0; j < Jp; j++) {
+                const uint32_t k = 2 * j;
                 RoundParams rp{};
-                rp.items = (uint32_t)(N >> (k + 1));
+                rp.items = (uint32_t)(N >> (k + 2));
                 rp.partials = J.partials;
-                if (k == 0) {
+                if (j <= 1) {
                     rp.fsrc = V; rp.fsrc_rs = 0;
                     rp.hsrc = h0; rp.hsrc_rs = h0_rs;
                 } else {
-                    rp.fsrc = (k == 1) ? V : ((k - 1) & 1 ? J.fA : J.fB);
-                    rp.fsrc_rs = (k == 1) ? 0 : ((k - 1) & 1 ? J.fA_rs : J.fB_rs);
-                    rp.fdst = (k & 1) ? J.fA : J.fB;
-                    rp.fdst_rs = (k & 1) ? J.fA_rs : J.fB_rs;
-                    rp.hsrc = (k == 1) ? h0 : ((k - 1) & 1 ? hA : hB);
-                    rp.hsrc_rs = (k == 1) ? h0_rs : ((k - 1) & 1 ? hA_rs : hB_rs);
-                    rp.hdst = (k & 1) ? hA : hB;
-                    rp.hdst_rs = (k & 1) ? hA_rs : hB_rs;
-                    rp.rprev = chal_slot(l) + (p2 ? 32 : 0) + (k - 1);
+                    const bool a = pair_in_A(j - 1);
+                    rp.fsrc = a ? J.fA : J.fB; rp.fsrc_rs = a ? J.fA_rs : J.fB_rs;
+                    rp.hsrc = a ? hA : hB; rp.hsrc_rs = a ? hA_rs : hB_rs;
+                }
+                if (j >= 1) {
+                    const bool a = pair_in_A(j);
+                    rp.fdst = a ? J.fA : J.fB; rp.fdst_rs = a ? J.fA_rs : J.fB_rs;
+                    rp.hdst = a ? hA : hB; rp.hdst_rs = a ? hA_rs : hB_rs;
+                    rp.rprev = chal_slot(l) + (p2 ? 32 : 0) + (k - 2);
                 }
                 LsEntry e;
-                LsTask t = ls_task(kOpRound, job, l);
+                LsTask t = ls_task(kOpPair, job, l);
                 t.phase = phase;
                 t.k = k;
                 t.rp = rp;
                 t.nb = ls_grid((uint64_t)rp.items * LG, BS);
                 e.w[0] = t;
                 e.nw = 1;
-                LsTask fz = ls_task(kOpRoundFin, job, l);
+                LsTask fz = ls_task(kOpPairFin, job, l);
                 fz.phase = phase;
                 fz.k = k;
                 fz.nb = t.nb;
                 e.f = fz;
                 e.nf = 1;
                 ops.push_back(e);
-                ts_len += (uint64_t)R * 4 * Q;
+                ts_len += 2ull * R * 4 * Q;
                 ts_hs = ts_len;
             }
+            k0 = K;
             const uint32_t klast = n - 1;
             if (k0 < klast) {
                 LsTask fz = ls_task(kOpTailRep, job, l);
@@ -1942,6 +2162,7 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
                 fz.kend = klast;
                 fz.len0 = (uint32_t)ts_len;
                 fz.hs0 = (uint32_t)ts_hs;
+                fz.dbl = K >= 2;
                 fz.nb = 1;
                 push_f(fz);
                 ts_len += (uint64_t)(klast - k0) * R * 4 * Q;
@@ -1973,7 +2194,7 @@ void ls_launch_work(uint32_t BS, uint32_t blocks, hipStream_t st, const BatchJob
 // Work ops in the order they take within a step: the gathers first (their block counts
 // are multiples of 8), then the rest.  With profiling on, each op's range of a step is
 // launched (and timed) on its own.
-constexpr uint32_t kLsWorkOrder[] = {kOpFwd, kOpGx, kOpHy, kOpSeed, kOpEqz, kOpEqx, kOpDot, kOpRound};
+constexpr uint32_t kLsWorkOrder[] = {kOpFwd, kOpGx, kOpHy, kOpSeed, kOpEqz, kOpEqx, kOpDot};
 
 int ls_op_class(uint32_t op)
 {
@@ -1983,16 +2204,19 @@ int ls_op_class(uint32_t op)
     case kOpHy: return kKLsHy;
     case kOpEqz:
     case kOpEqx: return kKLsEq;
-    case kOpRound: return kKLsRound;
     default: return kKLsWork;
     }
 }
 
-// One step: its work, fin and chain tasks (first task, first start, count, workgroups).
+// The task lists of a step: first task, first start, count, workgroups.
+struct LsList {
+    uint32_t task, start, n, blocks;
+};
+// One step: the wide work other than round pairs and the claims (main stream), the round
+// pairs and their transcript ends (pair stream), the serial chains (chain stream).
+enum { kLsWork = 0, kLsDotFin, kLsPairs, kLsPairFin, kLsChain, kLsLists };
 struct LsStep {
-    uint32_t w_task, w_start, w_n, w_blocks;
-    uint32_t f_task, f_start, f_n, f_blocks;
-    uint32_t c_task, c_start, c_n, c_blocks;
+    LsList l[kLsLists];
 };
 
 bool ls_is_chain(uint32_t op) { return op == kOpTailRep || op == kOpLast; }
@@ -2099,7 +2323,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             J.ab = take(2ull * (dc.D + 1));
             J.vxy = take(2ull * R * L);
             J.tstate = take(16);
-            J.partials = take((uint64_t)R * 3 * L * kMaxBlocksPerRep);
+            J.partials = take((uint64_t)R * kPairVals * L * kMaxBlocksPerRep);
             J.tail = tails + tail_off[i];
             J.claims = claims + i * (uint64_t)R * L;
             J.seed = seeds + i * seed_u32;
@@ -2131,40 +2355,56 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             steps.reserve(nsteps);
             for (size_t s = 0; s < nsteps; s++) {
                 LsStep S{};
-                S.w_task = (uint32_t)tasks.size();
-                S.w_start = (uint32_t)starts.size();
-                uint32_t blocks = 0;
-                for (uint32_t op : kLsWorkOrder)
+                // appends the tasks that `take` selects (in job order) as list `li`
+                auto list = [&](int li, auto &&take, auto &&nblocks) {
+                    LsList &ls = S.l[li];
+                    ls.task = (uint32_t)tasks.size();
+                    ls.start = (uint32_t)starts.size();
+                    uint32_t blocks = 0;
                     for (size_t i = 0; i < nj; i++) {
                         if (s >= ops[i].size()) continue;
-                        const LsEntry &e = ops[i][s];
-                        for (uint32_t q = 0; q < e.nw; q++)
-                            if (e.w[q].op == op) {
-                                starts.push_back(blocks);
-                                tasks.push_back(e.w[q]);
-                                blocks += ls_work_blocks(e.w[q], R);
-                            }
+                        take(ops[i][s], [&](const LsTask &t) {
+                            starts.push_back(blocks);
+                            tasks.push_back(t);
+                            blocks += nblocks(t);
+                        });
                     }
-                S.w_n = (uint32_t)tasks.size() - S.w_task;
-                S.w_blocks = blocks;
-                starts.push_back(blocks);
-                for (int chain = 0; chain < 2; chain++) {
-                    uint32_t &t0 = chain ? S.c_task : S.f_task, &s0 = chain ? S.c_start : S.f_start;
-                    uint32_t &cnt = chain ? S.c_n : S.f_n, &nb = chain ? S.c_blocks : S.f_blocks;
-                    t0 = (uint32_t)tasks.size();
-                    s0 = (uint32_t)starts.size();
-                    blocks = 0;
-                    for (size_t i = 0; i < nj; i++) {
-                        if (s >= ops[i].size() || !ops[i][s].nf || ls_is_chain(ops[i][s].f.op) != (chain == 1))
-                            continue;
-                        starts.push_back(blocks);
-                        tasks.push_back(ops[i][s].f);
-                        blocks += ls_fin_blocks(ops[i][s].f, R);
-                    }
-                    cnt = (uint32_t)tasks.size() - t0;
-                    nb = blocks;
+                    ls.n = (uint32_t)tasks.size() - ls.task;
+                    ls.blocks = blocks;
+                    starts.push_back(blocks);
+                };
+                auto work_blocks = [&](const LsTask &t) { return ls_work_blocks(t, R); };
+                auto fin_blocks = [&](const LsTask &t) { return ls_fin_blocks(t, R); };
+                // wide work: by op (the gathers first: their block counts are multiples of 8)
+                {
+                    LsList &ls = S.l[kLsWork];
+                    ls.task = (uint32_t)tasks.size();
+                    ls.start = (uint32_t)starts.size();
+                    uint32_t blocks = 0;
+                    for (uint32_t op : kLsWorkOrder)
+                        for (size_t i = 0; i < nj; i++) {
+                            if (s >= ops[i].size()) continue;
+                            const LsEntry &e = ops[i][s];
+                            for (uint32_t q = 0; q < e.nw; q++)
+                                if (e.w[q].op == op) {
+                                    starts.push_back(blocks);
+                                    tasks.push_back(e.w[q]);
+                                    blocks += ls_work_blocks(e.w[q], R);
+                                }
+                        }
+                    ls.n = (uint32_t)tasks.size() - ls.task;
+                    ls.blocks = blocks;
                     starts.push_back(blocks);
                 }
+                list(kLsDotFin, [](const LsEntry &e, auto &&put) { if (e.nf && e.f.op == kOpDotFin) put(e.f); },
+                     fin_blocks);
+                list(kLsPairs, [](const LsEntry &e, auto &&put) {
+                    for (uint32_t q = 0; q < e.nw; q++) if (e.w[q].op == kOpPair) put(e.w[q]);
+                }, work_blocks);
+                list(kLsPairFin, [](const LsEntry &e, auto &&put) { if (e.nf && e.f.op == kOpPairFin) put(e.f); },
+                     fin_blocks);
+                list(kLsChain, [](const LsEntry &e, auto &&put) { if (e.nf && ls_is_chain(e.f.op)) put(e.f); },
+                     fin_blocks);
                 steps.push_back(S);
             }
         }
@@ -2192,13 +2432,13 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         if (!tasks.empty()) memcpy(bs.host_meta + bs.o_tasks, tasks.data(), sizeof(LsTask) * tasks.size());
         if (!starts.empty()) memcpy(bs.host_meta + bs.o_starts, starts.data(), 4 * starts.size());
         GKRX_HIP_CHECK(hipMemcpyAsync(bs.meta, bs.host_meta, meta_need, hipMemcpyHostToDevice, st));
-        bs.meta_steps.resize(steps.size() * 12);
+        bs.meta_steps.resize(steps.size() * (sizeof(LsStep) / 4));
         if (!steps.empty()) memcpy(bs.meta_steps.data(), steps.data(), steps.size() * sizeof(LsStep));
         bs.meta_used = meta_need;
         bs.meta_key.swap(key);
     }
-    static_assert(sizeof(LsStep) == 12 * 4, "step records are 12 words");
-    std::vector<LsStep> steps(bs.meta_steps.size() / 12);
+    static_assert(sizeof(LsStep) % 4 == 0, "step records are whole words");
+    std::vector<LsStep> steps(bs.meta_steps.size() / (sizeof(LsStep) / 4));
     if (!steps.empty()) memcpy(steps.data(), bs.meta_steps.data(), steps.size() * sizeof(LsStep));
     // the seeds change with every batch
     memcpy(bs.host_meta + bs.meta_used, hseed.data(), hseed.size() * 4);
@@ -2217,71 +2457,129 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     } else {
         const LsTask *dtasks = reinterpret_cast<const LsTask *>(bs.meta + bs.o_tasks);
         const uint32_t *dstarts = reinterpret_cast<const uint32_t *>(bs.meta + bs.o_starts);
-        // chains on the second stream: step t's chains wait for the main stream's step t - 1
-        // (ev_main), and the main stream's step t + 1 waits for them (ev_aux)
-        hipStream_t cs = st;
-        if (bs.chain_stream) {
+        // Three lanes: 0 main (wide work, claims), 1 pairs (+ their transcript ends), 2 chains.
+        // With chain_stream they run on three streams: at step t each lane first waits for
+        // what the other lanes recorded at the end of steps before t (a proof's op at step t
+        // depends on its op at step t - 1, on whichever lane that ran), and every lane that
+        // launched records its event after all lanes of step t are enqueued.  Otherwise every
+        // lane is the main stream.
+        hipStream_t ls_st[3] = {st, st, st};
+        hipEvent_t ls_ev[3] = {nullptr, nullptr, nullptr};
+        uint64_t rec[3] = {0, 0, 0}, seen[3][3] = {};
+        const bool multi = bs.chain_stream;
+        if (multi) {
             if (!bs.aux) {
                 GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux, hipStreamNonBlocking));
+                GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux2, hipStreamNonBlocking));
                 GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_main, hipEventDisableTiming));
                 GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_aux, hipEventDisableTiming));
+                GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_aux2, hipEventDisableTiming));
             }
-            cs = bs.aux;
-            GKRX_HIP_CHECK(hipEventRecord(bs.ev_main, st));  // the uploads above
+            ls_st[1] = bs.aux2;
+            ls_st[2] = bs.aux;
+            ls_ev[0] = bs.ev_main;
+            ls_ev[1] = bs.ev_aux2;
+            ls_ev[2] = bs.ev_aux;
+            GKRX_HIP_CHECK(hipEventRecord(ls_ev[0], st));  // the uploads above
+            rec[0] = 1;
         }
-        bool chains_pending = false;
+        auto join = [&](int x) -> int {  // lane x waits for the other lanes' latest records
+            if (!multi) return GKRX_OK;
+            for (int o = 0; o < 3; o++)
+                if (o != x && rec[o] > seen[x][o]) {
+                    GKRX_HIP_CHECK(hipStreamWaitEvent(ls_st[x], ls_ev[o], 0));
+                    seen[x][o] = rec[o];
+                }
+            return GKRX_OK;
+        };
+        auto fin = [&](const LsList &ls, hipStream_t sx, int cls) {
+            timed(pf, cls, sx, [&] {
+                if (L == 8)
+                    hipLaunchKernelGGL(k_ls_fin<8>, dim3(ls.blocks), dim3(kFinThreads), 0, sx, djobs, dtasks + ls.task,
+                                       dstarts + ls.start, ls.n, R, hash_kind);
+                else
+                    hipLaunchKernelGGL(k_ls_fin<16>, dim3(ls.blocks), dim3(kFinThreads), 0, sx, djobs, dtasks + ls.task,
+                                       dstarts + ls.start, ls.n, R, hash_kind);
+            });
+        };
         for (const LsStep &S : steps) {
-            if (S.c_n) {
-                if (cs != st) GKRX_HIP_CHECK(hipStreamWaitEvent(cs, bs.ev_main, 0));
+            bool launched[3] = {false, false, false};
+            // chains
+            if (S.l[kLsChain].n) {
+                const LsList &ls = S.l[kLsChain];
+                hipStream_t cs = ls_st[2];
+                if (int rc = join(2)) return rc;
                 timed(pf, kKLsChain, cs, [&] {
                     if (L == 8)
-                        hipLaunchKernelGGL(k_ls_chain<8>, dim3(S.c_blocks), dim3(kChainThreads), 0, cs, djobs,
-                                           dtasks + S.c_task, dstarts + S.c_start, S.c_n, R, hash_kind);
+                        hipLaunchKernelGGL(k_ls_chain<8>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
+                                           dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
                     else
-                        hipLaunchKernelGGL(k_ls_chain<16>, dim3(S.c_blocks), dim3(kChainThreads), 0, cs, djobs,
-                                           dtasks + S.c_task, dstarts + S.c_start, S.c_n, R, hash_kind);
+                        hipLaunchKernelGGL(k_ls_chain<16>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
+                                           dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
                 });
+                launched[2] = true;
             }
-            if (chains_pending) GKRX_HIP_CHECK(hipStreamWaitEvent(st, bs.ev_aux, 0));
-            auto work = [&](uint32_t blocks, uint32_t base) {
-                if (L == 8)
-                    ls_launch_work<8>(BS, blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R, base);
-                else
-                    ls_launch_work<16>(BS, blocks, st, djobs, dtasks + S.w_task, dstarts + S.w_start, S.w_n, R, base);
-            };
-            if (S.w_n && !pf) {
-                work(S.w_blocks, 0);
-            } else if (S.w_n) {
-                // profiling: one launch per run of tasks of one op class
-                const LsTask *ht = reinterpret_cast<const LsTask *>(bs.host_meta + bs.o_tasks) + S.w_task;
-                const uint32_t *hs = reinterpret_cast<const uint32_t *>(bs.host_meta + bs.o_starts) + S.w_start;
-                for (uint32_t a = 0; a < S.w_n;) {
-                    uint32_t b = a + 1;
-                    while (b < S.w_n && ls_op_class(ht[b].op) == ls_op_class(ht[a].op)) b++;
-                    timed(pf, ls_op_class(ht[a].op), st, [&] { work(hs[b] - hs[a], hs[a]); });
-                    a = b;
-                }
-            }
-            if (S.f_n)
-                timed(pf, kKLsFin, st, [&] {
-                    if (L == 8)
-                        hipLaunchKernelGGL(k_ls_fin<8>, dim3(S.f_blocks), dim3(kFinThreads), 0, st, djobs,
-                                           dtasks + S.f_task, dstarts + S.f_start, S.f_n, R, hash_kind);
-                    else
-                        hipLaunchKernelGGL(k_ls_fin<16>, dim3(S.f_blocks), dim3(kFinThreads), 0, st, djobs,
-                                           dtasks + S.f_task, dstarts + S.f_start, S.f_n, R, hash_kind);
+            // round pairs and their transcript ends
+            if (S.l[kLsPairs].n) {
+                const LsList &ls = S.l[kLsPairs];
+                hipStream_t ps = ls_st[1];
+                if (int rc = join(1)) return rc;
+                timed(pf, kKLsPair, ps, [&] {
+                    const dim3 g(ls.blocks);
+                    const LsTask *tk = dtasks + ls.task;
+                    const uint32_t *sk = dstarts + ls.start;
+                    if (L == 8) {
+                        switch (BS) {
+                        case 64: hipLaunchKernelGGL((k_ls_pair<8, 64>), g, dim3(64), 0, ps, tk, sk, ls.n); break;
+                        case 128: hipLaunchKernelGGL((k_ls_pair<8, 128>), g, dim3(128), 0, ps, tk, sk, ls.n); break;
+                        case 512: hipLaunchKernelGGL((k_ls_pair<8, 512>), g, dim3(512), 0, ps, tk, sk, ls.n); break;
+                        default: hipLaunchKernelGGL((k_ls_pair<8, 256>), g, dim3(256), 0, ps, tk, sk, ls.n); break;
+                        }
+                    } else {
+                        switch (BS) {
+                        case 64: hipLaunchKernelGGL((k_ls_pair<16, 64>), g, dim3(64), 0, ps, tk, sk, ls.n); break;
+                        case 128: hipLaunchKernelGGL((k_ls_pair<16, 128>), g, dim3(128), 0, ps, tk, sk, ls.n); break;
+                        case 512: hipLaunchKernelGGL((k_ls_pair<16, 512>), g, dim3(512), 0, ps, tk, sk, ls.n); break;
+                        default: hipLaunchKernelGGL((k_ls_pair<16, 256>), g, dim3(256), 0, ps, tk, sk, ls.n); break;
+                        }
+                    }
                 });
-            chains_pending = false;
-            if (cs != st) {
-                GKRX_HIP_CHECK(hipEventRecord(bs.ev_main, st));
-                if (S.c_n) {
-                    GKRX_HIP_CHECK(hipEventRecord(bs.ev_aux, cs));
-                    chains_pending = true;
-                }
+                if (S.l[kLsPairFin].n) fin(S.l[kLsPairFin], ps, kKPairFin);
+                launched[1] = true;
             }
+            // wide work and claims
+            const LsList &W = S.l[kLsWork];
+            if (W.n || S.l[kLsDotFin].n) {
+                if (int rc = join(0)) return rc;
+                auto work = [&](uint32_t blocks, uint32_t base) {
+                    if (L == 8) ls_launch_work<8>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
+                    else ls_launch_work<16>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
+                };
+                if (W.n && !pf) {
+                    work(W.blocks, 0);
+                } else if (W.n) {
+                    // profiling: one launch per run of tasks of one op class
+                    const LsTask *ht = reinterpret_cast<const LsTask *>(bs.host_meta + bs.o_tasks) + W.task;
+                    const uint32_t *hs = reinterpret_cast<const uint32_t *>(bs.host_meta + bs.o_starts) + W.start;
+                    for (uint32_t a = 0; a < W.n;) {
+                        uint32_t b = a + 1;
+                        while (b < W.n && ls_op_class(ht[b].op) == ls_op_class(ht[a].op)) b++;
+                        timed(pf, ls_op_class(ht[a].op), st, [&] { work(hs[b] - hs[a], hs[a]); });
+                        a = b;
+                    }
+                }
+                if (S.l[kLsDotFin].n) fin(S.l[kLsDotFin], st, kKLsFin);
+                launched[0] = true;
+            }
+            if (multi)
+                for (int x = 0; x < 3; x++)
+                    if (launched[x]) {
+                        GKRX_HIP_CHECK(hipEventRecord(ls_ev[x], ls_st[x]));
+                        rec[x]++;
+                    }
         }
-        // the copies below (and the next batch, which synchronizes st) follow every chain
-        if (chains_pending) GKRX_HIP_CHECK(hipStreamWaitEvent(st, bs.ev_aux, 0));
+        // the copies below (and the next batch, which synchronizes st) follow every lane
+        if (int rc = join(0)) return rc;
     }
     GKRX_HIP_CHECK(hipGetLastError());
     GKRX_HIP_CHECK(hipMemcpyAsync(bs.tails_host, tails, tail_off[nj] * 4, hipMemcpyDeviceToHost, st));
@@ -2314,12 +2612,13 @@ int dev_batch_supported(const DevCircuit &dc)
 
 void dev_free_batch(BatchScratch &bs)
 {
-    if (bs.aux) {
-        (void)hipStreamSynchronize(bs.aux);
-        (void)hipStreamDestroy(bs.aux);
-    }
-    if (bs.ev_main) (void)hipEventDestroy(bs.ev_main);
-    if (bs.ev_aux) (void)hipEventDestroy(bs.ev_aux);
+    for (hipStream_t x : {bs.aux, bs.aux2})
+        if (x) {
+            (void)hipStreamSynchronize(x);
+            (void)hipStreamDestroy(x);
+        }
+    for (hipEvent_t e : {bs.ev_main, bs.ev_aux, bs.ev_aux2})
+        if (e) (void)hipEventDestroy(e);
     if (bs.pool) (void)hipFree(bs.pool);
     if (bs.meta) (void)hipFree(bs.meta);
     if (bs.host_meta) (void)hipHostFree(bs.host_meta);
