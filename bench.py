@@ -30,8 +30,9 @@ HBM_PEAK_GBS = 8000.0
 # with two rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, per dispatch)
 # and scaled by what the same counters read for a 256 MiB device-to-device copy
 # (--calibration-copy-mb 256): bytes read = 2 x FETCH_SIZE KiB, written = 1 x WRITE_SIZE KiB.
-# None until measured for the current kernel.
-ROOFLINE_TRAFFIC_BYTES = None
+# k_pair<16, phase 1, double-fold>: read 2 x FETCH_SIZE, written 1 x WRITE_SIZE (KiB) per
+# dispatch (profiles/pmc_traffic_m31x16_24x2e20_keccak256.md).
+ROOFLINE_TRAFFIC_BYTES = 140514307
 
 
 def log(msg):

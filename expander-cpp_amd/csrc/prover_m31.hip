@@ -421,6 +421,8 @@ __device__ __forceinline__ void round_item_dbl(const uint32_t *__restrict__ fs, 
 // canonical (u32 accumulators keep the register count, and with it the occupancy, up).
 constexpr uint32_t kPairVals = 9;
 
+__device__ __forceinline__ void pair_add(uint32_t &acc, uint32_t a, uint32_t b) { acc = m31_add(acc, m31_mul(a, b)); }
+
 template <int L, bool PH2, int MODE>
 __device__ __forceinline__ void pair_item(const uint32_t *__restrict__ fs, uint32_t *__restrict__ fd,
                                           const uint32_t *__restrict__ hs, uint32_t *__restrict__ hd, uint64_t i,
@@ -466,20 +468,20 @@ __device__ __forceinline__ void pair_item(const uint32_t *__restrict__ fs, uint3
                                 s02, s13, m31_sub(s13, s02)};
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            acc[q][0] = m31_add(acc[q][0], m31_mul(ff[q].x, hh[q]));
-            acc[q][1] = m31_add(acc[q][1], m31_mul(ff[q].y, hh[q]));
-            acc[q][2] = m31_add(acc[q][2], m31_mul(ff[q].z, hh[q]));
-            acc[q][3] = m31_add(acc[q][3], m31_mul(ff[q].w, hh[q]));
+            pair_add(acc[q][0], ff[q].x, hh[q]);
+            pair_add(acc[q][1], ff[q].y, hh[q]);
+            pair_add(acc[q][2], ff[q].z, hh[q]);
+            pair_add(acc[q][3], ff[q].w, hh[q]);
         }
     } else {
         const uint4 h02 = add4(h[0], h[2]), h13 = add4(h[1], h[3]);
         const uint4 hh[9] = {h[0], h[1], sub4(h[1], h[0]), h[2], h[3], sub4(h[3], h[2]), h02, h13, sub4(h13, h02)};
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            acc[q][0] = m31_add(acc[q][0], m31_mul(ff[q].x, hh[q].x));
-            acc[q][1] = m31_add(acc[q][1], m31_mul(ff[q].y, hh[q].y));
-            acc[q][2] = m31_add(acc[q][2], m31_mul(ff[q].z, hh[q].z));
-            acc[q][3] = m31_add(acc[q][3], m31_mul(ff[q].w, hh[q].w));
+            pair_add(acc[q][0], ff[q].x, hh[q].x);
+            pair_add(acc[q][1], ff[q].y, hh[q].y);
+            pair_add(acc[q][2], ff[q].z, hh[q].z);
+            pair_add(acc[q][3], ff[q].w, hh[q].w);
         }
     }
 }
@@ -666,7 +668,17 @@ struct RoundParams {
     const uint32_t *rprev; // r_{k-2} of rep 0 (r_{k-1} follows it); rep stride kChalStride
     uint32_t *partials;    // value-major: [rep][kPairVals L][blocks per rep]
     uint32_t items;        // items of the pair: N_k / 4 (four rows of table k each)
+    uint32_t R;
 };
+
+// Workgroup b of a pair launch of R * nb workgroups (nb a multiple of 8) -> (rep, tile): the
+// R repetitions of a tile are consecutive workgroups of one XCD (workgroup b runs on XCD
+// b mod 8), so the rows of the layer input that they share are read from HBM once.
+__device__ __forceinline__ uint2 pair_block(uint32_t b, uint32_t R)
+{
+    const uint32_t idx = b >> 3;
+    return make_uint2(idx % R, (idx / R) * 8 + (b & 7));
+}
 
 // Adds the NV x 4 canonical lane sums of a block of BS threads (lanes sharing t % LG) into
 // out[v * ostride], v = q L + lane < NV L.  Every thread must call it.
@@ -707,7 +719,8 @@ template <int L, bool PH2, int MODE>
 __global__ void __launch_bounds__(kThreads) k_pair(RoundParams p)
 {
     constexpr uint32_t LG = L / 4;
-    const uint32_t rep = blockIdx.y, g = threadIdx.x % LG;
+    const uint2 rb_ = pair_block(blockIdx.x, p.R);
+    const uint32_t rep = rb_.x, bx = rb_.y, nb = gridDim.x / p.R, g = threadIdx.x % LG;
     const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
     const uint32_t *hs = p.hsrc + rep * p.hsrc_rs;
     uint32_t *fd = MODE ? p.fdst + rep * p.fdst_rs : nullptr;
@@ -716,10 +729,9 @@ __global__ void __launch_bounds__(kThreads) k_pair(RoundParams p)
     const uint32_t rb = MODE ? p.rprev[(uint64_t)rep * kChalStride + 1] : 0u;
     uint32_t acc[kPairVals][4] = {};
     const uint64_t total = (uint64_t)p.items * LG;
-    for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
+    for (uint64_t w = (uint64_t)bx * kThreads + threadIdx.x; w < total; w += (uint64_t)nb * kThreads)
         pair_item<L, PH2, MODE>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
-    block_reduce_vals<L, kThreads, kPairVals>(acc, p.partials + (uint64_t)rep * kPairVals * L * gridDim.x + blockIdx.x,
-                                              gridDim.x);
+    block_reduce_vals<L, kThreads, kPairVals>(acc, p.partials + (uint64_t)rep * kPairVals * L * nb + bx, nb);
 }
 
 // Adds the value-major block partials [R][V][nb] of k_pair (V = kPairVals L) or k_dot
@@ -1318,7 +1330,8 @@ template <int L, int BS, bool PH2, int MODE>
 __device__ void ls_pair(const RoundParams &p, uint32_t nb, uint32_t lb)
 {
     constexpr uint32_t LG = L / 4;
-    const uint32_t rep = lb / nb, bx = lb % nb, g = threadIdx.x % LG;
+    const uint2 rb_ = pair_block(lb, p.R);  // pair tasks start on multiples of 8 workgroups
+    const uint32_t rep = rb_.x, bx = rb_.y, g = threadIdx.x % LG;
     const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
     const uint32_t *hs = p.hsrc + rep * p.hsrc_rs;
     uint32_t *fd = MODE ? p.fdst + rep * p.fdst_rs : nullptr;
@@ -1708,8 +1721,9 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
                     rp.hdst = a ? hA : hB; rp.hdst_rs = a ? hA_rs : hB_rs;
                     rp.rprev = chal_l + (phase == 2 ? 32 : 0) + (k - 2);
                 }
-                const uint32_t nbx = grid_for((uint64_t)rp.items * LG);
-                dim3 grid(nbx, R);
+                const uint32_t nbx = (grid_for((uint64_t)rp.items * LG) + 7) & ~7u;  // a multiple of 8
+                rp.R = R;
+                dim3 grid(nbx * R);
                 const int cls = phase == 1 ? (j == 0 ? kKPairPh1First : kKPairPh1) : (j == 0 ? kKPairPh2First : kKPairPh2);
                 timed(pf, cls, st, [&] {
                     if (phase == 1) {
@@ -2135,12 +2149,13 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
                     rp.hdst = a ? hA : hB; rp.hdst_rs = a ? hA_rs : hB_rs;
                     rp.rprev = chal_slot(l) + (p2 ? 32 : 0) + (k - 2);
                 }
+                rp.R = R;
                 LsEntry e;
                 LsTask t = ls_task(kOpPair, job, l);
                 t.phase = phase;
                 t.k = k;
                 t.rp = rp;
-                t.nb = ls_grid((uint64_t)rp.items * LG, BS);
+                t.nb = (ls_grid((uint64_t)rp.items * LG, BS) + 7) & ~7u;  // a multiple of 8 (pair_block)
                 e.w[0] = t;
                 e.nw = 1;
                 LsTask fz = ls_task(kOpPairFin, job, l);
@@ -2461,12 +2476,13 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         // With chain_stream they run on three streams: at step t each lane first waits for
         // what the other lanes recorded at the end of steps before t (a proof's op at step t
         // depends on its op at step t - 1, on whichever lane that ran), and every lane that
-        // launched records its event after all lanes of step t are enqueued.  Otherwise every
-        // lane is the main stream.
+        // launched records its event after all lanes of step t are enqueued.  Otherwise (and
+        // with profiling on, so that each op class is timed alone) every lane is the main
+        // stream.
         hipStream_t ls_st[3] = {st, st, st};
         hipEvent_t ls_ev[3] = {nullptr, nullptr, nullptr};
         uint64_t rec[3] = {0, 0, 0}, seen[3][3] = {};
-        const bool multi = bs.chain_stream;
+        const bool multi = bs.chain_stream && !pf;
         if (multi) {
             if (!bs.aux) {
                 GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux, hipStreamNonBlocking));
