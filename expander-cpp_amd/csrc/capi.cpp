@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -594,6 +595,123 @@ extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t
     return GKRX_OK;
 }
 
+namespace {
+
+struct BatchPrefix {
+    std::vector<uint32_t> rz1;
+    uint32_t tstate[10];
+    uint8_t grind[32];
+};
+
+// The transcript prefixes of a batch (independent serial hash chains) on host threads, in
+// the order in which the device needs them.  The first prefixes needed run one per thread
+// (a scalar sponge finishes one commitment soonest); with the eight-lane Keccak the rest
+// run as groups of up to eight commitments of one length, each led by the earliest-needed
+// item not yet taken and filled with the next-needed items of its length (eight sponges
+// take about 2.7 times as long as one on a core).
+class PrefixRunner final : public BatchPrefixes {
+  public:
+    PrefixRunner(gkrx_ctx *ctx, gkrx_circuit *const *circuits, uint32_t n, int kind, std::vector<BatchPrefix> &pre)
+        : ctx_(ctx), circuits_(circuits), n_(n), kind_(kind), pre_(pre), done_(n, 0)
+    {
+    }
+    ~PrefixRunner() override { join(); }
+
+    void start(const std::vector<uint32_t> &order) override
+    {
+        t0_ = std::chrono::steady_clock::now();
+        const bool x8 = kind_ == 1 && host_keccak_x8();
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<uint8_t> taken(n_, 0);
+        for (size_t k = 0; k < order.size(); k++) {
+            const uint32_t lead = order[k];
+            if (taken[lead]) continue;
+            std::vector<uint32_t> g{lead};
+            taken[lead] = 1;
+            for (size_t q = k + 1; x8 && k >= nt && q < order.size() && g.size() < 8; q++)
+                if (!taken[order[q]] && circuits_[order[q]]->n[0] == circuits_[lead]->n[0]) {
+                    g.push_back(order[q]);
+                    taken[order[q]] = 1;
+                }
+            groups_.push_back(std::move(g));
+        }
+        for (size_t t = 0; t < std::min<size_t>(nt, groups_.size()); t++)
+            pool_.emplace_back([this] {
+                for (size_t g; (g = next_++) < groups_.size();) {
+                    run(groups_[g]);
+                    {
+                        std::lock_guard<std::mutex> lk(mu_);
+                        for (uint32_t i : groups_[g]) done_[i] = 1;
+                        ndone_ += groups_[g].size();
+                        if (ndone_ == n_) t1_ = std::chrono::steady_clock::now();
+                    }
+                    cv_.notify_all();
+                }
+            });
+    }
+
+    void wait(uint32_t i) override
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return done_[i] != 0; });
+    }
+
+    // joins the threads; returns their wall time in ms (0 if they never started)
+    double join()
+    {
+        for (auto &th : pool_) th.join();
+        pool_.clear();
+        if (ndone_ != n_ || n_ == 0) return 0;
+        return std::chrono::duration<double, std::milli>(t1_ - t0_).count();
+    }
+
+  private:
+    void run(const std::vector<uint32_t> &g)
+    {
+        const size_t S = (size_t)ctx_->L * 4;
+        const uint32_t gb = (uint32_t)ctx_->cfg.grinding_bits;
+        gkrx_circuit *c0 = circuits_[g[0]];
+        const size_t len = ((size_t)1 << c0->n[0]) * S;
+        if (g.size() == 1) {
+            BatchPrefix &p = pre_[g[0]];
+            host_transcript_prefix(kind_, reinterpret_cast<const uint8_t *>(c0->witness.data()), len, ctx_->R,
+                                   c0->m[c0->D - 1], gb, p.grind, p.rz1.data(), p.tstate);
+            return;
+        }
+        const uint8_t *commits[8];
+        uint32_t ml[8], *rz[8];
+        uint8_t grind[8][32];
+        uint32_t ts[8][10];
+        for (size_t l = 0; l < g.size(); l++) {
+            gkrx_circuit *c = circuits_[g[l]];
+            commits[l] = reinterpret_cast<const uint8_t *>(c->witness.data());
+            ml[l] = c->m[c->D - 1];
+            rz[l] = pre_[g[l]].rz1.data();
+        }
+        host_transcript_prefix_x8((uint32_t)g.size(), commits, len, ctx_->R, ml, gb, grind, rz, ts);
+        for (size_t l = 0; l < g.size(); l++) {
+            memcpy(pre_[g[l]].grind, grind[l], 32);
+            memcpy(pre_[g[l]].tstate, ts[l], 40);
+        }
+    }
+
+    gkrx_ctx *ctx_;
+    gkrx_circuit *const *circuits_;
+    uint32_t n_;
+    int kind_;
+    std::vector<BatchPrefix> &pre_;
+    std::vector<uint8_t> done_;
+    size_t ndone_ = 0;
+    std::vector<std::vector<uint32_t>> groups_;
+    std::atomic<size_t> next_{0};
+    std::vector<std::thread> pool_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::chrono::steady_clock::time_point t0_, t1_;
+};
+
+}  // namespace
+
 extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, uint8_t *const *proofs,
                                 const size_t *proof_caps, size_t *proof_lens, uint32_t *const *claimed_v)
 {
@@ -632,57 +750,8 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
     GKRX_HIP_CHECK(hipSetDevice(ctx->device));
     const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
     const size_t S = (size_t)ctx->L * 4;
-    struct Prefix {
-        std::vector<uint32_t> rz1;
-        uint32_t tstate[10];
-        uint8_t grind[32];
-    };
-    std::vector<Prefix> pre(n);
-    // the transcript prefixes are independent serial hash chains: one per host thread,
-    // largest commitment first
-    std::vector<uint32_t> order(n);
-    for (uint32_t i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return circuits[a]->n[0] > circuits[b]->n[0]; });
+    std::vector<BatchPrefix> pre(n);
     for (uint32_t i = 0; i < n; i++) pre[i].rz1.resize((size_t)ctx->R * circuits[i]->m[circuits[i]->D - 1]);
-    if (kind == 1 && host_keccak_x8()) {
-        // groups of up to eight commitments of one length: eight sponges per core
-        std::vector<std::pair<uint32_t, uint32_t>> groups;  // [first, last) in `order`
-        for (uint32_t k = 0; k < n;) {
-            uint32_t e = k + 1;
-            while (e < n && e - k < 8 && circuits[order[e]]->n[0] == circuits[order[k]]->n[0]) e++;
-            groups.push_back({k, e});
-            k = e;
-        }
-        parallel_for(groups.size(), 16, [&](size_t g) {
-            const uint32_t k0 = groups[g].first, cnt = groups[g].second - k0;
-            const uint8_t *commits[8];
-            uint32_t ml[8], *rz[8];
-            uint8_t grind[8][32];
-            uint32_t ts[8][10];
-            for (uint32_t l = 0; l < cnt; l++) {
-                gkrx_circuit *c = circuits[order[k0 + l]];
-                commits[l] = reinterpret_cast<const uint8_t *>(c->witness.data());
-                ml[l] = c->m[c->D - 1];
-                rz[l] = pre[order[k0 + l]].rz1.data();
-            }
-            host_transcript_prefix_x8(cnt, commits, ((size_t)1 << circuits[order[k0]]->n[0]) * S, ctx->R, ml,
-                                      (uint32_t)ctx->cfg.grinding_bits, grind, rz, ts);
-            for (uint32_t l = 0; l < cnt; l++) {
-                memcpy(pre[order[k0 + l]].grind, grind[l], 32);
-                memcpy(pre[order[k0 + l]].tstate, ts[l], 40);
-            }
-        });
-    } else {
-        parallel_for(n, 16, [&](size_t k) {
-            const uint32_t i = order[k];
-            gkrx_circuit *c = circuits[i];
-            host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()),
-                                   ((size_t)1 << c->n[0]) * S, ctx->R, c->m[c->D - 1],
-                                   (uint32_t)ctx->cfg.grinding_bits, pre[i].grind, pre[i].rz1.data(), pre[i].tstate);
-        });
-    }
-    const auto t1 = std::chrono::steady_clock::now();
     std::vector<BatchItem> items(n);
     for (uint32_t i = 0; i < n; i++) {
         gkrx_circuit *c = circuits[i];
@@ -690,7 +759,12 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
         items[i] = BatchItem{&c->dev, pre[i].rz1.data(), pre[i].tstate, pre[i].grind, tail_bytes(ctx, c),
                              proofs[i] + commit_len};
     }
-    int rc = dev_prove_batch(items, ctx->R, kind, ctx->batch, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr);
+    // the prefixes run on host threads while the batch is enqueued and proved
+    PrefixRunner runner(ctx, circuits, n, kind, pre);
+    const auto t1 = std::chrono::steady_clock::now();
+    int rc = dev_prove_batch(items, ctx->R, kind, ctx->batch, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr,
+                             &runner);
+    const double prefix_ms = runner.join();
     if (rc) return rc;
     // while the device proves: the commitments (the input layers) into the proof buffers
     parallel_for(n, 16, [&](size_t i) {
@@ -707,7 +781,7 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
     }
     if (ctx->prof.on) ctx->prof.collect();
     const auto t2 = std::chrono::steady_clock::now();
-    ctx->last.host_prefix_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ctx->last.host_prefix_ms = prefix_ms;
     ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
     return GKRX_OK;

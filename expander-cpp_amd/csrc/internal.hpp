@@ -111,6 +111,8 @@ struct BatchScratch {
     std::vector<uint64_t> meta_key;
     uint64_t meta_used = 0, o_order = 0, o_tasks = 0, o_starts = 0;
     std::vector<uint32_t> meta_steps;
+    std::vector<uint32_t> meta_seed_step;   // lockstep: the step whose work seeds job i's transcript
+    std::vector<uint32_t> meta_seed_order;  // the jobs in the order of their seed slots
     uint32_t *claims_host = nullptr;
     uint64_t claims_cap = 0;
     // pinned staging of the proof tails: the device copies land here (a copy into pageable
@@ -201,12 +203,23 @@ int dev_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf);
 int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, const uint32_t *rz1_host,
               const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,
               Profiler *pf);
+// The transcript prefixes of a batch, computed while the batch is enqueued: start() hands
+// over the order in which the items' prefixes are needed, wait(i) returns once item i's
+// rz1, tstate and grind (which BatchItem points at) are written.
+struct BatchPrefixes {
+    virtual void start(const std::vector<uint32_t> &order) = 0;
+    virtual void wait(uint32_t i) = 0;
+    virtual ~BatchPrefixes() = default;
+};
+
 // Proves every item with the engine bs.engine selects.  Enqueues the copies of the
 // tails and of the claims (item-major, R * L each, to bs.host_claims()) into pinned
 // staging; once the stream has reached them, dev_batch_collect copies each tail to
-// items[i].tail_out.
+// items[i].tail_out.  With `pre`, the items' prefixes may still be in progress: each is
+// awaited just before the first step that needs it (the lockstep engine seeds a proof's
+// transcript after its forward pass), otherwise they must be ready.
 int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_kind, BatchScratch &bs, hipStream_t st,
-                    Profiler *pf);
+                    Profiler *pf, BatchPrefixes *pre = nullptr);
 void dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &bs);
 // Device bytes the batch prover takes for one proof of dc (its tables and transcript).
 uint64_t dev_batch_job_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes);

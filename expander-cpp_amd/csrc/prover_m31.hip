@@ -2043,8 +2043,10 @@ struct LsEntry {
 };
 
 // The steps of one proof, in the order of the single-proof path (prove_impl): the same
-// kernels, round split and transcript positions.  J holds the device addresses of the
-// proof's tables; ts_len / ts_hs is the transcript position after the host prefix.
+// kernels, round split and transcript positions.  The first D entries are the forward
+// pass; the rest (the sumcheck) needs the transcript seed (kOpSeed, placed by the caller)
+// one step before it.  J holds the device addresses of the proof's tables; ts_len / ts_hs
+// is the transcript position after the host prefix.
 void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t R, uint32_t BS, uint64_t ts_len,
                 uint64_t ts_hs, std::vector<LsEntry> &ops)
 {
@@ -2070,12 +2072,6 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
         t.nb = ls_gather_blocks((1ull << ly.m) * LG, BS);
         e.w[0] = t;
         e.nw = 1;
-        if (l == 0) {
-            LsTask s = ls_task(kOpSeed, job, 0);
-            s.nb = 1;
-            e.w[1] = s;
-            e.nw = 2;
-        }
         ops.push_back(e);
     }
     for (int li = (int)D - 1; li >= 0; li--) {
@@ -2241,7 +2237,7 @@ inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 }  // namespace
 
 int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_kind, BatchScratch &bs, hipStream_t st,
-                    Profiler *pf)
+                    Profiler *pf, BatchPrefixes *pre)
 {
     const size_t nj = items.size();
     if (nj == 0) return GKRX_OK;
@@ -2278,27 +2274,23 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     bs.tail_stage_off.assign(tail_off.begin(), tail_off.end());
     uint32_t *tails = bs.pool + job_off[nj];
     uint32_t *claims = tails + tail_off[nj];
+    // seed records (rz1 in chal-slot layout, transcript state, grind bytes), one per job in
+    // the order in which the schedule needs them
     uint32_t *seeds = claims + nj * (uint64_t)R * L;
-    // seed records: rz1 in chal-slot layout, transcript state, grind bytes
-    std::vector<uint32_t> hseed(nj * seed_u32, 0);
-    for (size_t i = 0; i < nj; i++) {
+    // The transcript position after the host prefix follows from the shapes alone: the 32
+    // grind bytes are in the buffer, hashed once if output challenges were drawn.
+    auto ts_hashed = [&](size_t i) -> uint64_t {
         const DevCircuit &dc = *items[i].dc;
-        uint32_t *sd = hseed.data() + i * seed_u32;
-        const uint32_t mlast = dc.layers[dc.D - 1].m;
-        for (uint32_t j = 0; j < R; j++)
-            for (uint32_t v = 0; v < mlast; v++) sd[j * kChalStride + v] = items[i].rz1[(size_t)j * mlast + v];
-        memcpy(sd + R * kChalStride, items[i].tstate, 40);
-        memcpy(sd + R * kChalStride + 10, items[i].grind, 32);
-    }
+        return (uint64_t)R * dc.layers[dc.D - 1].m ? 32 : 0;
+    };
     // The job table and the schedule depend only on the circuits' shapes, their device
-    // addresses, the transcript position after the prefix and the pool: a batch with the
-    // same key reuses the metadata already on the device.
+    // addresses and the pool: a batch with the same key reuses the metadata already on the
+    // device.
     std::vector<uint64_t> key = {(uint64_t)lockstep, BS, L, R, nj, (uint64_t)(uintptr_t)bs.pool};
     for (size_t i = 0; i < nj; i++) {
         const DevCircuit &dc = *items[i].dc;
         key.insert(key.end(), {(uint64_t)(uintptr_t)dc.values, (uint64_t)(uintptr_t)dc.csr_pool,
-                               (uint64_t)(uintptr_t)dc.d_layers, dc.D, items[i].tstate[0], items[i].tstate[1],
-                               items[i].tail_bytes});
+                               (uint64_t)(uintptr_t)dc.d_layers, dc.D, items[i].tail_bytes});
         for (uint32_t l = 0; l < dc.D; l++) {
             const LayerDev &ly = dc.layers[l];
             key.insert(key.end(), {((uint64_t)ly.n << 32) | ly.m, ly.v_off, ly.out_off, (uint64_t)(uintptr_t)ly.fw_ptr,
@@ -2341,7 +2333,6 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             J.partials = take((uint64_t)R * kPairVals * L * kMaxBlocksPerRep);
             J.tail = tails + tail_off[i];
             J.claims = claims + i * (uint64_t)R * L;
-            J.seed = seeds + i * seed_u32;
             uint64_t w = 0;
             for (uint32_t l = 0; l < dc.D; l++) w += (1ull << dc.layers[l].n);
             work[i] = {w, (uint32_t)i};
@@ -2351,20 +2342,51 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         std::vector<LsTask> tasks;
         std::vector<uint32_t> starts;
         std::vector<LsStep> steps;
+        bs.meta_seed_step.assign(nj, 0);
         if (!lockstep) {
             std::stable_sort(work.begin(), work.end(), [](auto &x, auto &y) { return x.first > y.first; });
             horder.resize(nj);
             for (size_t i = 0; i < nj; i++) horder[i] = work[i].second;
         } else {
+            // Every proof's forward pass first (steps [0, D)): it needs no transcript.  Then
+            // the sumchecks right-aligned: every proof ends at the last step, so a proof with
+            // fewer sumcheck steps starts later and needs its host prefix later.  A proof's
+            // seed runs one step before its first sumcheck op.
             std::vector<std::vector<LsEntry>> ops(nj);
-            size_t nsteps = 0, ntask = 0;
+            std::vector<LsEntry> seed_entry(nj);
+            std::vector<size_t> base(nj);
+            size_t dmax = 0, smax = 0, ntask = 0;
             for (size_t i = 0; i < nj; i++) {
-                ops[i].reserve((size_t)items[i].dc->D * (8 + 2 * items[i].dc->max_n));
-                ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, items[i].tstate[0], items[i].tstate[1],
-                           ops[i]);
-                nsteps = std::max(nsteps, ops[i].size());
-                ntask += 2 * ops[i].size();
+                const uint32_t D = items[i].dc->D;
+                ops[i].reserve((size_t)D * (8 + 2 * items[i].dc->max_n));
+                ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, 32, ts_hashed(i), ops[i]);
+                dmax = std::max<size_t>(dmax, D);
+                smax = std::max(smax, ops[i].size() - D);
+                ntask += 2 * ops[i].size() + 1;
             }
+            const size_t nsteps = dmax + smax;
+            for (size_t i = 0; i < nj; i++) {
+                const uint32_t D = items[i].dc->D;
+                base[i] = nsteps - (ops[i].size() - D);  // >= dmax >= D
+                bs.meta_seed_step[i] = (uint32_t)(base[i] - 1);
+                LsTask sd = ls_task(kOpSeed, (uint32_t)i, 0);
+                sd.nb = 1;
+                if (base[i] == D) {  // the seed shares the last forward step
+                    LsEntry &e = ops[i][D - 1];
+                    e.w[e.nw++] = sd;
+                } else {
+                    seed_entry[i].w[0] = sd;
+                    seed_entry[i].nw = 1;
+                }
+            }
+            // job i's entry at step s
+            auto at = [&](size_t i, size_t s) -> const LsEntry * {
+                const uint32_t D = items[i].dc->D;
+                if (s < D) return &ops[i][s];
+                if (s + 1 == base[i]) return &seed_entry[i];
+                if (s >= base[i]) return &ops[i][D + (s - base[i])];
+                return nullptr;
+            };
             tasks.reserve(ntask);
             starts.reserve(ntask + 3 * nsteps);
             steps.reserve(nsteps);
@@ -2377,8 +2399,9 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                     ls.start = (uint32_t)starts.size();
                     uint32_t blocks = 0;
                     for (size_t i = 0; i < nj; i++) {
-                        if (s >= ops[i].size()) continue;
-                        take(ops[i][s], [&](const LsTask &t) {
+                        const LsEntry *e = at(i, s);
+                        if (!e) continue;
+                        take(*e, [&](const LsTask &t) {
                             starts.push_back(blocks);
                             tasks.push_back(t);
                             blocks += nblocks(t);
@@ -2398,13 +2421,13 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                     uint32_t blocks = 0;
                     for (uint32_t op : kLsWorkOrder)
                         for (size_t i = 0; i < nj; i++) {
-                            if (s >= ops[i].size()) continue;
-                            const LsEntry &e = ops[i][s];
-                            for (uint32_t q = 0; q < e.nw; q++)
-                                if (e.w[q].op == op) {
+                            const LsEntry *e = at(i, s);
+                            if (!e) continue;
+                            for (uint32_t q = 0; q < e->nw; q++)
+                                if (e->w[q].op == op) {
                                     starts.push_back(blocks);
-                                    tasks.push_back(e.w[q]);
-                                    blocks += ls_work_blocks(e.w[q], R);
+                                    tasks.push_back(e->w[q]);
+                                    blocks += ls_work_blocks(e->w[q], R);
                                 }
                         }
                     ls.n = (uint32_t)tasks.size() - ls.task;
@@ -2423,6 +2446,13 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                 steps.push_back(S);
             }
         }
+        // seed slots in the order of need
+        std::vector<uint32_t> &sorder = bs.meta_seed_order;
+        sorder.resize(nj);
+        for (size_t i = 0; i < nj; i++) sorder[i] = (uint32_t)i;
+        std::stable_sort(sorder.begin(), sorder.end(),
+                         [&](uint32_t a, uint32_t b) { return bs.meta_seed_step[a] < bs.meta_seed_step[b]; });
+        for (size_t q = 0; q < nj; q++) hjobs[sorder[q]].seed = seeds + q * seed_u32;
         // device metadata: [jobs][order][tasks][starts]; the pinned staging adds the seeds
         bs.o_order = align256(nj * sizeof(BatchJob));
         bs.o_tasks = align256(bs.o_order + 4 * horder.size());
@@ -2435,12 +2465,13 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             GKRX_HIP_CHECK(hipMalloc(&bs.meta, meta_need));
             bs.meta_bytes = meta_need;
         }
-        if (meta_need + hseed.size() * 4 > bs.host_meta_bytes) {
+        const uint64_t stage_need = meta_need + nj * seed_u32 * 4;
+        if (stage_need > bs.host_meta_bytes) {
             if (bs.host_meta) (void)hipHostFree(bs.host_meta);
             bs.host_meta = nullptr;
             bs.host_meta_bytes = 0;
-            GKRX_HIP_CHECK(hipHostMalloc(&bs.host_meta, meta_need + hseed.size() * 4, hipHostMallocDefault));
-            bs.host_meta_bytes = meta_need + hseed.size() * 4;
+            GKRX_HIP_CHECK(hipHostMalloc(&bs.host_meta, stage_need, hipHostMallocDefault));
+            bs.host_meta_bytes = stage_need;
         }
         memcpy(bs.host_meta, hjobs.data(), nj * sizeof(BatchJob));
         if (!horder.empty()) memcpy(bs.host_meta + bs.o_order, horder.data(), 4 * horder.size());
@@ -2455,9 +2486,34 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     static_assert(sizeof(LsStep) % 4 == 0, "step records are whole words");
     std::vector<LsStep> steps(bs.meta_steps.size() / (sizeof(LsStep) / 4));
     if (!steps.empty()) memcpy(steps.data(), bs.meta_steps.data(), steps.size() * sizeof(LsStep));
-    // the seeds change with every batch
-    memcpy(bs.host_meta + bs.meta_used, hseed.data(), hseed.size() * 4);
-    GKRX_HIP_CHECK(hipMemcpyAsync(seeds, bs.host_meta + bs.meta_used, hseed.size() * 4, hipMemcpyHostToDevice, st));
+    // The seeds change with every batch.  upload(q) copies the records of slots [seeded, q)
+    // (waiting for their prefixes first) through the pinned staging after the metadata.
+    const std::vector<uint32_t> &sorder = bs.meta_seed_order;
+    uint32_t *stage = reinterpret_cast<uint32_t *>(bs.host_meta + bs.meta_used);
+    size_t seeded = 0;
+    if (pre) pre->start(sorder);
+    auto upload = [&](size_t q) -> int {
+        if (q <= seeded) return GKRX_OK;
+        for (size_t k = seeded; k < q; k++) {
+            const size_t i = sorder[k];
+            if (pre) pre->wait((uint32_t)i);
+            const DevCircuit &dc = *items[i].dc;
+            uint32_t *sd = stage + k * seed_u32;
+            const uint32_t mlast = dc.layers[dc.D - 1].m;
+            memset(sd, 0, seed_u32 * 4);
+            for (uint32_t j = 0; j < R; j++)
+                for (uint32_t v = 0; v < mlast; v++) sd[j * kChalStride + v] = items[i].rz1[(size_t)j * mlast + v];
+            memcpy(sd + R * kChalStride, items[i].tstate, 40);
+            memcpy(sd + R * kChalStride + 10, items[i].grind, 32);
+        }
+        GKRX_HIP_CHECK(hipMemcpyAsync(seeds + seeded * seed_u32, stage + seeded * seed_u32,
+                                      (q - seeded) * seed_u32 * 4, hipMemcpyHostToDevice, st));
+        seeded = q;
+        return GKRX_OK;
+    };
+    // with the prefixes still in progress, the lockstep engine uploads them step by step
+    if (!(pre && lockstep))
+        if (int rc = upload(nj)) return rc;
     const BatchJob *djobs = reinterpret_cast<const BatchJob *>(bs.meta);
     if (!lockstep) {
         const uint32_t *dorder = reinterpret_cast<const uint32_t *>(bs.meta + bs.o_order);
@@ -2499,6 +2555,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             GKRX_HIP_CHECK(hipEventRecord(ls_ev[0], st));  // the uploads above
             rec[0] = 1;
         }
+        uint32_t si = 0;  // step index
         auto join = [&](int x) -> int {  // lane x waits for the other lanes' latest records
             if (!multi) return GKRX_OK;
             for (int o = 0; o < 3; o++)
@@ -2519,6 +2576,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             });
         };
         for (const LsStep &S : steps) {
+            const uint32_t s = si++;
             bool launched[3] = {false, false, false};
             // chains
             if (S.l[kLsChain].n) {
@@ -2566,6 +2624,10 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             // wide work and claims
             const LsList &W = S.l[kLsWork];
             if (W.n || S.l[kLsDotFin].n) {
+                // the seeds of this step's SEED tasks (the other lanes are enqueued already)
+                size_t q = seeded;
+                while (q < nj && bs.meta_seed_step[sorder[q]] <= s) q++;
+                if (int rc = upload(q)) return rc;
                 if (int rc = join(0)) return rc;
                 auto work = [&](uint32_t blocks, uint32_t base) {
                     if (L == 8) ls_launch_work<8>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
@@ -2594,6 +2656,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                         rec[x]++;
                     }
         }
+        if (seeded != nj) return GKRX_ERR_STATE;  // every proof has a SEED task
         // the copies below (and the next batch, which synchronizes st) follow every lane
         if (int rc = join(0)) return rc;
     }

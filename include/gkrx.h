@@ -169,8 +169,11 @@ int gkrx_server_stats(gkrx_server *s, uint64_t out[6]);
 /* Finishes every queued request, then frees the server. */
 int gkrx_server_destroy(gkrx_server *s);
 
-/* Millisecond split of the last gkrx_prove on this context: [0] host transcript prefix
- * (commitment hash + grinding + output challenges), [1] device work, [2] total. */
+/* Millisecond split of the last gkrx_prove or M31 gkrx_prove_batch on this context:
+ * [0] host transcript prefix (commitment hash + grinding + output challenges), [1] device
+ * work, [2] total.  gkrx_prove runs them in turn.  gkrx_prove_batch overlaps them: [0] is
+ * the wall time of the prefix threads, [1] the time from the first launch to the proofs
+ * on the host (it includes waits for prefixes the device needs before they are done). */
 int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3);
 
 /* Per-kernel-class timing with HIP events on the context's stream (benchmark support; no
