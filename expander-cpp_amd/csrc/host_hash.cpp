@@ -111,45 +111,72 @@ const uint64_t kRC[24] = {
     0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
     0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
 
-inline uint64_t rol(uint64_t x, int n) { return (x << n) | (x >> (64 - n)); }
+inline uint64_t rol(uint64_t x, int n) { return (x << n) | (x >> ((64 - n) & 63)); }
+
+// Keccak-f[1600] on 25 named lanes (sXY = lane (x, y) of state s), two rounds per trip
+// (a -> e -> a) so that the state stays in registers.  KF_THETA forms the column parities
+// c and t[x] = c[x-1] ^ rol(c[x+1], 1).  KF_ROW builds output row Y: its lane X is input
+// lane (x, y) = ((3Y + X) mod 5, X) (pi moves (x, y) to (y, 2x + 3y)) xor t[x], rotated by
+// rho(x, y) (the triangular numbers 1, 3, 6, ... mod 64 along pi's orbit of (1, 0)), then
+// chi along the row.
+#define KF_THETA(s)                                                                                           \
+    const uint64_t c0 = s##00 ^ s##01 ^ s##02 ^ s##03 ^ s##04, c1 = s##10 ^ s##11 ^ s##12 ^ s##13 ^ s##14,    \
+                   c2 = s##20 ^ s##21 ^ s##22 ^ s##23 ^ s##24, c3 = s##30 ^ s##31 ^ s##32 ^ s##33 ^ s##34,    \
+                   c4 = s##40 ^ s##41 ^ s##42 ^ s##43 ^ s##44;                                                \
+    const uint64_t t0 = c4 ^ rol(c1, 1), t1 = c0 ^ rol(c2, 1), t2 = c1 ^ rol(c3, 1), t3 = c2 ^ rol(c4, 1),     \
+                   t4 = c3 ^ rol(c0, 1);
+#define KF_ROW(o, s, Y, l0, x0, r0, l1, x1, r1, l2, x2, r2, l3, x3, r3, l4, x4, r4)                          \
+    {                                                                                                         \
+        const uint64_t b0 = rol(s##l0 ^ t##x0, r0), b1 = rol(s##l1 ^ t##x1, r1), b2 = rol(s##l2 ^ t##x2, r2), \
+                       b3 = rol(s##l3 ^ t##x3, r3), b4 = rol(s##l4 ^ t##x4, r4);                              \
+        o##0##Y = b0 ^ (~b1 & b2);                                                                            \
+        o##1##Y = b1 ^ (~b2 & b3);                                                                            \
+        o##2##Y = b2 ^ (~b3 & b4);                                                                            \
+        o##3##Y = b3 ^ (~b4 & b0);                                                                            \
+        o##4##Y = b4 ^ (~b0 & b1);                                                                            \
+    }
+#define KF_ROUND(o, s, rc)                                                   \
+    {                                                                        \
+        KF_THETA(s)                                                          \
+        KF_ROW(o, s, 0, 00, 0, 0, 11, 1, 44, 22, 2, 43, 33, 3, 21, 44, 4, 14) \
+        KF_ROW(o, s, 1, 30, 3, 28, 41, 4, 20, 02, 0, 3, 13, 1, 45, 24, 2, 61) \
+        KF_ROW(o, s, 2, 10, 1, 1, 21, 2, 6, 32, 3, 25, 43, 4, 8, 04, 0, 18)   \
+        KF_ROW(o, s, 3, 40, 4, 27, 01, 0, 36, 12, 1, 10, 23, 2, 15, 34, 3, 56) \
+        KF_ROW(o, s, 4, 20, 2, 62, 31, 3, 55, 42, 4, 39, 03, 0, 41, 14, 1, 2) \
+        o##00 ^= (rc);                                                       \
+    }
+
+__attribute__((always_inline)) inline void keccak_f_lanes(uint64_t A[25])
+{
+    uint64_t a00 = A[0], a10 = A[1], a20 = A[2], a30 = A[3], a40 = A[4], a01 = A[5], a11 = A[6], a21 = A[7],
+             a31 = A[8], a41 = A[9], a02 = A[10], a12 = A[11], a22 = A[12], a32 = A[13], a42 = A[14], a03 = A[15],
+             a13 = A[16], a23 = A[17], a33 = A[18], a43 = A[19], a04 = A[20], a14 = A[21], a24 = A[22], a34 = A[23],
+             a44 = A[24];
+    uint64_t e00, e10, e20, e30, e40, e01, e11, e21, e31, e41, e02, e12, e22, e32, e42, e03, e13, e23, e33, e43,
+        e04, e14, e24, e34, e44;
+    for (int round = 0; round < 24; round += 2) {
+        KF_ROUND(e, a, kRC[round])
+        KF_ROUND(a, e, kRC[round + 1])
+    }
+    A[0] = a00, A[1] = a10, A[2] = a20, A[3] = a30, A[4] = a40, A[5] = a01, A[6] = a11, A[7] = a21, A[8] = a31;
+    A[9] = a41, A[10] = a02, A[11] = a12, A[12] = a22, A[13] = a32, A[14] = a42, A[15] = a03, A[16] = a13;
+    A[17] = a23, A[18] = a33, A[19] = a43, A[20] = a04, A[21] = a14, A[22] = a24, A[23] = a34, A[24] = a44;
+}
+#undef KF_ROUND
+#undef KF_ROW
+#undef KF_THETA
+
+// the same code twice: with BMI1/BMI2 (andn for ~b & c, rorx for the rotations) where the
+// CPU has them, plain x86-64 otherwise
+void keccak_f_plain(uint64_t A[25]) { keccak_f_lanes(A); }
+__attribute__((target("bmi,bmi2"))) void keccak_f_bmi(uint64_t A[25]) { keccak_f_lanes(A); }
+
+const bool kHasBmi2 = __builtin_cpu_supports("bmi") && __builtin_cpu_supports("bmi2");
 
 void keccak_f(uint64_t A[25])
 {
-    for (int round = 0; round < 24; round++) {
-        uint64_t C[5];
-        for (int x = 0; x < 5; x++) C[x] = A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20];
-        uint64_t D[5];
-        for (int x = 0; x < 5; x++) D[x] = C[(x + 4) % 5] ^ rol(C[(x + 1) % 5], 1);
-        uint64_t B[25];
-        B[0] = A[0] ^ D[0];
-        B[10] = rol(A[1] ^ D[1], 1);
-        B[20] = rol(A[2] ^ D[2], 62);
-        B[5] = rol(A[3] ^ D[3], 28);
-        B[15] = rol(A[4] ^ D[4], 27);
-        B[16] = rol(A[5] ^ D[0], 36);
-        B[1] = rol(A[6] ^ D[1], 44);
-        B[11] = rol(A[7] ^ D[2], 6);
-        B[21] = rol(A[8] ^ D[3], 55);
-        B[6] = rol(A[9] ^ D[4], 20);
-        B[7] = rol(A[10] ^ D[0], 3);
-        B[17] = rol(A[11] ^ D[1], 10);
-        B[2] = rol(A[12] ^ D[2], 43);
-        B[12] = rol(A[13] ^ D[3], 25);
-        B[22] = rol(A[14] ^ D[4], 39);
-        B[23] = rol(A[15] ^ D[0], 41);
-        B[8] = rol(A[16] ^ D[1], 45);
-        B[18] = rol(A[17] ^ D[2], 15);
-        B[3] = rol(A[18] ^ D[3], 21);
-        B[13] = rol(A[19] ^ D[4], 8);
-        B[14] = rol(A[20] ^ D[0], 18);
-        B[24] = rol(A[21] ^ D[1], 2);
-        B[9] = rol(A[22] ^ D[2], 61);
-        B[19] = rol(A[23] ^ D[3], 56);
-        B[4] = rol(A[24] ^ D[4], 14);
-        for (int y = 0; y < 25; y += 5)
-            for (int x = 0; x < 5; x++) A[y + x] = B[y + x] ^ (~B[y + (x + 1) % 5] & B[y + (x + 2) % 5]);
-        A[0] ^= kRC[round];
-    }
+    if (kHasBmi2) keccak_f_bmi(A);
+    else keccak_f_plain(A);
 }
 
 // ---- eight Keccak-256 sponges at once: lane l of every state word belongs to message l
