@@ -6,9 +6,14 @@
 //                                                     include/LinearGKR/sumcheck_helper.hpp:188-196
 //   k_gx       <- _prepare_g_x_vals scatters          sumcheck_helper.hpp:198-229
 //   k_hy       <- _prepare_h_y_vals scatter           sumcheck_helper.hpp:233-261
-//   k_round    <- receive_challenge (fold of round k-1) fused with poly_eval_at (round k)
+//   k_pair     <- receive_challenge (folds of rounds k-2, k-1) fused with poly_eval_at
+//                 (rounds k and k+1, the latter as a quadratic in r_k)
 //                                                     sumcheck_helper.hpp:68-133, on the whole GPU
-//   k_tail     <- the same for the small last rounds of a phase, in one workgroup
+//   k_tail_rep, k_tail <- the same one round at a time for the small last rounds of a
+//                 phase, in one workgroup
+//   k_ls_*     <- all of the above for a batch of independent proofs, one step of every
+//                 proof per launch (no reference counterpart: the reference proves one
+//                 proof per thread, bench_keccak.cpp:32-42)
 //   transcript_step <- the transcript steps of sumcheck_prove_gkr_layer / gkr_prove
 //                                                     include/LinearGKR/sumcheck.hpp:72-103,
 //                                                     include/LinearGKR/gkr.hpp:58-67
