@@ -844,13 +844,14 @@ extern "C" int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *cons
         int engine;
         uint32_t block;
     };
-    const Cand cands[5] = {{GKRX_BATCH_PER_PROOF, 256}, {GKRX_BATCH_LOCKSTEP, 64}, {GKRX_BATCH_LOCKSTEP, 128},
-                           {GKRX_BATCH_LOCKSTEP, 256}, {GKRX_BATCH_LOCKSTEP, 512}};
+    constexpr int kCands = 5;
+    const Cand cands[kCands] = {{GKRX_BATCH_PER_PROOF, 256}, {GKRX_BATCH_LOCKSTEP, 64}, {GKRX_BATCH_LOCKSTEP, 128},
+                                {GKRX_BATCH_LOCKSTEP, 256}, {GKRX_BATCH_LOCKSTEP, 512}};
     const int saved_engine = ctx->batch.engine;
     const uint32_t saved_block = ctx->batch.block;
     double best = 1e300;
     Cand pick{saved_engine, saved_block};
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < kCands; k++) {
         ctx->batch.engine = cands[k].engine;
         ctx->batch.block = cands[k].block;
         double ms = 0;

@@ -2557,10 +2557,17 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             ls_ev[0] = bs.ev_main;
             ls_ev[1] = bs.ev_aux2;
             ls_ev[2] = bs.ev_aux;
-            GKRX_HIP_CHECK(hipEventRecord(ls_ev[0], st));  // the uploads above
-            rec[0] = 1;
         }
-        uint32_t si = 0;  // step index
+        // the lanes' fork point: every lane's first wait is for the main stream's work so far
+        auto fork = [&]() -> int {
+            if (!multi) return GKRX_OK;
+            for (int x = 0; x < 3; x++) {
+                rec[x] = x == 0;
+                for (int o = 0; o < 3; o++) seen[x][o] = 0;
+            }
+            GKRX_HIP_CHECK(hipEventRecord(ls_ev[0], st));
+            return GKRX_OK;
+        };
         auto join = [&](int x) -> int {  // lane x waits for the other lanes' latest records
             if (!multi) return GKRX_OK;
             for (int o = 0; o < 3; o++)
@@ -2580,90 +2587,96 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                                        dstarts + ls.start, ls.n, R, hash_kind);
             });
         };
-        for (const LsStep &S : steps) {
-            const uint32_t s = si++;
-            bool launched[3] = {false, false, false};
-            // chains
-            if (S.l[kLsChain].n) {
-                const LsList &ls = S.l[kLsChain];
-                hipStream_t cs = ls_st[2];
-                if (int rc = join(2)) return rc;
-                timed(pf, kKLsChain, cs, [&] {
-                    if (L == 8)
-                        hipLaunchKernelGGL(k_ls_chain<8>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
-                                           dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
-                    else
-                        hipLaunchKernelGGL(k_ls_chain<16>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
-                                           dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
-                });
-                launched[2] = true;
-            }
-            // round pairs and their transcript ends
-            if (S.l[kLsPairs].n) {
-                const LsList &ls = S.l[kLsPairs];
-                hipStream_t ps = ls_st[1];
-                if (int rc = join(1)) return rc;
-                timed(pf, kKLsPair, ps, [&] {
-                    const dim3 g(ls.blocks);
-                    const LsTask *tk = dtasks + ls.task;
-                    const uint32_t *sk = dstarts + ls.start;
-                    if (L == 8) {
-                        switch (BS) {
-                        case 64: hipLaunchKernelGGL((k_ls_pair<8, 64>), g, dim3(64), 0, ps, tk, sk, ls.n); break;
-                        case 128: hipLaunchKernelGGL((k_ls_pair<8, 128>), g, dim3(128), 0, ps, tk, sk, ls.n); break;
-                        case 512: hipLaunchKernelGGL((k_ls_pair<8, 512>), g, dim3(512), 0, ps, tk, sk, ls.n); break;
-                        default: hipLaunchKernelGGL((k_ls_pair<8, 256>), g, dim3(256), 0, ps, tk, sk, ls.n); break;
-                        }
-                    } else {
-                        switch (BS) {
-                        case 64: hipLaunchKernelGGL((k_ls_pair<16, 64>), g, dim3(64), 0, ps, tk, sk, ls.n); break;
-                        case 128: hipLaunchKernelGGL((k_ls_pair<16, 128>), g, dim3(128), 0, ps, tk, sk, ls.n); break;
-                        case 512: hipLaunchKernelGGL((k_ls_pair<16, 512>), g, dim3(512), 0, ps, tk, sk, ls.n); break;
-                        default: hipLaunchKernelGGL((k_ls_pair<16, 256>), g, dim3(256), 0, ps, tk, sk, ls.n); break;
-                        }
-                    }
-                });
-                if (S.l[kLsPairFin].n) fin(S.l[kLsPairFin], ps, kKPairFin);
-                launched[1] = true;
-            }
-            // wide work and claims
-            const LsList &W = S.l[kLsWork];
-            if (W.n || S.l[kLsDotFin].n) {
-                // the seeds of this step's SEED tasks (the other lanes are enqueued already)
-                size_t q = seeded;
-                while (q < nj && bs.meta_seed_step[sorder[q]] <= s) q++;
-                if (int rc = upload(q)) return rc;
-                if (int rc = join(0)) return rc;
-                auto work = [&](uint32_t blocks, uint32_t base) {
-                    if (L == 8) ls_launch_work<8>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
-                    else ls_launch_work<16>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
-                };
-                if (W.n && !pf) {
-                    work(W.blocks, 0);
-                } else if (W.n) {
-                    // profiling: one launch per run of tasks of one op class
-                    const LsTask *ht = reinterpret_cast<const LsTask *>(bs.host_meta + bs.o_tasks) + W.task;
-                    const uint32_t *hs = reinterpret_cast<const uint32_t *>(bs.host_meta + bs.o_starts) + W.start;
-                    for (uint32_t a = 0; a < W.n;) {
-                        uint32_t b = a + 1;
-                        while (b < W.n && ls_op_class(ht[b].op) == ls_op_class(ht[a].op)) b++;
-                        timed(pf, ls_op_class(ht[a].op), st, [&] { work(hs[b] - hs[a], hs[a]); });
-                        a = b;
-                    }
+        // enqueues steps [s0, s1), each step's seeds uploaded just before it
+        auto run_steps = [&](size_t s0, size_t s1) -> int {
+            for (size_t s = s0; s < s1; s++) {
+                const LsStep &S = steps[s];
+                bool launched[3] = {false, false, false};
+                // chains
+                if (S.l[kLsChain].n) {
+                    const LsList &ls = S.l[kLsChain];
+                    hipStream_t cs = ls_st[2];
+                    if (int rc = join(2)) return rc;
+                    timed(pf, kKLsChain, cs, [&] {
+                        if (L == 8)
+                            hipLaunchKernelGGL(k_ls_chain<8>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
+                                               dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
+                        else
+                            hipLaunchKernelGGL(k_ls_chain<16>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
+                                               dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
+                    });
+                    launched[2] = true;
                 }
-                if (S.l[kLsDotFin].n) fin(S.l[kLsDotFin], st, kKLsFin);
-                launched[0] = true;
-            }
-            if (multi)
-                for (int x = 0; x < 3; x++)
-                    if (launched[x]) {
-                        GKRX_HIP_CHECK(hipEventRecord(ls_ev[x], ls_st[x]));
-                        rec[x]++;
+                // round pairs and their transcript ends
+                if (S.l[kLsPairs].n) {
+                    const LsList &ls = S.l[kLsPairs];
+                    hipStream_t ps = ls_st[1];
+                    if (int rc = join(1)) return rc;
+                    timed(pf, kKLsPair, ps, [&] {
+                        const dim3 g(ls.blocks);
+                        const LsTask *tk = dtasks + ls.task;
+                        const uint32_t *sk = dstarts + ls.start;
+                        if (L == 8) {
+                            switch (BS) {
+                            case 64: hipLaunchKernelGGL((k_ls_pair<8, 64>), g, dim3(64), 0, ps, tk, sk, ls.n); break;
+                            case 128: hipLaunchKernelGGL((k_ls_pair<8, 128>), g, dim3(128), 0, ps, tk, sk, ls.n); break;
+                            case 512: hipLaunchKernelGGL((k_ls_pair<8, 512>), g, dim3(512), 0, ps, tk, sk, ls.n); break;
+                            default: hipLaunchKernelGGL((k_ls_pair<8, 256>), g, dim3(256), 0, ps, tk, sk, ls.n); break;
+                            }
+                        } else {
+                            switch (BS) {
+                            case 64: hipLaunchKernelGGL((k_ls_pair<16, 64>), g, dim3(64), 0, ps, tk, sk, ls.n); break;
+                            case 128: hipLaunchKernelGGL((k_ls_pair<16, 128>), g, dim3(128), 0, ps, tk, sk, ls.n); break;
+                            case 512: hipLaunchKernelGGL((k_ls_pair<16, 512>), g, dim3(512), 0, ps, tk, sk, ls.n); break;
+                            default: hipLaunchKernelGGL((k_ls_pair<16, 256>), g, dim3(256), 0, ps, tk, sk, ls.n); break;
+                            }
+                        }
+                    });
+                    if (S.l[kLsPairFin].n) fin(S.l[kLsPairFin], ps, kKPairFin);
+                    launched[1] = true;
+                }
+                // wide work and claims
+                const LsList &W = S.l[kLsWork];
+                if (W.n || S.l[kLsDotFin].n) {
+                    // the seeds of this step's SEED tasks (the other lanes are enqueued already)
+                    size_t q = seeded;
+                    while (q < nj && bs.meta_seed_step[sorder[q]] <= s) q++;
+                    if (int rc = upload(q)) return rc;
+                    if (int rc = join(0)) return rc;
+                    auto work = [&](uint32_t blocks, uint32_t base) {
+                        if (L == 8) ls_launch_work<8>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
+                        else ls_launch_work<16>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
+                    };
+                    if (W.n && !pf) {
+                        work(W.blocks, 0);
+                    } else if (W.n) {
+                        // profiling: one launch per run of tasks of one op class
+                        const LsTask *ht = reinterpret_cast<const LsTask *>(bs.host_meta + bs.o_tasks) + W.task;
+                        const uint32_t *hs = reinterpret_cast<const uint32_t *>(bs.host_meta + bs.o_starts) + W.start;
+                        for (uint32_t a = 0; a < W.n;) {
+                            uint32_t b = a + 1;
+                            while (b < W.n && ls_op_class(ht[b].op) == ls_op_class(ht[a].op)) b++;
+                            timed(pf, ls_op_class(ht[a].op), st, [&] { work(hs[b] - hs[a], hs[a]); });
+                            a = b;
+                        }
                     }
-        }
-        if (seeded != nj) return GKRX_ERR_STATE;  // every proof has a SEED task
+                    if (S.l[kLsDotFin].n) fin(S.l[kLsDotFin], st, kKLsFin);
+                    launched[0] = true;
+                }
+                if (multi)
+                    for (int x = 0; x < 3; x++)
+                        if (launched[x]) {
+                            GKRX_HIP_CHECK(hipEventRecord(ls_ev[x], ls_st[x]));
+                            rec[x]++;
+                        }
+            }
+            return GKRX_OK;
+        };
+        if (int rc = fork()) return rc;
+        if (int rc = run_steps(0, steps.size())) return rc;
         // the copies below (and the next batch, which synchronizes st) follow every lane
         if (int rc = join(0)) return rc;
+        if (seeded != nj) return GKRX_ERR_STATE;  // every proof has a SEED task
     }
     GKRX_HIP_CHECK(hipGetLastError());
     GKRX_HIP_CHECK(hipMemcpyAsync(bs.tails_host, tails, tail_off[nj] * 4, hipMemcpyDeviceToHost, st));

@@ -115,8 +115,9 @@ int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, u
 /* Batch engines (no reference counterpart).  PER_PROOF: one launch, one workgroup runs a
  * whole proof.  LOCKSTEP (the default): the batch advances one op of every proof per
  * step; a step is one launch of the wide ops (each proof's op on as many workgroups of
- * `block` threads as its tables need) and one launch of the transcript steps.  block is
- * 64, 128, 256 or 512.  Results do not depend on the choice. */
+ * `block` threads as its tables need), one of the round pairs, one of the serial chains
+ * and the launches of the transcript ends.  block is 64, 128, 256 or 512.  Results do
+ * not depend on the choice. */
 #define GKRX_BATCH_PER_PROOF 1
 #define GKRX_BATCH_LOCKSTEP 2
 int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block);

@@ -143,6 +143,37 @@ def test_autotune_picks_an_engine_and_keeps_results(ctx, mixed_plain):
         assert np.array_equal(claimed, c1) and bytes(proof) == bytes(p1)
 
 
+GRAPH_ENGINE = 3  # the lockstep launches replayed from HIP graphs (engine taken out)
+
+
+@pytest.mark.skip(reason="HIP-graph replay of the lockstep batch is taken out: its first batch (three "
+                         "streams joined by events, captured per segment of steps) ended the process with a "
+                         "host segmentation fault inside the library call, cause not found")
+def test_graph_replays_follow_new_witnesses(ctx):
+    """The graph engine captures a batch's launches once and replays them: every replay,
+    with plain proofs (eager launches on the same stream) and new witnesses between the
+    replays, must equal the plain proofs."""
+    print("[gpu] batch graph replays", flush=True)
+    specs = _mixed(11, 6, 3, 14, 1, 5)
+    circs = _circuits(ctx, specs)
+    prover = gkrx.Prover(ctx)
+    saved = ctx.batch_engine()
+    ctx.set_batch_engine(GRAPH_ENGINE, 128)
+    try:
+        for rep in range(3):
+            batch = prover.prove_batch(circs)
+            for (cl, pr), c in zip(batch, circs):
+                c.evaluate()
+                c1, p1 = prover.prove(c)
+                assert np.array_equal(cl, c1) and bytes(pr) == bytes(p1), f"replay {rep}"
+            for (s, d, k), c in zip(specs, circs):
+                c.set_witness(gkrx.synth_witness(s + 100 * (rep + 1), 1 << k, 16))
+    finally:
+        ctx.set_batch_engine(*saved)
+    for c in circs:
+        c.close()
+
+
 def test_engine_arguments_checked(ctx):
     with pytest.raises(gkrx.GkrxError):
         ctx.set_batch_engine(gkrx.BATCH_LOCKSTEP, 96)
