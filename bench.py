@@ -129,10 +129,10 @@ def batch_leg(gkrx, fs, count, steps, warmup):
     prover = gkrx.Prover(ctx)
     outs = [np.empty(prover.proof_size(c), dtype=np.uint8) for c in circs]
     # autotuner: every batch engine and block size on this batch's device work
-    engine, block, tune_ms = ctx.batch_autotune(circs)
+    engine, block, lanes, tune_ms = ctx.batch_autotune(circs)
     names = {gkrx.BATCH_PER_PROOF: "per_proof", gkrx.BATCH_LOCKSTEP: "lockstep"}
-    tuned = {f"{names[e]}_{b}": round(v, 3) for (e, b), v in tune_ms.items()}
-    log(f"autotune: {tuned} -> {names[engine]} block {block}")
+    tuned = {f"{names[e]}_b{b}_v{v}": round(ms, 3) for (e, b, v), ms in tune_ms.items()}
+    log(f"autotune: {tuned} -> {names[engine]} block {block} lanes {lanes}")
     for _ in range(warmup):
         prover.prove_batch(circs, outs)
     t0 = time.perf_counter()
@@ -163,7 +163,7 @@ def batch_leg(gkrx, fs, count, steps, warmup):
            "ratio": round(plain_s / batched_s, 3),
            "plain_latency_ms": {"p50": round(percentile(plain_lat, 50), 3), "p99": round(percentile(plain_lat, 99), 3)},
            "identical_to_plain": same, "verified_sample": ok,
-           "engine": names[engine], "block": block, "autotune_device_ms": tuned,
+           "engine": names[engine], "block": block, "lanes_per_thread": lanes, "autotune_device_ms": tuned,
            "batched_split_ms": {k: round(v, 3) for k, v in split.items()}}
     # the context and circuits stay resident for the serving leg (the caller closes ctx)
     return out, (ctx, circs, [g for _, g in made], batched_s / count)

@@ -809,8 +809,27 @@ extern "C" int gkrx_ctx_get_batch_engine(const gkrx_ctx *ctx, int *engine, uint3
     return GKRX_OK;
 }
 
+extern "C" int gkrx_ctx_set_batch_lanes(gkrx_ctx *ctx, uint32_t lanes)
+{
+    if (!ctx) return GKRX_ERR_ARG;
+    if (lanes != 1 && lanes != 2 && lanes != 4) {
+        set_error("lanes per thread must be 1, 2 or 4");
+        return GKRX_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->batch.lanes = lanes;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_ctx_get_batch_lanes(const gkrx_ctx *ctx, uint32_t *lanes)
+{
+    if (!ctx || !lanes) return GKRX_ERR_ARG;
+    *lanes = ctx->batch.lanes;
+    return GKRX_OK;
+}
+
 extern "C" int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, int *engine,
-                                   uint32_t *block, double *ms_each)
+                                   uint32_t *block, uint32_t *lanes, double *ms_each)
 {
     if (!ctx || n == 0 || !circuits) return GKRX_ERR_ARG;
     if (ctx->bn254) {
@@ -842,18 +861,20 @@ extern "C" int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *cons
     }
     struct Cand {
         int engine;
-        uint32_t block;
+        uint32_t block, lanes;
     };
-    constexpr int kCands = 5;
-    const Cand cands[kCands] = {{GKRX_BATCH_PER_PROOF, 256}, {GKRX_BATCH_LOCKSTEP, 64}, {GKRX_BATCH_LOCKSTEP, 128},
-                                {GKRX_BATCH_LOCKSTEP, 256}, {GKRX_BATCH_LOCKSTEP, 512}};
+    // the one-workgroup-per-proof engine, then the lockstep engine over block size x lanes
+    std::vector<Cand> cands = {{GKRX_BATCH_PER_PROOF, 256, 4}};
+    for (uint32_t b : {64u, 128u, 256u, 512u})
+        for (uint32_t v : {1u, 2u, 4u}) cands.push_back({GKRX_BATCH_LOCKSTEP, b, v});
     const int saved_engine = ctx->batch.engine;
-    const uint32_t saved_block = ctx->batch.block;
+    const uint32_t saved_block = ctx->batch.block, saved_lanes = ctx->batch.lanes;
     double best = 1e300;
-    Cand pick{saved_engine, saved_block};
-    for (int k = 0; k < kCands; k++) {
+    Cand pick{saved_engine, saved_block, saved_lanes};
+    for (size_t k = 0; k < cands.size(); k++) {
         ctx->batch.engine = cands[k].engine;
         ctx->batch.block = cands[k].block;
+        ctx->batch.lanes = cands[k].lanes;
         double ms = 0;
         for (int rep = 0; rep < 2; rep++) {  // the first run allocates and loads code
             const auto t0 = std::chrono::steady_clock::now();
@@ -862,6 +883,7 @@ extern "C" int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *cons
             if (rc) {
                 ctx->batch.engine = saved_engine;
                 ctx->batch.block = saved_block;
+                ctx->batch.lanes = saved_lanes;
                 return rc;
             }
             ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -874,8 +896,10 @@ extern "C" int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *cons
     }
     ctx->batch.engine = pick.engine;
     ctx->batch.block = pick.block;
+    ctx->batch.lanes = pick.lanes;
     if (engine) *engine = pick.engine;
     if (block) *block = pick.block;
+    if (lanes) *lanes = pick.lanes;
     return GKRX_OK;
 }
 

@@ -95,6 +95,7 @@ enum BatchEngine : int { kBatchPerProof = 1, kBatchLockstep = 2 };
 struct BatchScratch {
     int engine = kBatchLockstep;
     uint32_t block = 256;           // workgroup size of the lockstep work kernel: 64, 128, 256 or 512
+    uint32_t lanes = 4;             // lockstep round pairs: lanes of an entry per thread (1, 2 or 4)
     // lockstep: run the serial chains and the round pairs on streams of their own, beside the
     // other wide ops (false: everything on the caller's stream)
     bool chain_stream = true;

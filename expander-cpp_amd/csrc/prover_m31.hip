@@ -69,6 +69,59 @@ __device__ __forceinline__ uint4 fold4(uint4 a, uint4 b, uint32_t r)
     return make_uint4(fold1(a.x, b.x, r), fold1(a.y, b.y, r), fold1(a.z, b.z, r), fold1(a.w, b.w, r));
 }
 
+// V consecutive lanes of one table entry (V = 1, 2 or 4 lanes per thread: one 4-, 8- or
+// 16-byte access).
+template <int V>
+struct Lanes {
+    uint32_t v[V];
+};
+template <int V>
+__device__ __forceinline__ Lanes<V> ldv(const uint32_t *p)
+{
+    Lanes<V> r;
+    if constexpr (V == 4) {
+        const uint4 x = ld4(p);
+        r.v[0] = x.x; r.v[1] = x.y; r.v[2] = x.z; r.v[3] = x.w;
+    } else if constexpr (V == 2) {
+        const uint2 x = *reinterpret_cast<const uint2 *>(p);
+        r.v[0] = x.x; r.v[1] = x.y;
+    } else {
+        r.v[0] = *p;
+    }
+    return r;
+}
+template <int V>
+__device__ __forceinline__ void stv(uint32_t *p, const Lanes<V> &a)
+{
+    if constexpr (V == 4) st4(p, make_uint4(a.v[0], a.v[1], a.v[2], a.v[3]));
+    else if constexpr (V == 2) *reinterpret_cast<uint2 *>(p) = make_uint2(a.v[0], a.v[1]);
+    else *p = a.v[0];
+}
+template <int V>
+__device__ __forceinline__ Lanes<V> addv(const Lanes<V> &a, const Lanes<V> &b)
+{
+    Lanes<V> r;
+#pragma unroll
+    for (int c = 0; c < V; c++) r.v[c] = m31_add(a.v[c], b.v[c]);
+    return r;
+}
+template <int V>
+__device__ __forceinline__ Lanes<V> subv(const Lanes<V> &a, const Lanes<V> &b)
+{
+    Lanes<V> r;
+#pragma unroll
+    for (int c = 0; c < V; c++) r.v[c] = m31_sub(a.v[c], b.v[c]);
+    return r;
+}
+template <int V>
+__device__ __forceinline__ Lanes<V> foldv(const Lanes<V> &a, const Lanes<V> &b, uint32_t rr)
+{
+    Lanes<V> r;
+#pragma unroll
+    for (int c = 0; c < V; c++) r.v[c] = fold1(a.v[c], b.v[c], rr);
+    return r;
+}
+
 // ------------------------------------------------------------------ forward pass
 // Work item w: output o = w / LG, lanes 4g .. 4g+3 (g = w % LG).
 template <int L>
@@ -428,66 +481,62 @@ constexpr uint32_t kPairVals = 9;
 
 __device__ __forceinline__ void pair_add(uint32_t &acc, uint32_t a, uint32_t b) { acc = m31_add(acc, m31_mul(a, b)); }
 
-template <int L, bool PH2, int MODE>
+// g indexes the thread's V lanes [V g, V g + V) of an entry (L / V threads per entry).
+template <int L, bool PH2, int MODE, int V = 4>
 __device__ __forceinline__ void pair_item(const uint32_t *__restrict__ fs, uint32_t *__restrict__ fd,
                                           const uint32_t *__restrict__ hs, uint32_t *__restrict__ hd, uint64_t i,
-                                          uint32_t g, uint32_t ra, uint32_t rb, uint32_t (&acc)[kPairVals][4])
+                                          uint32_t g, uint32_t ra, uint32_t rb, uint32_t (&acc)[kPairVals][V])
 {
-    uint4 f[4];
-    uint4 h[4];       // phase 1
+    Lanes<V> f[4];
+    Lanes<V> h[4];    // phase 1
     uint32_t sv[4];   // phase 2: the scalar h of each row
     if (MODE == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) f[r] = ld4(fs + (4 * i + r) * L + 4 * g);
+        for (int r = 0; r < 4; r++) f[r] = ldv<V>(fs + (4 * i + r) * L + V * g);
         if (PH2) {
             const uint4 b = *reinterpret_cast<const uint4 *>(hs + 4 * i);
             sv[0] = b.x; sv[1] = b.y; sv[2] = b.z; sv[3] = b.w;
         } else {
 #pragma unroll
-            for (int r = 0; r < 4; r++) h[r] = ld4(hs + (4 * i + r) * L + 4 * g);
+            for (int r = 0; r < 4; r++) h[r] = ldv<V>(hs + (4 * i + r) * L + V * g);
         }
     } else {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const uint32_t *src = fs + (16 * i + 4 * r) * L + 4 * g;
-            f[r] = fold4(fold4(ld4(src), ld4(src + L), ra), fold4(ld4(src + 2 * L), ld4(src + 3 * L), ra), rb);
-            st4(fd + (4 * i + r) * L + 4 * g, f[r]);
+            const uint32_t *src = fs + (16 * i + 4 * r) * L + V * g;
+            f[r] = foldv(foldv(ldv<V>(src), ldv<V>(src + L), ra), foldv(ldv<V>(src + 2 * L), ldv<V>(src + 3 * L), ra),
+                         rb);
+            stv(fd + (4 * i + r) * L + V * g, f[r]);
             if (PH2) {
                 const uint4 b = *reinterpret_cast<const uint4 *>(hs + 16 * i + 4 * r);
                 sv[r] = fold1(fold1(b.x, b.y, ra), fold1(b.z, b.w, ra), rb);
             } else {
-                const uint32_t *hsrc = hs + (16 * i + 4 * r) * L + 4 * g;
-                h[r] = fold4(fold4(ld4(hsrc), ld4(hsrc + L), ra), fold4(ld4(hsrc + 2 * L), ld4(hsrc + 3 * L), ra),
-                             rb);
-                st4(hd + (4 * i + r) * L + 4 * g, h[r]);
+                const uint32_t *hsrc = hs + (16 * i + 4 * r) * L + V * g;
+                h[r] = foldv(foldv(ldv<V>(hsrc), ldv<V>(hsrc + L), ra),
+                             foldv(ldv<V>(hsrc + 2 * L), ldv<V>(hsrc + 3 * L), ra), rb);
+                stv(hd + (4 * i + r) * L + V * g, h[r]);
             }
         }
         if (PH2 && g == 0) *reinterpret_cast<uint4 *>(hd + 4 * i) = make_uint4(sv[0], sv[1], sv[2], sv[3]);
     }
     // the nine factors of f (lanes) and of h (lanes, or one scalar in phase 2)
-    const uint4 f02 = add4(f[0], f[2]), f13 = add4(f[1], f[3]);
-    const uint4 ff[9] = {f[0], f[1], sub4(f[1], f[0]), f[2], f[3], sub4(f[3], f[2]), f02, f13, sub4(f13, f02)};
+    const Lanes<V> f02 = addv(f[0], f[2]), f13 = addv(f[1], f[3]);
+    const Lanes<V> ff[9] = {f[0], f[1], subv(f[1], f[0]), f[2], f[3], subv(f[3], f[2]), f02, f13, subv(f13, f02)};
     if (PH2) {
         const uint32_t s02 = m31_add(sv[0], sv[2]), s13 = m31_add(sv[1], sv[3]);
         const uint32_t hh[9] = {sv[0], sv[1], m31_sub(sv[1], sv[0]), sv[2], sv[3], m31_sub(sv[3], sv[2]),
                                 s02, s13, m31_sub(s13, s02)};
 #pragma unroll
-        for (int q = 0; q < 9; q++) {
-            pair_add(acc[q][0], ff[q].x, hh[q]);
-            pair_add(acc[q][1], ff[q].y, hh[q]);
-            pair_add(acc[q][2], ff[q].z, hh[q]);
-            pair_add(acc[q][3], ff[q].w, hh[q]);
-        }
-    } else {
-        const uint4 h02 = add4(h[0], h[2]), h13 = add4(h[1], h[3]);
-        const uint4 hh[9] = {h[0], h[1], sub4(h[1], h[0]), h[2], h[3], sub4(h[3], h[2]), h02, h13, sub4(h13, h02)};
+        for (int q = 0; q < 9; q++)
 #pragma unroll
-        for (int q = 0; q < 9; q++) {
-            pair_add(acc[q][0], ff[q].x, hh[q].x);
-            pair_add(acc[q][1], ff[q].y, hh[q].y);
-            pair_add(acc[q][2], ff[q].z, hh[q].z);
-            pair_add(acc[q][3], ff[q].w, hh[q].w);
-        }
+            for (int c = 0; c < V; c++) pair_add(acc[q][c], ff[q].v[c], hh[q]);
+    } else {
+        const Lanes<V> h02 = addv(h[0], h[2]), h13 = addv(h[1], h[3]);
+        const Lanes<V> hh[9] = {h[0], h[1], subv(h[1], h[0]), h[2], h[3], subv(h[3], h[2]), h02, h13, subv(h13, h02)};
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+#pragma unroll
+            for (int c = 0; c < V; c++) pair_add(acc[q][c], ff[q].v[c], hh[q].v[c]);
     }
 }
 
@@ -685,26 +734,27 @@ __device__ __forceinline__ uint2 pair_block(uint32_t b, uint32_t R)
     return make_uint2(idx % R, (idx / R) * 8 + (b & 7));
 }
 
-// Adds the NV x 4 canonical lane sums of a block of BS threads (lanes sharing t % LG) into
-// out[v * ostride], v = q L + lane < NV L.  Every thread must call it.
-template <int L, int BS, int NV>
-__device__ __forceinline__ void block_reduce_vals(uint32_t (&acc)[NV][4], uint32_t *out, uint32_t ostride)
+// Adds the NV x V canonical lane sums of a block of BS threads (V lanes per thread; the
+// threads sharing t % (L / V) hold the same lanes) into out[v * ostride], v = q L + lane <
+// NV L.  Every thread must call it.
+template <int L, int BS, int NV, int V = 4>
+__device__ __forceinline__ void block_reduce_vals(uint32_t (&acc)[NV][V], uint32_t *out, uint32_t ostride)
 {
-    constexpr uint32_t LG = L / 4, W = BS / 64;
+    constexpr uint32_t LG = L / V, W = BS / 64;
     __shared__ uint32_t red[W][NV * L];
 #pragma unroll
     for (uint32_t off = LG; off < 64; off <<= 1) {
 #pragma unroll
         for (int q = 0; q < NV; q++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) acc[q][c] = m31_add(acc[q][c], __shfl_xor(acc[q][c], off, 64));
+            for (int c = 0; c < V; c++) acc[q][c] = m31_add(acc[q][c], __shfl_xor(acc[q][c], off, 64));
     }
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane < LG) {
 #pragma unroll
         for (int q = 0; q < NV; q++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) red[wave][q * L + 4 * lane + c] = acc[q][c];
+            for (int c = 0; c < V; c++) red[wave][q * L + V * lane + c] = acc[q][c];
     }
     __syncthreads();
     for (uint32_t v = threadIdx.x; v < NV * L; v += BS) {
@@ -1330,11 +1380,12 @@ __device__ __forceinline__ void block_reduce_to(uint64_t (&acc)[3][4], uint32_t 
     }
 }
 
-// Workgroup lb of a round-pair task (rep = lb / nb): k_pair on a block range.
-template <int L, int BS, bool PH2, int MODE>
+// Workgroup lb of a round-pair task (rep = lb / nb): k_pair on a block range, V lanes per
+// thread.
+template <int L, int BS, bool PH2, int MODE, int V>
 __device__ void ls_pair(const RoundParams &p, uint32_t nb, uint32_t lb)
 {
-    constexpr uint32_t LG = L / 4;
+    constexpr uint32_t LG = L / V;
     const uint2 rb_ = pair_block(lb, p.R);  // pair tasks start on multiples of 8 workgroups
     const uint32_t rep = rb_.x, bx = rb_.y, g = threadIdx.x % LG;
     const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
@@ -1343,11 +1394,11 @@ __device__ void ls_pair(const RoundParams &p, uint32_t nb, uint32_t lb)
     uint32_t *hd = MODE ? p.hdst + rep * p.hdst_rs : nullptr;
     const uint32_t ra = MODE ? p.rprev[(uint64_t)rep * kChalStride] : 0u;
     const uint32_t rb = MODE ? p.rprev[(uint64_t)rep * kChalStride + 1] : 0u;
-    uint32_t acc[kPairVals][4] = {};
+    uint32_t acc[kPairVals][V] = {};
     const uint64_t total = (uint64_t)p.items * LG;
     for (uint64_t w = (uint64_t)bx * BS + threadIdx.x; w < total; w += (uint64_t)nb * BS)
-        pair_item<L, PH2, MODE>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
-    block_reduce_vals<L, BS, kPairVals>(acc, p.partials + (uint64_t)rep * kPairVals * L * nb + bx, nb);
+        pair_item<L, PH2, MODE, V>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
+    block_reduce_vals<L, BS, kPairVals, V>(acc, p.partials + (uint64_t)rep * kPairVals * L * nb + bx, nb);
 }
 
 template <int L, int BS>
@@ -1426,19 +1477,44 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
 }
 
 // The round pairs of a step (their own kernel and stream: a pair item holds many more
-// registers than the other ops, which would lower their occupancy).
-template <int L, int BS>
+// registers than the other ops, which would lower their occupancy), V lanes per thread.
+template <int L, int BS, int V>
 __global__ void __launch_bounds__(BS) k_ls_pair(const LsTask *__restrict__ tasks, const uint32_t *__restrict__ start,
                                                 uint32_t ntasks)
 {
     const uint2 f = ls_find(start, ntasks, blockIdx.x);
     const LsTask &T = tasks[f.x];
     if (T.phase == 1) {
-        if (T.k == 0) ls_pair<L, BS, false, 0>(T.rp, T.nb, f.y);
-        else ls_pair<L, BS, false, 2>(T.rp, T.nb, f.y);
+        if (T.k == 0) ls_pair<L, BS, false, 0, V>(T.rp, T.nb, f.y);
+        else ls_pair<L, BS, false, 2, V>(T.rp, T.nb, f.y);
     } else {
-        if (T.k == 0) ls_pair<L, BS, true, 0>(T.rp, T.nb, f.y);
-        else ls_pair<L, BS, true, 2>(T.rp, T.nb, f.y);
+        if (T.k == 0) ls_pair<L, BS, true, 0, V>(T.rp, T.nb, f.y);
+        else ls_pair<L, BS, true, 2, V>(T.rp, T.nb, f.y);
+    }
+}
+
+template <int L, int BS>
+void ls_launch_pair_v(uint32_t V, uint32_t blocks, hipStream_t st, const LsTask *tasks, const uint32_t *start,
+                      uint32_t n)
+{
+    const dim3 g(blocks);
+    switch (V) {
+    case 1: hipLaunchKernelGGL((k_ls_pair<L, BS, 1>), g, dim3(BS), 0, st, tasks, start, n); break;
+    case 2: hipLaunchKernelGGL((k_ls_pair<L, BS, 2>), g, dim3(BS), 0, st, tasks, start, n); break;
+    default: hipLaunchKernelGGL((k_ls_pair<L, BS, 4>), g, dim3(BS), 0, st, tasks, start, n); break;
+    }
+}
+
+// k_ls_pair for block size BS in {64, 128, 256, 512} and V in {1, 2, 4} lanes per thread
+template <int L>
+void ls_launch_pair(uint32_t BS, uint32_t V, uint32_t blocks, hipStream_t st, const LsTask *tasks,
+                    const uint32_t *start, uint32_t n)
+{
+    switch (BS) {
+    case 64: ls_launch_pair_v<L, 64>(V, blocks, st, tasks, start, n); break;
+    case 128: ls_launch_pair_v<L, 128>(V, blocks, st, tasks, start, n); break;
+    case 512: ls_launch_pair_v<L, 512>(V, blocks, st, tasks, start, n); break;
+    default: ls_launch_pair_v<L, 256>(V, blocks, st, tasks, start, n); break;
     }
 }
 
@@ -2052,8 +2128,8 @@ struct LsEntry {
 // pass; the rest (the sumcheck) needs the transcript seed (kOpSeed, placed by the caller)
 // one step before it.  J holds the device addresses of the proof's tables; ts_len / ts_hs
 // is the transcript position after the host prefix.
-void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t R, uint32_t BS, uint64_t ts_len,
-                uint64_t ts_hs, std::vector<LsEntry> &ops)
+void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t R, uint32_t BS, uint32_t LPT,
+                uint64_t ts_len, uint64_t ts_hs, std::vector<LsEntry> &ops)
 {
     const uint32_t L = dc.L, LG = L / 4, Q = 3 * L, D = dc.D;
     auto push_w = [&](const LsTask &t) {
@@ -2156,7 +2232,7 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
                 t.phase = phase;
                 t.k = k;
                 t.rp = rp;
-                t.nb = (ls_grid((uint64_t)rp.items * LG, BS) + 7) & ~7u;  // a multiple of 8 (pair_block)
+                t.nb = (ls_grid((uint64_t)rp.items * (L / LPT), BS) + 7) & ~7u;  // a multiple of 8 (pair_block)
                 e.w[0] = t;
                 e.nw = 1;
                 LsTask fz = ls_task(kOpPairFin, job, l);
@@ -2250,6 +2326,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     const uint32_t L = items[0].dc->L;
     const bool lockstep = bs.engine != kBatchPerProof && nj <= kLsMaxJobs;
     const uint32_t BS = (bs.block == 64 || bs.block == 128 || bs.block == 512) ? bs.block : 256u;
+    const uint32_t LPT = (bs.lanes == 1 || bs.lanes == 2) ? bs.lanes : 4u;  // round-pair lanes per thread
     // sizes of every job's scratch, one pool
     std::vector<uint64_t> job_off(nj + 1, 0), tail_off(nj + 1, 0);
     const uint64_t seed_u32 = (uint64_t)R * kChalStride + 10 + 8;
@@ -2291,7 +2368,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     // The job table and the schedule depend only on the circuits' shapes, their device
     // addresses and the pool: a batch with the same key reuses the metadata already on the
     // device.
-    std::vector<uint64_t> key = {(uint64_t)lockstep, BS, L, R, nj, (uint64_t)(uintptr_t)bs.pool};
+    std::vector<uint64_t> key = {(uint64_t)lockstep, BS, LPT, L, R, nj, (uint64_t)(uintptr_t)bs.pool};
     for (size_t i = 0; i < nj; i++) {
         const DevCircuit &dc = *items[i].dc;
         key.insert(key.end(), {(uint64_t)(uintptr_t)dc.values, (uint64_t)(uintptr_t)dc.csr_pool,
@@ -2364,7 +2441,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             for (size_t i = 0; i < nj; i++) {
                 const uint32_t D = items[i].dc->D;
                 ops[i].reserve((size_t)D * (8 + 2 * items[i].dc->max_n));
-                ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, 32, ts_hashed(i), ops[i]);
+                ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, LPT, 32, ts_hashed(i), ops[i]);
                 dmax = std::max<size_t>(dmax, D);
                 smax = std::max(smax, ops[i].size() - D);
                 ntask += 2 * ops[i].size() + 1;
@@ -2613,24 +2690,8 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                     hipStream_t ps = ls_st[1];
                     if (int rc = join(1)) return rc;
                     timed(pf, kKLsPair, ps, [&] {
-                        const dim3 g(ls.blocks);
-                        const LsTask *tk = dtasks + ls.task;
-                        const uint32_t *sk = dstarts + ls.start;
-                        if (L == 8) {
-                            switch (BS) {
-                            case 64: hipLaunchKernelGGL((k_ls_pair<8, 64>), g, dim3(64), 0, ps, tk, sk, ls.n); break;
-                            case 128: hipLaunchKernelGGL((k_ls_pair<8, 128>), g, dim3(128), 0, ps, tk, sk, ls.n); break;
-                            case 512: hipLaunchKernelGGL((k_ls_pair<8, 512>), g, dim3(512), 0, ps, tk, sk, ls.n); break;
-                            default: hipLaunchKernelGGL((k_ls_pair<8, 256>), g, dim3(256), 0, ps, tk, sk, ls.n); break;
-                            }
-                        } else {
-                            switch (BS) {
-                            case 64: hipLaunchKernelGGL((k_ls_pair<16, 64>), g, dim3(64), 0, ps, tk, sk, ls.n); break;
-                            case 128: hipLaunchKernelGGL((k_ls_pair<16, 128>), g, dim3(128), 0, ps, tk, sk, ls.n); break;
-                            case 512: hipLaunchKernelGGL((k_ls_pair<16, 512>), g, dim3(512), 0, ps, tk, sk, ls.n); break;
-                            default: hipLaunchKernelGGL((k_ls_pair<16, 256>), g, dim3(256), 0, ps, tk, sk, ls.n); break;
-                            }
-                        }
+                        if (L == 8) ls_launch_pair<8>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n);
+                        else ls_launch_pair<16>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n);
                     });
                     if (S.l[kLsPairFin].n) fin(S.l[kLsPairFin], ps, kKPairFin);
                     launched[1] = true;

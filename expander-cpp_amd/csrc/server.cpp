@@ -365,6 +365,7 @@ extern "C" int gkrx_server_create(gkrx_ctx *ctx, uint64_t hbm_budget_bytes, uint
         sl->single = ctx->bn254 || i == slots;
         sl->bs.engine = ctx->batch.engine;  // the context's batch engine (gkrx_ctx_set_batch_engine)
         sl->bs.block = ctx->batch.block;
+        sl->bs.lanes = ctx->batch.lanes;
         sl->bs.chain_stream = false;  // slots already overlap; stay within the hardware queues
         // an M31 single slot (circuits wider than the batch engines take) gets its stream on
         // first use, so that an idle one holds no hardware queue

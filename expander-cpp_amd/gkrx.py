@@ -89,8 +89,10 @@ def lib():
     L.gkrx_ctx_profile.argtypes = [vp, C.c_int]
     L.gkrx_ctx_set_batch_engine.argtypes = [vp, C.c_int, C.c_uint32]
     L.gkrx_ctx_get_batch_engine.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]
+    L.gkrx_ctx_set_batch_lanes.argtypes = [vp, C.c_uint32]
+    L.gkrx_ctx_get_batch_lanes.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.gkrx_batch_autotune.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_int), C.POINTER(C.c_uint32),
-                                      C.POINTER(C.c_double)]
+                                      C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
     L.gkrx_ctx_kernel_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_double)]
     _lib = L
@@ -307,15 +309,25 @@ class Context:
         return e.value, b.value
 
     def batch_autotune(self, circuits):
-        """Times every batch engine / block size on these circuits and keeps the fastest.
-        Returns (engine, block, {(engine, block): ms})."""
+        """Times every batch engine / block size / lanes per thread on these circuits and keeps
+        the fastest.  Returns (engine, block, lanes, {(engine, block, lanes): ms})."""
         n = len(circuits)
         hs = (C.c_void_p * n)(*[c._h.value for c in circuits])
-        e, b = C.c_int(), C.c_uint32()
-        ms = (C.c_double * 5)()
-        _check(lib().gkrx_batch_autotune(self._h, n, hs, C.byref(e), C.byref(b), ms), "gkrx_batch_autotune")
-        keys = [(BATCH_PER_PROOF, 256)] + [(BATCH_LOCKSTEP, x) for x in (64, 128, 256, 512)]
-        return e.value, b.value, {k: ms[i] for i, k in enumerate(keys)}
+        e, b, v = C.c_int(), C.c_uint32(), C.c_uint32()
+        ms = (C.c_double * 13)()
+        _check(lib().gkrx_batch_autotune(self._h, n, hs, C.byref(e), C.byref(b), C.byref(v), ms),
+               "gkrx_batch_autotune")
+        keys = [(BATCH_PER_PROOF, 256, 4)] + [(BATCH_LOCKSTEP, x, y) for x in (64, 128, 256, 512) for y in (1, 2, 4)]
+        return e.value, b.value, v.value, {k: ms[i] for i, k in enumerate(keys)}
+
+    def set_batch_lanes(self, lanes: int) -> None:
+        """Lanes per thread (1, 2 or 4) of the lockstep engine's round-pair kernel."""
+        _check(lib().gkrx_ctx_set_batch_lanes(self._h, lanes), "gkrx_ctx_set_batch_lanes")
+
+    def batch_lanes(self) -> int:
+        v = C.c_uint32()
+        _check(lib().gkrx_ctx_get_batch_lanes(self._h, C.byref(v)), "gkrx_ctx_get_batch_lanes")
+        return v.value
 
 
 class Circuit:

@@ -122,11 +122,16 @@ int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, u
 #define GKRX_BATCH_LOCKSTEP 2
 int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block);
 int gkrx_ctx_get_batch_engine(const gkrx_ctx *ctx, int *engine, uint32_t *block);
+/* Lanes of a table entry per thread in the lockstep engine's round-pair kernel: 1, 2 or 4
+ * (one 4-, 8- or 16-byte access per thread; L / lanes threads per entry).  Default 4. */
+int gkrx_ctx_set_batch_lanes(gkrx_ctx *ctx, uint32_t lanes);
+int gkrx_ctx_get_batch_lanes(const gkrx_ctx *ctx, uint32_t *lanes);
 /* Autotuner: times the device part of a batch of these circuits (witnesses set) under
- * every engine and block size, keeps the fastest in the context, and reports it.
- * ms_each (optional, 5 entries): PER_PROOF, then LOCKSTEP with block 64, 128, 256, 512. */
+ * every engine, block size and lanes per thread, keeps the fastest in the context, and
+ * reports it (lanes is 4 for PER_PROOF).  ms_each (optional, 13 entries): PER_PROOF, then
+ * LOCKSTEP with block 64, 128, 256, 512, each with lanes 1, 2, 4 (block-major). */
 int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, int *engine, uint32_t *block,
-                        double *ms_each);
+                        uint32_t *lanes, double *ms_each);
 /* Verifier::verify (LinearGKR.hpp:104-149), on the host.  Returns 1 = accepted,
  * 0 = rejected, negative = error. */
 int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v, const uint8_t *proof,

@@ -133,12 +133,29 @@ def test_autotune_picks_an_engine_and_keeps_results(ctx, mixed_plain):
     circs, plain = mixed_plain
     saved = ctx.batch_engine()
     try:
-        engine, block, ms = ctx.batch_autotune(circs)
-        assert (engine, block) in ENGINES and ctx.batch_engine() == (engine, block)
-        assert all(v > 0 for v in ms.values()) and ms[(engine, block)] == min(ms.values())
+        engine, block, lanes, ms = ctx.batch_autotune(circs)
+        assert (engine, block) in ENGINES and ctx.batch_engine() == (engine, block) and ctx.batch_lanes() == lanes
+        assert len(ms) == 13 and all(v > 0 for v in ms.values()) and ms[(engine, block, lanes)] == min(ms.values())
         batch = gkrx.Prover(ctx).prove_batch(circs)
     finally:
         ctx.set_batch_engine(*saved)
+        ctx.set_batch_lanes(4)
+    for (claimed, proof), (c1, p1) in zip(batch, plain):
+        assert np.array_equal(claimed, c1) and bytes(proof) == bytes(p1)
+
+
+@pytest.mark.parametrize("block,lanes", [(64, 1), (128, 2), (256, 1), (512, 2)])
+def test_lanes_per_thread_equal_plain(ctx, mixed_plain, block, lanes):
+    print(f"[gpu] lockstep block {block} lanes {lanes} vs plain proofs", flush=True)
+    circs, plain = mixed_plain
+    saved = ctx.batch_engine()
+    ctx.set_batch_engine(gkrx.BATCH_LOCKSTEP, block)
+    ctx.set_batch_lanes(lanes)
+    try:
+        batch = gkrx.Prover(ctx).prove_batch(circs)
+    finally:
+        ctx.set_batch_engine(*saved)
+        ctx.set_batch_lanes(4)
     for (claimed, proof), (c1, p1) in zip(batch, plain):
         assert np.array_equal(claimed, c1) and bytes(proof) == bytes(p1)
 
@@ -177,6 +194,8 @@ def test_graph_replays_follow_new_witnesses(ctx):
 def test_engine_arguments_checked(ctx):
     with pytest.raises(gkrx.GkrxError):
         ctx.set_batch_engine(gkrx.BATCH_LOCKSTEP, 96)
+    with pytest.raises(gkrx.GkrxError):
+        ctx.set_batch_lanes(3)
     with pytest.raises(gkrx.GkrxError):
         ctx.set_batch_engine(7, 256)
 
