@@ -251,9 +251,13 @@ def serve_leg(gkrx, fs, m31_state, seconds, budget_gb, slots):
     th.start()
     start = time.perf_counter()
     t_next = start
+    t_log = start + 30.0
     arrivals = waited = 0
     while t_next - start < seconds:
         now = time.perf_counter()
+        if now >= t_log:  # progress of a long serving run
+            log(f"serving: {now - start:.0f}s, {arrivals} arrivals, {len(lat)} completed")
+            t_log += 30.0
         if now < t_next:
             time.sleep(min(t_next - now, 0.002))
             continue
