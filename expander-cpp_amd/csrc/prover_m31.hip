@@ -2313,6 +2313,156 @@ struct LsStep {
 
 bool ls_is_chain(uint32_t op) { return op == kOpTailRep || op == kOpLast; }
 
+// The device addresses of every job's tables in the batch pool (job i's tables start at
+// pool + job_off[i]; its tail and claims sit in the tail and claim areas).
+void batch_job_table(const std::vector<BatchItem> &items, uint32_t R, uint32_t *pool,
+                     const std::vector<uint64_t> &job_off, uint32_t *tails, const std::vector<uint64_t> &tail_off,
+                     uint32_t *claims, std::vector<BatchJob> &hjobs)
+{
+    const uint32_t L = items[0].dc->L;
+    for (size_t i = 0; i < items.size(); i++) {
+        const DevCircuit &dc = *items[i].dc;
+        const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
+        const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
+        uint32_t *at = pool + job_off[i];
+        auto take = [&](uint64_t n) { uint32_t *p = at; at += (n + 63) & ~63ull; return p; };
+        BatchJob &J = hjobs[i];
+        J = BatchJob{};
+        J.D = dc.D;
+        J.layers = dc.d_layers;
+        J.values = dc.values;
+        J.eqz_rs = maxM; J.eqz = take(R * maxM);
+        J.eqx_rs = maxN; J.eqx = take(R * maxN);
+        J.hg0_rs = maxN * L; J.hg0 = take(R * maxN * L);
+        J.hgA_rs = N2 * L; J.hgA = take(R * N2 * L);
+        J.hgB_rs = N4 * L; J.hgB = take(R * N4 * L);
+        J.fA_rs = N2 * L; J.fA = take(R * N2 * L);
+        J.fB_rs = N4 * L; J.fB = take(R * N4 * L);
+        J.s0_rs = N1; J.s0 = take(R * N1);
+        J.sA_rs = N2; J.sA = take(R * N2);
+        J.sB_rs = N4; J.sB = take(R * N4);
+        J.chal = take((uint64_t)(dc.D + 1) * R * kChalStride);
+        J.ab = take(2ull * (dc.D + 1));
+        J.vxy = take(2ull * R * L);
+        J.tstate = take(16);
+        J.partials = take((uint64_t)R * kPairVals * L * kMaxBlocksPerRep);
+        J.tail = tails + tail_off[i];
+        J.claims = claims + i * (uint64_t)R * L;
+    }
+}
+
+// The lockstep schedule of a batch: for every step its task lists (tasks and the prefix
+// arrays of their first workgroups) and, per job, the step that installs its transcript seed.
+void ls_build_schedule(const std::vector<BatchItem> &items, const std::vector<BatchJob> &hjobs, uint32_t R,
+                       uint32_t BS, uint32_t LPT, std::vector<LsTask> &tasks, std::vector<uint32_t> &starts,
+                       std::vector<LsStep> &steps, std::vector<uint32_t> &seed_step)
+{
+    const size_t nj = items.size();
+    // The transcript position after the host prefix follows from the shapes alone: the 32
+    // grind bytes are in the buffer, hashed once if output challenges were drawn.
+    auto ts_hashed = [&](size_t i) -> uint64_t {
+        const DevCircuit &dc = *items[i].dc;
+        return (uint64_t)R * dc.layers[dc.D - 1].m ? 32 : 0;
+    };
+    // Every proof's forward pass first (steps [0, D)): it needs no transcript.  Then
+    // the sumchecks right-aligned: every proof ends at the last step, so a proof with
+    // fewer sumcheck steps starts later and needs its host prefix later.  A proof's
+    // seed runs one step before its first sumcheck op.
+    std::vector<std::vector<LsEntry>> ops(nj);
+    std::vector<LsEntry> seed_entry(nj);
+    std::vector<size_t> base(nj);
+    size_t dmax = 0, smax = 0, ntask = 0;
+    for (size_t i = 0; i < nj; i++) {
+        const uint32_t D = items[i].dc->D;
+        ops[i].reserve((size_t)D * (8 + 2 * items[i].dc->max_n));
+        ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, LPT, 32, ts_hashed(i), ops[i]);
+        dmax = std::max<size_t>(dmax, D);
+        smax = std::max(smax, ops[i].size() - D);
+        ntask += 2 * ops[i].size() + 1;
+    }
+    const size_t nsteps = dmax + smax;
+    for (size_t i = 0; i < nj; i++) {
+        const uint32_t D = items[i].dc->D;
+        base[i] = nsteps - (ops[i].size() - D);  // >= dmax >= D
+        seed_step[i] = (uint32_t)(base[i] - 1);
+        LsTask sd = ls_task(kOpSeed, (uint32_t)i, 0);
+        sd.nb = 1;
+        if (base[i] == D) {  // the seed shares the last forward step
+            LsEntry &e = ops[i][D - 1];
+            e.w[e.nw++] = sd;
+        } else {
+            seed_entry[i].w[0] = sd;
+            seed_entry[i].nw = 1;
+        }
+    }
+    // job i's entry at step s
+    auto at = [&](size_t i, size_t s) -> const LsEntry * {
+        const uint32_t D = items[i].dc->D;
+        if (s < D) return &ops[i][s];
+        if (s + 1 == base[i]) return &seed_entry[i];
+        if (s >= base[i]) return &ops[i][D + (s - base[i])];
+        return nullptr;
+    };
+    tasks.reserve(ntask);
+    starts.reserve(ntask + 3 * nsteps);
+    steps.reserve(nsteps);
+    for (size_t s = 0; s < nsteps; s++) {
+        LsStep S{};
+        // appends the tasks that `take` selects (in job order) as list `li`
+        auto list = [&](int li, auto &&take, auto &&nblocks) {
+            LsList &ls = S.l[li];
+            ls.task = (uint32_t)tasks.size();
+            ls.start = (uint32_t)starts.size();
+            uint32_t blocks = 0;
+            for (size_t i = 0; i < nj; i++) {
+                const LsEntry *e = at(i, s);
+                if (!e) continue;
+                take(*e, [&](const LsTask &t) {
+                    starts.push_back(blocks);
+                    tasks.push_back(t);
+                    blocks += nblocks(t);
+                });
+            }
+            ls.n = (uint32_t)tasks.size() - ls.task;
+            ls.blocks = blocks;
+            starts.push_back(blocks);
+        };
+        auto work_blocks = [&](const LsTask &t) { return ls_work_blocks(t, R); };
+        auto fin_blocks = [&](const LsTask &t) { return ls_fin_blocks(t, R); };
+        // wide work: by op (the gathers first: their block counts are multiples of 8)
+        {
+            LsList &ls = S.l[kLsWork];
+            ls.task = (uint32_t)tasks.size();
+            ls.start = (uint32_t)starts.size();
+            uint32_t blocks = 0;
+            for (uint32_t op : kLsWorkOrder)
+                for (size_t i = 0; i < nj; i++) {
+                    const LsEntry *e = at(i, s);
+                    if (!e) continue;
+                    for (uint32_t q = 0; q < e->nw; q++)
+                        if (e->w[q].op == op) {
+                            starts.push_back(blocks);
+                            tasks.push_back(e->w[q]);
+                            blocks += ls_work_blocks(e->w[q], R);
+                        }
+                }
+            ls.n = (uint32_t)tasks.size() - ls.task;
+            ls.blocks = blocks;
+            starts.push_back(blocks);
+        }
+        list(kLsDotFin, [](const LsEntry &e, auto &&put) { if (e.nf && e.f.op == kOpDotFin) put(e.f); },
+             fin_blocks);
+        list(kLsPairs, [](const LsEntry &e, auto &&put) {
+            for (uint32_t q = 0; q < e.nw; q++) if (e.w[q].op == kOpPair) put(e.w[q]);
+        }, work_blocks);
+        list(kLsPairFin, [](const LsEntry &e, auto &&put) { if (e.nf && e.f.op == kOpPairFin) put(e.f); },
+             fin_blocks);
+        list(kLsChain, [](const LsEntry &e, auto &&put) { if (e.nf && ls_is_chain(e.f.op)) put(e.f); },
+             fin_blocks);
+        steps.push_back(S);
+    }
+}
+
 inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
 }  // namespace
@@ -2359,12 +2509,6 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     // seed records (rz1 in chal-slot layout, transcript state, grind bytes), one per job in
     // the order in which the schedule needs them
     uint32_t *seeds = claims + nj * (uint64_t)R * L;
-    // The transcript position after the host prefix follows from the shapes alone: the 32
-    // grind bytes are in the buffer, hashed once if output challenges were drawn.
-    auto ts_hashed = [&](size_t i) -> uint64_t {
-        const DevCircuit &dc = *items[i].dc;
-        return (uint64_t)R * dc.layers[dc.D - 1].m ? 32 : 0;
-    };
     // The job table and the schedule depend only on the circuits' shapes, their device
     // addresses and the pool: a batch with the same key reuses the metadata already on the
     // device.
@@ -2386,39 +2530,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         bs.meta_key.clear();
         bs.meta_used = 0;
         std::vector<BatchJob> hjobs(nj);
-        std::vector<std::pair<uint64_t, uint32_t>> work(nj);
-        for (size_t i = 0; i < nj; i++) {
-            const DevCircuit &dc = *items[i].dc;
-            const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
-            const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
-            uint32_t *at = bs.pool + job_off[i];
-            auto take = [&](uint64_t n) { uint32_t *p = at; at += (n + 63) & ~63ull; return p; };
-            BatchJob &J = hjobs[i];
-            J = BatchJob{};
-            J.D = dc.D;
-            J.layers = dc.d_layers;
-            J.values = dc.values;
-            J.eqz_rs = maxM; J.eqz = take(R * maxM);
-            J.eqx_rs = maxN; J.eqx = take(R * maxN);
-            J.hg0_rs = maxN * L; J.hg0 = take(R * maxN * L);
-            J.hgA_rs = N2 * L; J.hgA = take(R * N2 * L);
-            J.hgB_rs = N4 * L; J.hgB = take(R * N4 * L);
-            J.fA_rs = N2 * L; J.fA = take(R * N2 * L);
-            J.fB_rs = N4 * L; J.fB = take(R * N4 * L);
-            J.s0_rs = N1; J.s0 = take(R * N1);
-            J.sA_rs = N2; J.sA = take(R * N2);
-            J.sB_rs = N4; J.sB = take(R * N4);
-            J.chal = take((uint64_t)(dc.D + 1) * R * kChalStride);
-            J.ab = take(2ull * (dc.D + 1));
-            J.vxy = take(2ull * R * L);
-            J.tstate = take(16);
-            J.partials = take((uint64_t)R * kPairVals * L * kMaxBlocksPerRep);
-            J.tail = tails + tail_off[i];
-            J.claims = claims + i * (uint64_t)R * L;
-            uint64_t w = 0;
-            for (uint32_t l = 0; l < dc.D; l++) w += (1ull << dc.layers[l].n);
-            work[i] = {w, (uint32_t)i};
-        }
+        batch_job_table(items, R, bs.pool, job_off, tails, tail_off, claims, hjobs);
         // per-proof engine: workgroup order, largest proof first; lockstep: the step schedule
         std::vector<uint32_t> horder;
         std::vector<LsTask> tasks;
@@ -2426,107 +2538,17 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         std::vector<LsStep> steps;
         bs.meta_seed_step.assign(nj, 0);
         if (!lockstep) {
+            std::vector<std::pair<uint64_t, uint32_t>> work(nj);  // (input rows of all layers, job)
+            for (size_t i = 0; i < nj; i++) {
+                uint64_t w = 0;
+                for (uint32_t l = 0; l < items[i].dc->D; l++) w += (1ull << items[i].dc->layers[l].n);
+                work[i] = {w, (uint32_t)i};
+            }
             std::stable_sort(work.begin(), work.end(), [](auto &x, auto &y) { return x.first > y.first; });
             horder.resize(nj);
             for (size_t i = 0; i < nj; i++) horder[i] = work[i].second;
         } else {
-            // Every proof's forward pass first (steps [0, D)): it needs no transcript.  Then
-            // the sumchecks right-aligned: every proof ends at the last step, so a proof with
-            // fewer sumcheck steps starts later and needs its host prefix later.  A proof's
-            // seed runs one step before its first sumcheck op.
-            std::vector<std::vector<LsEntry>> ops(nj);
-            std::vector<LsEntry> seed_entry(nj);
-            std::vector<size_t> base(nj);
-            size_t dmax = 0, smax = 0, ntask = 0;
-            for (size_t i = 0; i < nj; i++) {
-                const uint32_t D = items[i].dc->D;
-                ops[i].reserve((size_t)D * (8 + 2 * items[i].dc->max_n));
-                ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, LPT, 32, ts_hashed(i), ops[i]);
-                dmax = std::max<size_t>(dmax, D);
-                smax = std::max(smax, ops[i].size() - D);
-                ntask += 2 * ops[i].size() + 1;
-            }
-            const size_t nsteps = dmax + smax;
-            for (size_t i = 0; i < nj; i++) {
-                const uint32_t D = items[i].dc->D;
-                base[i] = nsteps - (ops[i].size() - D);  // >= dmax >= D
-                bs.meta_seed_step[i] = (uint32_t)(base[i] - 1);
-                LsTask sd = ls_task(kOpSeed, (uint32_t)i, 0);
-                sd.nb = 1;
-                if (base[i] == D) {  // the seed shares the last forward step
-                    LsEntry &e = ops[i][D - 1];
-                    e.w[e.nw++] = sd;
-                } else {
-                    seed_entry[i].w[0] = sd;
-                    seed_entry[i].nw = 1;
-                }
-            }
-            // job i's entry at step s
-            auto at = [&](size_t i, size_t s) -> const LsEntry * {
-                const uint32_t D = items[i].dc->D;
-                if (s < D) return &ops[i][s];
-                if (s + 1 == base[i]) return &seed_entry[i];
-                if (s >= base[i]) return &ops[i][D + (s - base[i])];
-                return nullptr;
-            };
-            tasks.reserve(ntask);
-            starts.reserve(ntask + 3 * nsteps);
-            steps.reserve(nsteps);
-            for (size_t s = 0; s < nsteps; s++) {
-                LsStep S{};
-                // appends the tasks that `take` selects (in job order) as list `li`
-                auto list = [&](int li, auto &&take, auto &&nblocks) {
-                    LsList &ls = S.l[li];
-                    ls.task = (uint32_t)tasks.size();
-                    ls.start = (uint32_t)starts.size();
-                    uint32_t blocks = 0;
-                    for (size_t i = 0; i < nj; i++) {
-                        const LsEntry *e = at(i, s);
-                        if (!e) continue;
-                        take(*e, [&](const LsTask &t) {
-                            starts.push_back(blocks);
-                            tasks.push_back(t);
-                            blocks += nblocks(t);
-                        });
-                    }
-                    ls.n = (uint32_t)tasks.size() - ls.task;
-                    ls.blocks = blocks;
-                    starts.push_back(blocks);
-                };
-                auto work_blocks = [&](const LsTask &t) { return ls_work_blocks(t, R); };
-                auto fin_blocks = [&](const LsTask &t) { return ls_fin_blocks(t, R); };
-                // wide work: by op (the gathers first: their block counts are multiples of 8)
-                {
-                    LsList &ls = S.l[kLsWork];
-                    ls.task = (uint32_t)tasks.size();
-                    ls.start = (uint32_t)starts.size();
-                    uint32_t blocks = 0;
-                    for (uint32_t op : kLsWorkOrder)
-                        for (size_t i = 0; i < nj; i++) {
-                            const LsEntry *e = at(i, s);
-                            if (!e) continue;
-                            for (uint32_t q = 0; q < e->nw; q++)
-                                if (e->w[q].op == op) {
-                                    starts.push_back(blocks);
-                                    tasks.push_back(e->w[q]);
-                                    blocks += ls_work_blocks(e->w[q], R);
-                                }
-                        }
-                    ls.n = (uint32_t)tasks.size() - ls.task;
-                    ls.blocks = blocks;
-                    starts.push_back(blocks);
-                }
-                list(kLsDotFin, [](const LsEntry &e, auto &&put) { if (e.nf && e.f.op == kOpDotFin) put(e.f); },
-                     fin_blocks);
-                list(kLsPairs, [](const LsEntry &e, auto &&put) {
-                    for (uint32_t q = 0; q < e.nw; q++) if (e.w[q].op == kOpPair) put(e.w[q]);
-                }, work_blocks);
-                list(kLsPairFin, [](const LsEntry &e, auto &&put) { if (e.nf && e.f.op == kOpPairFin) put(e.f); },
-                     fin_blocks);
-                list(kLsChain, [](const LsEntry &e, auto &&put) { if (e.nf && ls_is_chain(e.f.op)) put(e.f); },
-                     fin_blocks);
-                steps.push_back(S);
-            }
+            ls_build_schedule(items, hjobs, R, BS, LPT, tasks, starts, steps, bs.meta_seed_step);
         }
         // seed slots in the order of need
         std::vector<uint32_t> &sorder = bs.meta_seed_order;
