@@ -66,7 +66,9 @@ def pair_kernel_bytes(n, R, S):
 
 def cpu_baseline(layers_all, L, fs, seed, n_layers):
     """The reference prover (oracle/_ref, built from the reference sources) on a bounded
-    sample: the first n_layers layers of the workload circuit, single thread."""
+    sample: the first n_layers layers of the workload circuit, single thread.  Also returns
+    the reference harness's throughput mode (bench_keccak.cpp:82-92: one proof per thread on
+    every core) on the same sample, or None without the reference build."""
     import oracle_lib as ol
     import gkrx
 
@@ -74,16 +76,31 @@ def cpu_baseline(layers_all, L, fs, seed, n_layers):
     k = int(np.log2(len(sample[0][0]) + len(sample[0][1])))
     wit = gkrx.synth_witness(seed, 1 << k, L)
     gates = sum(len(m) + len(a) for m, a in sample)
+    desc = (f"first {n_layers} of the 24 layers (2^{k} gates each, same wiring and witness shape), "
+            f"evaluate excluded as in the reference harness")
+    allcores = None
     with tempfile.TemporaryDirectory() as d:
         mp, ap, wp = (os.path.join(d, x) for x in ("mul.txt", "add.txt", "w.bin"))
         gkrx.write_text(sample, mp, ap)
         wit.tofile(wp)
         kind, binpath = "reference", ol.ref_binary(L, fs)
         if ol.ref_available(L, fs):
+            cmd = [binpath, "time", mp, ap, wp, "1"]
             t0 = time.time()
-            out = subprocess.run([binpath, "time", mp, ap, wp, "1"], check=True, capture_output=True, text=True)
+            out = subprocess.run(cmd, check=True, capture_output=True, text=True)
             secs = float(out.stdout.strip().splitlines()[-1])
             log(f"cpu baseline (reference) wall {time.time() - t0:.1f}s, prove {secs:.3f}s")
+            # one proof per core at once, each in its own process
+            ncores = min(16, os.cpu_count() or 1)
+            t0 = time.time()
+            procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True) for _ in range(ncores)]
+            each = [float(p.communicate()[0].strip().splitlines()[-1]) for p in procs]
+            if any(p.returncode for p in procs):
+                raise RuntimeError("reference prover failed")
+            log(f"cpu baseline (reference, {ncores} processes) wall {time.time() - t0:.1f}s, "
+                f"prove {min(each):.3f} .. {max(each):.3f}s")
+            allcores = {"value": ncores * gates / max(each), "unit": "gates/s", "cores": ncores, "kind": kind,
+                        "sample": desc + f"; {ncores} proofs at once, one process per core, the slowest prove time"}
         else:
             kind = "port"
             oc = ol.OracleCircuit(mp, ap)
@@ -91,8 +108,7 @@ def cpu_baseline(layers_all, L, fs, seed, n_layers):
             oc.prove(L, fs, wit)
             secs = time.time() - t0
     return {"value": gates / secs, "unit": "gates/s", "cores": 1, "kind": kind,
-            "sample": f"first {n_layers} of the 24 layers (2^{k} gates each, same wiring and witness shape), "
-                      f"one proof on one core, evaluate excluded as in the reference harness"}
+            "sample": desc + "; one proof on one core"}, allcores
 
 
 def batch_specs(count, seed=3):
@@ -440,10 +456,10 @@ def main():
     bn254 = None
     if args.bn254_layers > 0:
         bn254 = bn254_leg(gkrx, fs, args.bn254_layers, args.bn254_log_width, 2, 1)
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(layers, L, fs, seed, args.cpu_layers)
+            cpu, cpu_all = cpu_baseline(layers, L, fs, seed, args.cpu_layers)
         except Exception as e:  # reported, never fatal
             log(f"cpu baseline failed: {e}")
     line = {
@@ -458,6 +474,7 @@ def main():
         "latency_ms": {"p50": round(percentile(lat, 50), 3), "p99": round(percentile(lat, 99), 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "breakdown_ms": {k2: round(float(np.mean([p[k2] for p in parts])), 3) for k2 in parts[0]},
         "kernels_ms_per_proof": {k2: round(v["ms"] / args.steps, 3) for k2, v in sorted(stats.items())},
         "verified": bool(ok),
