@@ -116,8 +116,8 @@ int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, u
  * whole proof.  LOCKSTEP (the default): the batch advances one op of every proof per
  * step; a step is one launch of the wide ops (each proof's op on as many workgroups of
  * `block` threads as its tables need), one of the round pairs, one of the serial chains
- * and the launches of the transcript ends.  block is 64, 128, 256 or 512.  Results do
- * not depend on the choice. */
+ * and the launches of the transcript ends; batches of more than 2048 proofs run on the
+ * PER_PROOF engine.  block is 64, 128, 256 or 512.  Results do not depend on the choice. */
 #define GKRX_BATCH_PER_PROOF 1
 #define GKRX_BATCH_LOCKSTEP 2
 int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block);

@@ -191,6 +191,27 @@ def test_graph_replays_follow_new_witnesses(ctx):
         c.close()
 
 
+def test_batch_beyond_the_lockstep_limit(ctx):
+    """More than 2048 proofs (the lockstep engine's limit) run on the per-proof engine; every
+    proof must still equal its plain proof."""
+    print("[gpu] batch of 2100 small proofs", flush=True)
+    specs = [(5000 + i, 1 + i % 3, 2 + i % 4) for i in range(2100)]
+    circs = _circuits(ctx, specs)
+    prover = gkrx.Prover(ctx)
+    saved = ctx.batch_engine()
+    ctx.set_batch_engine(gkrx.BATCH_LOCKSTEP, 256)
+    try:
+        batch = prover.prove_batch(circs)
+    finally:
+        ctx.set_batch_engine(*saved)
+    for i in range(0, len(circs), 97):
+        circs[i].evaluate()
+        c1, p1 = prover.prove(circs[i])
+        assert np.array_equal(batch[i][0], c1) and bytes(batch[i][1]) == bytes(p1), i
+    for c in circs:
+        c.close()
+
+
 def test_engine_arguments_checked(ctx):
     with pytest.raises(gkrx.GkrxError):
         ctx.set_batch_engine(gkrx.BATCH_LOCKSTEP, 96)
