@@ -1015,6 +1015,19 @@ extern "C" int gkrx_ctx_profile(gkrx_ctx *ctx, int enable)
     (void)hipStreamSynchronize(ctx->stream);
     ctx->prof.reset();
     ctx->prof.on = enable != 0;
+    ctx->prof.mask = ~0ull;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_ctx_profile_classes(gkrx_ctx *ctx, uint64_t mask)
+{
+    if (!ctx) return GKRX_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->prof.reset();
+    ctx->prof.on = mask != 0;
+    ctx->prof.mask = mask;
     return GKRX_OK;
 }
 

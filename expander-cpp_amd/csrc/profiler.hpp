@@ -1,5 +1,7 @@
 // Optional per-kernel-class timing with HIP events recorded on the launch stream.
-// Enabled per context (gkrx_ctx_profile); when off, begin/end are no-ops.
+// Enabled per context (gkrx_ctx_profile, gkrx_ctx_profile_classes); when off, or for a
+// class outside the mask, begin/end are no-ops (each recorded event adds a marker packet
+// to the stream, so timing every launch slows a proof down measurably).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -55,6 +57,8 @@ inline const char *kernel_class_name(int c)
 
 struct Profiler {
     bool on = false;
+    uint64_t mask = ~0ull;  // classes that are timed (bit c = class c)
+    bool open = false;      // begin recorded an event that end must close
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     std::vector<int> cls;  // class of event pair i (events 2i, 2i+1)
@@ -72,18 +76,36 @@ struct Profiler {
     }
     void begin(int c, hipStream_t st)
     {
-        if (!on) return;
+        open = false;
+        if (!on || !((mask >> c) & 1)) return;
         hipEvent_t e = next();
         if (e) {
             (void)hipEventRecord(e, st);
             cls.push_back(c);
+            open = true;
         }
     }
     void end(hipStream_t st)
     {
-        if (!on) return;
+        if (!open) return;
+        open = false;
         hipEvent_t e = next();
         if (e) (void)hipEventRecord(e, st);
+    }
+    // A start / stop event pair for a launch that records them itself
+    // (hipExtLaunchKernelGGL: the kernel's own start and end, no marker packets in the
+    // stream); false when class c is not timed.
+    bool kernel_events(int c, hipEvent_t &e0, hipEvent_t &e1)
+    {
+        if (!on || !((mask >> c) & 1)) return false;
+        e0 = next();
+        e1 = e0 ? next() : nullptr;
+        if (!e1) {
+            if (e0) used--;
+            return false;
+        }
+        cls.push_back(c);
+        return true;
     }
     // after the stream has been synchronized
     void collect()

@@ -27,6 +27,7 @@
 // Folding is ping-pong between buffers (the reference folds in place, which is only
 // safe sequentially).  The gate_exists skip of the reference is exact (hg is zero where
 // no gate exists) and is therefore not needed.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1670,6 +1671,16 @@ inline void timed(Profiler *pf, int cls, hipStream_t st, F &&launch)
     if (pf) pf->end(st);
 }
 
+// Launches kernel<<<grid, block, 0, st>>>(args); when class cls is timed, the launch records
+// the kernel's own start and end in a profiler event pair.
+template <typename K, typename... A>
+void launch_timed(Profiler *pf, int cls, K kernel, dim3 grid, dim3 block, hipStream_t st, A... args)
+{
+    hipEvent_t e0, e1;
+    if (pf && pf->kernel_events(cls, e0, e1)) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, e0, e1, 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
+
 template <int L>
 int forward_impl(const DevCircuit &dc, hipStream_t st, Profiler *pf)
 {
@@ -1806,15 +1817,14 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
                 rp.R = R;
                 dim3 grid(nbx * R);
                 const int cls = phase == 1 ? (j == 0 ? kKPairPh1First : kKPairPh1) : (j == 0 ? kKPairPh2First : kKPairPh2);
-                timed(pf, cls, st, [&] {
-                    if (phase == 1) {
-                        if (j == 0) hipLaunchKernelGGL((k_pair<L, false, 0>), grid, dim3(kThreads), 0, st, rp);
-                        else hipLaunchKernelGGL((k_pair<L, false, 2>), grid, dim3(kThreads), 0, st, rp);
-                    } else {
-                        if (j == 0) hipLaunchKernelGGL((k_pair<L, true, 0>), grid, dim3(kThreads), 0, st, rp);
-                        else hipLaunchKernelGGL((k_pair<L, true, 2>), grid, dim3(kThreads), 0, st, rp);
-                    }
-                });
+                const dim3 blk(kThreads);
+                if (phase == 1) {
+                    if (j == 0) launch_timed(pf, cls, k_pair<L, false, 0>, grid, blk, st, rp);
+                    else launch_timed(pf, cls, k_pair<L, false, 2>, grid, blk, st, rp);
+                } else {
+                    if (j == 0) launch_timed(pf, cls, k_pair<L, true, 0>, grid, blk, st, rp);
+                    else launch_timed(pf, cls, k_pair<L, true, 2>, grid, blk, st, rp);
+                }
                 timed(pf, kKPairFin, st, [&] {
                     hipLaunchKernelGGL(k_pair_fin<L>, dim3(1), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials,
                                        nbx, tk);

@@ -90,6 +90,7 @@ def lib():
     L.gkrx_ctx_set_batch_engine.argtypes = [vp, C.c_int, C.c_uint32]
     L.gkrx_ctx_get_batch_engine.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]
     L.gkrx_ctx_set_batch_lanes.argtypes = [vp, C.c_uint32]
+    L.gkrx_ctx_profile_classes.argtypes = [vp, C.c_uint64]
     L.gkrx_ctx_get_batch_lanes.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.gkrx_batch_autotune.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_int), C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
@@ -222,6 +223,18 @@ def kernel_stats(ctx: "Context") -> dict:
             out[name.value.decode()] = {"count": int(cnt.value), "ms": float(ms.value)}
         i += 1
     return out
+
+
+def kernel_class_index(ctx: "Context", name: str) -> int:
+    """Index of a kernel class by name (for gkrx_ctx_profile_classes masks)."""
+    i = 0
+    while True:
+        nm = C.c_char_p()
+        if lib().gkrx_ctx_kernel_stat(ctx._h, i, C.byref(nm), None, None) != 0:
+            raise KeyError(name)
+        if nm.value.decode() == name:
+            return i
+        i += 1
 
 
 def host_hash(fs_hash: int, data: bytes) -> bytes:

@@ -187,6 +187,10 @@ int gkrx_last_timing(const gkrx_ctx *ctx, double *ms3);
  * prints wall-clock phases).  enable = 1 resets the counters and starts recording;
  * gkrx_ctx_kernel_stat reads class idx (0 .. classes-1; GKRX_ERR_ARG past the end). */
 int gkrx_ctx_profile(gkrx_ctx *ctx, int enable);
+/* The same, timing only the classes whose bit is set in mask (bit idx = class idx; 0 turns
+ * timing off).  Every timed launch adds two event packets to the stream, so a benchmark
+ * that times one kernel inside its timed loop selects just that class. */
+int gkrx_ctx_profile_classes(gkrx_ctx *ctx, uint64_t mask);
 int gkrx_ctx_kernel_stat(gkrx_ctx *ctx, int idx, const char **name, uint64_t *launches, double *ms);
 
 /* Thread-local message of the last failed call. */

@@ -106,3 +106,31 @@ def test_repeat_prove_is_deterministic(tmp_path, contexts):
     c2, pr2 = p.prove(circ)
     assert np.array_equal(c1, c2) and bytes(pr1) == bytes(pr2)
     circ.close()
+
+
+def test_profile_classes_times_only_the_selected_class(contexts):
+    """Class-masked timing records the selected kernel class only, with the same launch
+    count as timing every class, and the proof bytes do not change."""
+    _say("[gpu] class-masked kernel timing")
+    layers = gkrx.synth_circuit(22, 3, 14, 700, True)
+    ctx = _ctx(contexts, 16, gkrx.FS_KECCAK256)
+    circ = gkrx.Circuit.from_layers(ctx, layers)
+    circ.set_witness(gkrx.synth_witness(22, 1 << 14, 16))
+    p = gkrx.Prover(ctx)
+    plain = p.prove(circ)
+    lib = gkrx.lib()
+    try:
+        lib.gkrx_ctx_profile(ctx._h, 1)
+        every = p.prove(circ)
+        all_stats = gkrx.kernel_stats(ctx)
+        idx = gkrx.kernel_class_index(ctx, "k_pair_ph1")
+        lib.gkrx_ctx_profile_classes(ctx._h, 1 << idx)
+        masked = p.prove(circ)
+        one = gkrx.kernel_stats(ctx)
+    finally:
+        lib.gkrx_ctx_profile(ctx._h, 0)
+    assert set(one) == {"k_pair_ph1"} and len(all_stats) > 5
+    assert one["k_pair_ph1"]["count"] == all_stats["k_pair_ph1"]["count"] > 0
+    for cl, pr in (every, masked):
+        assert np.array_equal(cl, plain[0]) and bytes(pr) == bytes(plain[1])
+    circ.close()
