@@ -830,44 +830,78 @@ __device__ __forceinline__ void reduce_partials(const uint32_t *__restrict__ par
     __syncthreads();
 }
 
-// The end of a pair of grid rounds (one workgroup of kFinThreads): adds the partials (the
-// nine products P0..P8 of pair_item per lane), forms round k's sums and runs its transcript
-// step, evaluates round k+1's sums at the r_k just drawn and runs its transcript step.
-// ts.k is round k.
+// The end of a pair of grid rounds for repetition `rep` (one workgroup of kFinThreads per
+// repetition): adds rep's partials (the nine products P0..P8 of pair_item per lane), forms
+// round k's sums and draws r_k, evaluates round k+1's sums at r_k and draws r_{k+1}.  Before
+// the last x round each repetition's challenge hashes only its own message (rep j's message
+// of round k sits at len0 + 4 Q j, of round k+1 at len0 + 4 Q (R + j); rep 0 of round k also
+// hashes the unhashed bytes from hs0 on), so the R workgroups run at once and read no
+// transcript state: the host tracks len0 and hs0, and the workgroup of the last rep writes
+// the state after round k+1.  ts.k is round k.
 template <int L>
-__device__ void pair_fin_body(const uint32_t *__restrict__ partials, uint32_t nb, TsArgs ts)
+__device__ void pair_fin_rep_body(const uint32_t *__restrict__ partials, uint32_t nb, const TsArgs &ts,
+                                  uint32_t len0, uint32_t hs0, uint32_t rep)
 {
     constexpr uint32_t Q = 3 * L, V = kPairVals * L;
-    __shared__ uint32_t vals[3 * V], sums[3 * Q], rr[3];
-    const uint32_t R = ts.R;
-    reduce_partials<L>(partials, nb, R * V, vals);
-    for (uint32_t t = threadIdx.x; t < R * L; t += blockDim.x) {
-        const uint32_t j = t / L, c = t % L;
-        const uint32_t *P = vals + j * V + c;  // P[q L] = product q of lane c
+    __shared__ uint32_t vals[V], sums[Q], hb[L + Q], rr;
+    const uint32_t R = ts.R, t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    reduce_partials<L>(partials + (uint64_t)rep * V * nb, nb, V, vals);
+    for (uint32_t c = t; c < L; c += blockDim.x) {
+        const uint32_t *P = vals + c;  // P[q L] = product q of lane c
         const uint32_t u = m31_add(P[0], P[3 * L]), v = m31_add(P[L], P[4 * L]), w = m31_add(P[2 * L], P[5 * L]);
-        sums[j * Q + c] = u;                                               // sum f0 h0
-        sums[j * Q + L + c] = v;                                           // sum f1 h1
-        sums[j * Q + 2 * L + c] = m31_sub(m31_add(m31_add(u, u), m31_add(v, v)), w);  // sum (f0 + f1)(h0 + h1)
+        sums[c] = u;                                                       // sum f0 h0
+        sums[L + c] = v;                                                   // sum f1 h1
+        sums[2 * L + c] = m31_sub(m31_add(m31_add(u, u), m31_add(v, v)), w);  // sum (f0 + f1)(h0 + h1)
     }
     __syncthreads();
-    transcript_step<L>(sums, ts, rr);
-    for (uint32_t t = threadIdx.x; t < R * Q; t += blockDim.x) {
-        const uint32_t j = t / Q, q = t % Q, s = q / L, c = q % L;
-        const uint32_t *P = vals + j * V + 3 * s * L + c;  // the three products of s_s
-        const uint32_t r = rr[j];
-        const uint32_t c1 = m31_sub(P[L], m31_add(P[0], P[2 * L]));
-        sums[t] = m31_add(P[0], m31_mul(r, m31_add(c1, m31_mul(r, P[2 * L]))));
+    uint32_t *chal = ts.chal + rep * kChalStride + (ts.phase == 2 ? 32u : 0u);
+    uint32_t d[8] = {};
+    for (uint32_t step = 0; step < 2; step++) {
+        if (wave == 0) {
+            // message words to the proof tail and to hb (behind `lead` pending words), then the hash
+            const uint32_t base = len0 + step * 4 * Q * R + 4 * Q * rep;
+            const uint32_t start = (rep == 0 && step == 0) ? hs0 : base;
+            const uint32_t lead = (base - start) / 4;
+            for (uint32_t w = lane; w < Q; w += 64) {
+                const uint32_t v = msg_word<L>(sums, ts.vx + rep * L, ts.phase, w);
+                ts.tail[base / 4 + w] = v;
+                hb[L + w] = v;
+            }
+            if (lane < lead && lead <= L) hb[L - lead + lane] = ts.tail[start / 4 + lane];
+            __threadfence_block();
+            hash_words_wave(ts.hash_kind, lead <= L ? hb + L - lead : ts.tail + start / 4, lead + Q, d);
+            if (lane == 0) {
+                const uint32_t c = challenge_from(d);
+                chal[ts.k + step] = c;
+                rr = c;
+            }
+        }
+        __syncthreads();
+        if (step == 0) {
+            // round k+1: the three sums of each lane as quadratics in r_k, at r_k
+            const uint32_t r = rr;
+            for (uint32_t q = t; q < Q; q += blockDim.x) {
+                const uint32_t s = q / L, c = q % L;
+                const uint32_t *P = vals + 3 * s * L + c;  // the three products of s_s
+                const uint32_t c1 = m31_sub(P[L], m31_add(P[0], P[2 * L]));
+                sums[q] = m31_add(P[0], m31_mul(r, m31_add(c1, m31_mul(r, P[2 * L]))));
+            }
+            __syncthreads();
+        }
     }
-    __syncthreads();
-    ts.k += 1;
-    transcript_step<L>(sums, ts, rr);
+    if (rep == R - 1 && t == 0) {
+        const uint32_t len = len0 + 2 * 4 * Q * R;
+        ts.tstate[0] = len;
+        ts.tstate[1] = len;
+        for (int i = 0; i < 8; i++) ts.tstate[2 + i] = d[i];
+    }
 }
 
 template <int L>
 __global__ void __launch_bounds__(kFinThreads) k_pair_fin(const uint32_t *__restrict__ partials, uint32_t nb,
-                                                          TsArgs ts)
+                                                          TsArgs ts, uint32_t len0, uint32_t hs0)
 {
-    pair_fin_body<L>(partials, nb, ts);
+    pair_fin_rep_body<L>(partials, nb, ts, len0, hs0, blockIdx.x);
 }
 
 // Rounds k0 .. n-1 of one phase in a single workgroup of 768 threads: the tables are
@@ -1588,7 +1622,7 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restri
     if (T.op == kOpPairFin) {
         TsArgs ts = ls_ts<L>(J, T.l, T.phase, R, hash_kind);
         ts.k = T.k;
-        pair_fin_body<L>(J.partials, T.nb, ts);
+        pair_fin_rep_body<L>(J.partials, T.nb, ts, T.len0, T.hs0, f.y);  // one workgroup per rep
     } else {
         reduce_partials<L>(J.partials, T.nb, R * Q, sums);
         if (threadIdx.x < R * L) J.claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
@@ -1826,8 +1860,8 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
                     else launch_timed(pf, cls, k_pair<L, true, 2>, grid, blk, st, rp);
                 }
                 timed(pf, kKPairFin, st, [&] {
-                    hipLaunchKernelGGL(k_pair_fin<L>, dim3(1), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials,
-                                       nbx, tk);
+                    hipLaunchKernelGGL(k_pair_fin<L>, dim3(R), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials,
+                                       nbx, tk, (uint32_t)ts_len, (uint32_t)ts_hs);
                 });
                 ts_len += 2ull * R * 4 * Q;
                 ts_hs = ts_len;
@@ -2113,7 +2147,7 @@ uint32_t ls_work_blocks(const LsTask &t, uint32_t R)
     return (t.op == kOpEqz || t.op == kOpEqx || t.op == kOpDot || t.op == kOpPair) ? R * t.nb : t.nb;
 }
 
-uint32_t ls_fin_blocks(const LsTask &t, uint32_t R) { return t.op == kOpTailRep ? R : 1u; }
+uint32_t ls_fin_blocks(const LsTask &t, uint32_t R) { return t.op == kOpTailRep || t.op == kOpPairFin ? R : 1u; }
 
 LsTask ls_task(uint32_t op, uint32_t job, uint32_t l)
 {
@@ -2249,6 +2283,8 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
                 fz.phase = phase;
                 fz.k = k;
                 fz.nb = t.nb;
+                fz.len0 = (uint32_t)ts_len;
+                fz.hs0 = (uint32_t)ts_hs;
                 e.f = fz;
                 e.nf = 1;
                 ops.push_back(e);
