@@ -980,7 +980,7 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(TailParams p)
 // message (rep 0 of round k0 also takes the unhashed bytes from hs0 on).  The host tracks
 // len0 and hs0 (the proof layout is fixed by the circuit shape), so this kernel reads no
 // transcript state; the workgroup of the last rep writes the final state.
-constexpr uint32_t kRepThreads = 1024;
+constexpr uint32_t kRepThreads = 512;
 struct TailRepParams {
     TailParams t;
     uint32_t kend;
