@@ -481,12 +481,17 @@ __device__ __forceinline__ void round_item_dbl(const uint32_t *__restrict__ fs, 
 constexpr uint32_t kPairVals = 9;
 
 __device__ __forceinline__ void pair_add(uint32_t &acc, uint32_t a, uint32_t b) { acc = m31_add(acc, m31_mul(a, b)); }
+// lazy form: one 64-bit multiply-add per product; the caller folds (pair_fold) before the
+// sum can reach 2^64 (a product of canonical values is below 2^62)
+__device__ __forceinline__ void pair_add(uint64_t &acc, uint32_t a, uint32_t b) { acc += (uint64_t)a * b; }
+__device__ __forceinline__ void pair_fold(uint64_t &acc) { acc = (acc & kP) + (acc >> 31); }  // < 2^34
 
-// g indexes the thread's V lanes [V g, V g + V) of an entry (L / V threads per entry).
-template <int L, bool PH2, int MODE, int V = 4>
+// g indexes the thread's V lanes [V g, V g + V) of an entry (L / V threads per entry).  A is
+// the accumulator: canonical u32, or u64 sums of products (see pair_add).
+template <int L, bool PH2, int MODE, int V = 4, typename A = uint32_t>
 __device__ __forceinline__ void pair_item(const uint32_t *__restrict__ fs, uint32_t *__restrict__ fd,
                                           const uint32_t *__restrict__ hs, uint32_t *__restrict__ hd, uint64_t i,
-                                          uint32_t g, uint32_t ra, uint32_t rb, uint32_t (&acc)[kPairVals][V])
+                                          uint32_t g, uint32_t ra, uint32_t rb, A (&acc)[kPairVals][V])
 {
     Lanes<V> f[4];
     Lanes<V> h[4];    // phase 1
@@ -785,8 +790,29 @@ __global__ void __launch_bounds__(kThreads) k_pair(RoundParams p)
     const uint32_t rb = MODE ? p.rprev[(uint64_t)rep * kChalStride + 1] : 0u;
     uint32_t acc[kPairVals][4] = {};
     const uint64_t total = (uint64_t)p.items * LG;
-    for (uint64_t w = (uint64_t)bx * kThreads + threadIdx.x; w < total; w += (uint64_t)nb * kThreads)
-        pair_item<L, PH2, MODE>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
+    if constexpr (MODE == 0) {
+        // the first pair forms its products from rows that need no fold, so the multiplies
+        // dominate: 64-bit sums, folded every third item (2^34 + 3 * 2^62 < 2^64)
+        uint64_t a64[kPairVals][4] = {};
+        uint32_t since = 0;
+        for (uint64_t w = (uint64_t)bx * kThreads + threadIdx.x; w < total; w += (uint64_t)nb * kThreads) {
+            pair_item<L, PH2, MODE, 4, uint64_t>(fs, fd, hs, hd, w / LG, g, ra, rb, a64);
+            if (++since == 3) {
+                since = 0;
+#pragma unroll
+                for (int q = 0; q < kPairVals; q++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) pair_fold(a64[q][c]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kPairVals; q++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[q][c] = m31_reduce64(a64[q][c]);
+    } else {
+        for (uint64_t w = (uint64_t)bx * kThreads + threadIdx.x; w < total; w += (uint64_t)nb * kThreads)
+            pair_item<L, PH2, MODE>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
+    }
     block_reduce_vals<L, kThreads, kPairVals>(acc, p.partials + (uint64_t)rep * kPairVals * L * nb + bx, nb);
 }
 
