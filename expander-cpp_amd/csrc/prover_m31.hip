@@ -1457,8 +1457,28 @@ __device__ void ls_pair(const RoundParams &p, uint32_t nb, uint32_t lb)
     const uint32_t rb = MODE ? p.rprev[(uint64_t)rep * kChalStride + 1] : 0u;
     uint32_t acc[kPairVals][V] = {};
     const uint64_t total = (uint64_t)p.items * LG;
-    for (uint64_t w = (uint64_t)bx * BS + threadIdx.x; w < total; w += (uint64_t)nb * BS)
-        pair_item<L, PH2, MODE, V>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
+    if constexpr (MODE == 0) {
+        // 64-bit product sums, folded every third item (as in k_pair)
+        uint64_t a64[kPairVals][V] = {};
+        uint32_t since = 0;
+        for (uint64_t w = (uint64_t)bx * BS + threadIdx.x; w < total; w += (uint64_t)nb * BS) {
+            pair_item<L, PH2, MODE, V, uint64_t>(fs, fd, hs, hd, w / LG, g, ra, rb, a64);
+            if (++since == 3) {
+                since = 0;
+#pragma unroll
+                for (int q = 0; q < kPairVals; q++)
+#pragma unroll
+                    for (int c = 0; c < V; c++) pair_fold(a64[q][c]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kPairVals; q++)
+#pragma unroll
+            for (int c = 0; c < V; c++) acc[q][c] = m31_reduce64(a64[q][c]);
+    } else {
+        for (uint64_t w = (uint64_t)bx * BS + threadIdx.x; w < total; w += (uint64_t)nb * BS)
+            pair_item<L, PH2, MODE, V>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
+    }
     block_reduce_vals<L, BS, kPairVals, V>(acc, p.partials + (uint64_t)rep * kPairVals * L * nb + bx, nb);
 }
 
