@@ -54,6 +54,20 @@ GKRX_HD uint32_t m31_mul(uint32_t a, uint32_t b)
     return umin32(t, t - kP);
 }
 
+// A value below 2^32 congruent to a * b (a, b < p): the product folded once, without the
+// final conditional subtract; for sums that are reduced once at the end (m31_reduce64).
+GKRX_HD uint32_t m31_mul_wide(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t lo = a * b;
+    uint32_t hi = __umulhi(a, b);
+    return (lo & kP) + ((hi << 1) | (lo >> 31));
+#else
+    uint64_t x = (uint64_t)a * b;
+    return (uint32_t)((x & kP) + (x >> 31));
+#endif
+}
+
 // M31::from_bytes on a little-endian word (M31.hpp:111-116): full reduction.
 GKRX_HD uint32_t m31_from_u32_full(uint32_t x)
 {

@@ -259,12 +259,13 @@ __device__ void gx_tile(unsigned long long *__restrict__ acc, uint32_t *__restri
 #pragma unroll
         for (uint32_t j = 0; j < 3; j++)
             if (j < R) {
-                const uint4 p = mul4s(vb, m31_mul(en.z, eqz[j * eqz_rs + en.y]));
+                // products below 2^32, reduced once when the row is written
+                const uint32_t c = m31_mul(en.z, eqz[j * eqz_rs + en.y]);
                 unsigned long long *aj = a + j * ROWS * L;
-                atomicAdd(aj, (unsigned long long)p.x);
-                atomicAdd(aj + 1, (unsigned long long)p.y);
-                atomicAdd(aj + 2, (unsigned long long)p.z);
-                atomicAdd(aj + 3, (unsigned long long)p.w);
+                atomicAdd(aj, (unsigned long long)m31_mul_wide(vb.x, c));
+                atomicAdd(aj + 1, (unsigned long long)m31_mul_wide(vb.y, c));
+                atomicAdd(aj + 2, (unsigned long long)m31_mul_wide(vb.z, c));
+                atomicAdd(aj + 3, (unsigned long long)m31_mul_wide(vb.w, c));
             }
     }
     __syncthreads();
@@ -312,7 +313,8 @@ __device__ void hy_tile(unsigned long long *__restrict__ acc, uint32_t *__restri
         for (uint32_t j = 0; j < 3; j++)
             if (j < R)
                 atomicAdd(acc + j * kHyRows + (en.w - y0),
-                          (unsigned long long)m31_mul(m31_mul(eqz[j * eqz_rs + en.y], eqx[j * eqx_rs + en.x]), en.z));
+                          (unsigned long long)m31_mul_wide(m31_mul(eqz[j * eqz_rs + en.y], eqx[j * eqx_rs + en.x]),
+                                                           en.z));
     }
     __syncthreads();
     for (uint32_t i = t; i < R * rows; i += BS) {
