@@ -192,7 +192,8 @@ __device__ __forceinline__ void hash_words(int kind, const uint32_t *msg, uint32
 // that the x - 1, x + 1 and x + 2 neighbours of every state lane sit in the same
 // 8-lane group and a DPP row shift reaches them.  Lanes 40..63 stay zero.
 //   theta : column parity = XOR of the 5 rows, formed with a DPP row rotate by 8 (rows
-//           2r, 2r+1 share a 16-lane DPP row) and the gfx950 permlane16 / permlane32 swaps;
+//           2r, 2r+1 share a 16-lane DPP row) and three gfx950 permlane16 / permlane32 swaps
+//           that serve both 32-bit halves of the words at once;
 //   rho   : per-lane rotate (v_alignbit with a per-lane shift);
 //   pi    : one ds_bpermute (the only LDS-latency step of a round);
 //   chi   : DPP row shifts by 1 and 2 inside the group;
@@ -236,13 +237,22 @@ __device__ __forceinline__ uint32_t dpp_from_prev(uint32_t v) { return __builtin
 __device__ __forceinline__ uint32_t dpp_from_prev5(uint32_t v) { return __builtin_amdgcn_update_dpp(0u, v, 0x115, 0xf, 0xf, true); }
 __device__ __forceinline__ uint32_t dpp_ror8(uint32_t v) { return __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xf, 0xf, true); }
 
-// XOR of the 4 rows of 16 lanes (every lane gets the XOR over its lane % 16 position).
-__device__ __forceinline__ uint32_t xor_rows(uint32_t v)
+// XOR of the 4 rows of 16 lanes of tlo and of thi at once (every lane gets the XOR over its
+// lane % 16 position, of tlo in clo and of thi in chi).  permlane16_swap exchanges the odd
+// rows of its first operand with the even rows of its second, permlane32_swap the upper half
+// of the first with the lower half of the second:
+//   swap16(tlo, thi) -> [lo0 hi0 lo2 hi2], [lo1 hi1 lo3 hi3];  x = their XOR
+//   swap32(x, x)     -> [x0 x1 x0 x1], [x2 x3 x2 x3];          y = [LO HI LO HI]
+//   swap16(y, y)     -> [LO LO LO LO], [HI HI HI HI]
+__device__ __forceinline__ void xor_rows2(uint32_t tlo, uint32_t thi, uint32_t &clo, uint32_t &chi)
 {
-    auto a = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-    const uint32_t p = a[0] ^ a[1];
-    auto b = __builtin_amdgcn_permlane32_swap(p, p, false, false);
-    return b[0] ^ b[1];
+    auto a = __builtin_amdgcn_permlane16_swap(tlo, thi, false, false);
+    const uint32_t x = a[0] ^ a[1];
+    auto b = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    const uint32_t y = b[0] ^ b[1];
+    auto c = __builtin_amdgcn_permlane16_swap(y, y, false, false);
+    clo = c[0];
+    chi = c[1];
 }
 
 __device__ __forceinline__ void keccak_f1600_wave(uint32_t &lo, uint32_t &hi, const KeccakWaveLane &k)
@@ -260,7 +270,8 @@ __device__ __forceinline__ void keccak_f1600_wave(uint32_t &lo, uint32_t &hi, co
 #pragma unroll
     for (int round = 0; round < 24; round++) {
         // theta
-        const uint32_t clo = xor_rows(lo ^ dpp_ror8(lo)), chi = xor_rows(hi ^ dpp_ror8(hi));
+        uint32_t clo, chi;
+        xor_rows2(lo ^ dpp_ror8(lo), hi ^ dpp_ror8(hi), clo, chi);
         const uint32_t nlo = dpp_from_next(clo), nhi = dpp_from_next(chi);
         lo ^= dpp_from_prev(clo) ^ ((nlo << 1) | (nhi >> 31));
         hi ^= dpp_from_prev(chi) ^ ((nhi << 1) | (nlo >> 31));
