@@ -573,7 +573,7 @@ extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t
                            (uint32_t)ctx->cfg.grinding_bits, grind, rz1.data(), tstate);
     const auto t1 = std::chrono::steady_clock::now();
     rc = dev_prove(c->dev, ctx->scratch, ctx->R, ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0, rz1.data(), tstate,
-                   grind, 32, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr);
+                   grind, 32, ctx->stream, ctx->prof.on ? &ctx->prof : nullptr, ctx->graphs);
     if (rc) return rc;
     uint32_t tfinal[10];
     memcpy(proof, c->witness.data(), commit_len);  // RAW commitment bytes (raw.hpp:19-27)
@@ -806,6 +806,29 @@ extern "C" int gkrx_ctx_get_batch_engine(const gkrx_ctx *ctx, int *engine, uint3
     if (!ctx) return GKRX_ERR_ARG;
     if (engine) *engine = ctx->batch.engine;
     if (block) *block = ctx->batch.block;
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_ctx_set_graphs(gkrx_ctx *ctx, int on)
+{
+    if (!ctx || (on != 0 && on != 1)) {
+        set_error("graphs: 0 or 1");
+        return GKRX_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->graphs = on != 0;
+    if (!ctx->graphs) {
+        GKRX_HIP_CHECK(hipSetDevice(ctx->device));
+        GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // a replay may still run
+        dev_drop_graph(ctx->scratch);
+    }
+    return GKRX_OK;
+}
+
+extern "C" int gkrx_ctx_get_graphs(const gkrx_ctx *ctx, int *on)
+{
+    if (!ctx || !on) return GKRX_ERR_ARG;
+    *on = ctx->graphs ? 1 : 0;
     return GKRX_OK;
 }
 

@@ -23,6 +23,7 @@ struct gkrx_ctx {
     gkrx::ProveTimings last;
     gkrx::Profiler prof;
     gkrx::BatchScratch batch;
+    bool graphs = false;  // gkrx_prove replays its device part as a captured HIP graph
     std::mutex mu;
     std::atomic<int> live_circuits{0};  // circuits (and servers) bound to this context
 };

@@ -73,6 +73,7 @@ struct DevCircuit {
     uint64_t csr_u32 = 0;            // size of csr_pool
     uint32_t max_n = 0, max_m = 0;
     LayerDev *d_layers = nullptr;    // device copy of `layers` (read by the batch kernel)
+    uint64_t serial = 0;             // unique per upload (identifies the circuit to a cached graph)
 };
 
 // One proof of a batch: its circuit, the host transcript prefix, and where the proof
@@ -161,6 +162,12 @@ struct DevScratch {
     // pinned host staging for the small uploads of a prove (outlives the async copies)
     uint32_t *stage = nullptr;
     uint64_t stage_u32 = 0;
+    // the device part of a prove captured as a HIP graph (the seed copies from `stage` and
+    // every launch), replayed while the circuit, R, hash and transcript offsets are the same
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_serial = 0, graph_len = 0, graph_hs = 0;
+    uint32_t graph_R = 0;
+    int graph_kind = -1;
 };
 
 // Device scratch of the BN254 prover (one lane, one repetition; every table entry is one
@@ -197,13 +204,16 @@ int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &
 void dev_free_circuit(DevCircuit &dc);
 int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t tail_bytes);
 void dev_free_scratch(DevScratch &s);
+void dev_drop_graph(DevScratch &s);  // releases the cached prove graph, if any
 int dev_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf);
 // Runs the on-device part of the prove: output claims, every layer sumcheck and the
 // transcript from the state in `tstate_host` (len, hash_start, digest) and the tail
 // prefix already in s.tail.  rz1 (R * m_last) seeds the last layer.
+// graph: run the device part as a captured HIP graph (captured on first use, replayed after);
+// ignored while profiling.
 int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, const uint32_t *rz1_host,
               const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,
-              Profiler *pf);
+              Profiler *pf, bool graph = false);
 // The transcript prefixes of a batch, computed while the batch is enqueued: start() hands
 // over the order in which the items' prefixes are needed, wait(i) returns once item i's
 // rz1, tstate and grind (which BatchItem points at) are written.

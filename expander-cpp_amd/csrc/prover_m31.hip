@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 
 #include "hash_dev.hpp"
@@ -1988,7 +1989,7 @@ int dev_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf)
 
 int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, const uint32_t *rz1_host,
               const uint32_t *tstate_host, const uint8_t *tail_prefix, uint32_t tail_prefix_len, hipStream_t st,
-              Profiler *pf)
+              Profiler *pf, bool graph)
 {
     if (R > 3) return GKRX_ERR_CONFIG;
     const uint32_t D = dc.D;
@@ -2007,15 +2008,60 @@ int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, co
     ab_seed[1] = 0u;
     memcpy(ts, tstate_host, 40);
     if (tail_prefix_len) memcpy(pre, tail_prefix, tail_prefix_len);
-    GKRX_HIP_CHECK(hipMemcpyAsync(s.chal + (uint64_t)D * R * kChalStride, seed, seed_u32 * 4,
-                                  hipMemcpyHostToDevice, st));
-    GKRX_HIP_CHECK(hipMemcpyAsync(s.ab + 2ull * D, ab_seed, 8, hipMemcpyHostToDevice, st));
-    GKRX_HIP_CHECK(hipMemcpyAsync(s.tstate, ts, 40, hipMemcpyHostToDevice, st));
-    if (tail_prefix_len)
-        GKRX_HIP_CHECK(hipMemcpyAsync(s.tail, pre, tail_prefix_len, hipMemcpyHostToDevice, st));
-    if (dc.L == 8) return prove_impl<8>(dc, s, R, hash_kind, tstate_host[0], tstate_host[1], st, pf);
-    if (dc.L == 16) return prove_impl<16>(dc, s, R, hash_kind, tstate_host[0], tstate_host[1], st, pf);
-    return GKRX_ERR_CONFIG;
+    // the copies from the pinned stage and every launch; in a graph the copies read the stage
+    // when the graph runs, so a replay picks up the seed written above
+    auto enqueue = [&]() -> int {
+        GKRX_HIP_CHECK(hipMemcpyAsync(s.chal + (uint64_t)D * R * kChalStride, seed, seed_u32 * 4,
+                                      hipMemcpyHostToDevice, st));
+        GKRX_HIP_CHECK(hipMemcpyAsync(s.ab + 2ull * D, ab_seed, 8, hipMemcpyHostToDevice, st));
+        GKRX_HIP_CHECK(hipMemcpyAsync(s.tstate, ts, 40, hipMemcpyHostToDevice, st));
+        if (tail_prefix_len)
+            GKRX_HIP_CHECK(hipMemcpyAsync(s.tail, pre, tail_prefix_len, hipMemcpyHostToDevice, st));
+        if (dc.L == 8) return prove_impl<8>(dc, s, R, hash_kind, tstate_host[0], tstate_host[1], st, pf);
+        if (dc.L == 16) return prove_impl<16>(dc, s, R, hash_kind, tstate_host[0], tstate_host[1], st, pf);
+        return GKRX_ERR_CONFIG;
+    };
+    if (!graph || pf) return enqueue();
+    const bool same = s.graph && s.graph_serial == dc.serial && s.graph_R == R && s.graph_kind == hash_kind &&
+                      s.graph_len == tstate_host[0] && s.graph_hs == tstate_host[1];
+    if (!same) {
+        // capture the sequence once (nothing runs while capturing), then keep the executable
+        // graph; everything it touches (scratch, circuit tables, stage) stays at its address
+        // until the scratch or the circuit changes, which changes the key
+        dev_drop_graph(s);
+        GKRX_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc != GKRX_OK || ec != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc != GKRX_OK) return rc;
+            set_error(std::string("graph capture: ") + hipGetErrorString(ec));
+            return GKRX_ERR_DEVICE;
+        }
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess || !x) {
+            set_error(std::string("graph instantiate: ") + hipGetErrorString(ei));
+            return GKRX_ERR_DEVICE;
+        }
+        s.graph = x;
+        s.graph_serial = dc.serial;
+        s.graph_R = R;
+        s.graph_kind = hash_kind;
+        s.graph_len = tstate_host[0];
+        s.graph_hs = tstate_host[1];
+    }
+    GKRX_HIP_CHECK(hipGraphLaunch(s.graph, st));
+    return GKRX_OK;
+}
+
+void dev_drop_graph(DevScratch &s)
+{
+    if (s.graph) (void)hipGraphExecDestroy(s.graph);
+    s.graph = nullptr;
+    s.graph_serial = 0;
 }
 
 // CSR for one key: counting sort of entries by key.
@@ -2036,6 +2082,8 @@ static void build_csr(uint32_t nrows, const std::vector<uint32_t> &keys, const s
 int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &n, const std::vector<uint32_t> &m,
                        const std::vector<std::vector<uint32_t>> &mul, const std::vector<std::vector<uint32_t>> &add)
 {
+    static std::atomic<uint64_t> serials{0};
+    dc.serial = ++serials;
     dc.D = (uint32_t)n.size();
     dc.L = L;
     dc.layers.assign(dc.D, LayerDev{});
@@ -2143,6 +2191,7 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
 
 void dev_free_scratch(DevScratch &s)
 {
+    dev_drop_graph(s);
     if (s.pool) (void)hipFree(s.pool);
     if (s.stage) (void)hipHostFree(s.stage);
     s = DevScratch{};

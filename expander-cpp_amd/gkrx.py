@@ -92,6 +92,8 @@ def lib():
     L.gkrx_ctx_set_batch_lanes.argtypes = [vp, C.c_uint32]
     L.gkrx_ctx_profile_classes.argtypes = [vp, C.c_uint64]
     L.gkrx_ctx_get_batch_lanes.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.gkrx_ctx_set_graphs.argtypes = [vp, C.c_int]
+    L.gkrx_ctx_get_graphs.argtypes = [vp, C.POINTER(C.c_int)]
     L.gkrx_batch_autotune.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_int), C.POINTER(C.c_uint32),
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
     L.gkrx_ctx_kernel_stat.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
@@ -341,6 +343,15 @@ class Context:
         v = C.c_uint32()
         _check(lib().gkrx_ctx_get_batch_lanes(self._h, C.byref(v)), "gkrx_ctx_get_batch_lanes")
         return v.value
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay the device part of Prover.prove as a HIP graph captured per circuit."""
+        _check(lib().gkrx_ctx_set_graphs(self._h, 1 if on else 0), "gkrx_ctx_set_graphs")
+
+    def graphs(self) -> bool:
+        v = C.c_int()
+        _check(lib().gkrx_ctx_get_graphs(self._h, C.byref(v)), "gkrx_ctx_get_graphs")
+        return bool(v.value)
 
 
 class Circuit:

@@ -122,6 +122,13 @@ int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, u
 #define GKRX_BATCH_LOCKSTEP 2
 int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block);
 int gkrx_ctx_get_batch_engine(const gkrx_ctx *ctx, int *engine, uint32_t *block);
+/* HIP graphs for gkrx_prove (0 off, the default; 1 on): the device part of a prove (the
+ * seed copies and every launch) is captured once per circuit and replayed on later proves of
+ * the same circuit (new witnesses); the proof bytes are the same either way.  Ignored while
+ * profiling.  No counterpart in the reference (its prover runs on the CPU,
+ * include/LinearGKR/LinearGKR.hpp:58-90). */
+int gkrx_ctx_set_graphs(gkrx_ctx *ctx, int on);
+int gkrx_ctx_get_graphs(const gkrx_ctx *ctx, int *on);
 /* Lanes of a table entry per thread in the lockstep engine's round-pair kernel: 1, 2 or 4
  * (one 4-, 8- or 16-byte access per thread; L / lanes threads per entry).  Default 4. */
 int gkrx_ctx_set_batch_lanes(gkrx_ctx *ctx, uint32_t lanes);
