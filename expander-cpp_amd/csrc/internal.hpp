@@ -105,6 +105,7 @@ struct BatchScratch {
     uint32_t *pool = nullptr;
     uint64_t pool_u32 = 0;
     uint8_t *meta = nullptr;        // job table, order or step schedule
+    uint32_t *queue = nullptr;      // per-proof engine: the work queue's head (next order slot)
     uint64_t meta_bytes = 0;
     uint8_t *host_meta = nullptr;   // pinned staging of the metadata and seeds
     uint64_t host_meta_bytes = 0;

@@ -1245,13 +1245,36 @@ __device__ void block_claims(const uint32_t *__restrict__ vals, const uint32_t *
 // LDS accumulators shared by the g_x and h_y tiles of one workgroup
 constexpr uint32_t kTileAccWords = kGxAccWords > kHyAccWords ? kGxAccWords : kHyAccWords;
 
+// A persistent grid over a device work queue: each workgroup takes the next slot of `order`
+// (largest proofs first) from the queue head until the slots run out, and proves that job
+// whole.  *queue must be 0 at launch.
+template <int L>
+__device__ void prove_batch_job(const BatchJob &J, uint32_t R, int hash_kind, unsigned long long *tacc);
+
 template <int L>
 __global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *__restrict__ jobs,
-                                                               const uint32_t *__restrict__ order, uint32_t R,
-                                                               int hash_kind)
+                                                               const uint32_t *__restrict__ order, uint32_t nj,
+                                                               uint32_t *queue, uint32_t R, int hash_kind)
 {
     __shared__ unsigned long long tacc[kTileAccWords];
-    const BatchJob J = jobs[order[blockIdx.x]];
+    __shared__ uint32_t slot;
+    for (;;) {
+        if (threadIdx.x == 0) slot = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t q = slot;
+        __syncthreads();  // every thread has read the slot before thread 0 takes the next
+        if (q >= nj) return;
+        const BatchJob J = jobs[order[q]];
+        prove_batch_job<L>(J, R, hash_kind, tacc);
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
+template <int L>
+__device__ void prove_batch_job(const BatchJob &J, uint32_t R, int hash_kind, unsigned long long *tacc)
+{
     const uint32_t D = J.D;
     auto chal_slot = [&](uint32_t l) { return J.chal + (uint64_t)l * R * kChalStride; };
     auto ab_slot = [&](uint32_t l) { return J.ab + 2ull * l; };
@@ -2764,13 +2787,20 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     const BatchJob *djobs = reinterpret_cast<const BatchJob *>(bs.meta);
     if (!lockstep) {
         const uint32_t *dorder = reinterpret_cast<const uint32_t *>(bs.meta + bs.o_order);
+        if (!bs.queue) GKRX_HIP_CHECK(hipMalloc(&bs.queue, 256));
+        // two workgroups per CU stay resident (768 threads each) and drain the queue
+        int dev = 0, cus = 0;
+        GKRX_HIP_CHECK(hipGetDevice(&dev));
+        GKRX_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(nj, 2ull * (uint64_t)std::max(cus, 1));
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, bs.queue);
         timed(pf, kKBatch, st, [&] {
             if (L == 8)
-                hipLaunchKernelGGL(k_prove_batch<8>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder,
-                                   R, hash_kind);
+                hipLaunchKernelGGL(k_prove_batch<8>, dim3(grid), dim3(kBatchThreads), 0, st, djobs, dorder,
+                                   (uint32_t)nj, bs.queue, R, hash_kind);
             else
-                hipLaunchKernelGGL(k_prove_batch<16>, dim3((uint32_t)nj), dim3(kBatchThreads), 0, st, djobs, dorder,
-                                   R, hash_kind);
+                hipLaunchKernelGGL(k_prove_batch<16>, dim3(grid), dim3(kBatchThreads), 0, st, djobs, dorder,
+                                   (uint32_t)nj, bs.queue, R, hash_kind);
         });
     } else {
         const LsTask *dtasks = reinterpret_cast<const LsTask *>(bs.meta + bs.o_tasks);
@@ -2944,6 +2974,7 @@ void dev_free_batch(BatchScratch &bs)
         if (e) (void)hipEventDestroy(e);
     if (bs.pool) (void)hipFree(bs.pool);
     if (bs.meta) (void)hipFree(bs.meta);
+    if (bs.queue) (void)hipFree(bs.queue);
     if (bs.host_meta) (void)hipHostFree(bs.host_meta);
     if (bs.claims_host) (void)hipHostFree(bs.claims_host);
     if (bs.tails_host) (void)hipHostFree(bs.tails_host);
