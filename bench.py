@@ -392,8 +392,8 @@ def main():
                     help="also run device copies of this size (for scaling rocprofv3 --pmc byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (proofs shard by proof id "
-              "with no communication; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path here (independent proofs would "
+              "shard by proof id, one process per GPU, with no communication)", file=sys.stderr)
         sys.exit(2)
 
     import gkrx
