@@ -167,7 +167,7 @@ struct DevScratch {
     // every launch), replayed while the circuit, R, hash and transcript offsets are the same
     hipGraphExec_t graph = nullptr;
     uint64_t graph_serial = 0, graph_len = 0, graph_hs = 0;
-    uint32_t graph_R = 0;
+    uint32_t graph_R = 0, graph_pre = 0;  // graph_pre: bytes of the tail prefix it copies
     int graph_kind = -1;
 };
 

@@ -2046,7 +2046,7 @@ int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, co
     };
     if (!graph || pf) return enqueue();
     const bool same = s.graph && s.graph_serial == dc.serial && s.graph_R == R && s.graph_kind == hash_kind &&
-                      s.graph_len == tstate_host[0] && s.graph_hs == tstate_host[1];
+                      s.graph_len == tstate_host[0] && s.graph_hs == tstate_host[1] && s.graph_pre == tail_prefix_len;
     if (!same) {
         // capture the sequence once (nothing runs while capturing), then keep the executable
         // graph; everything it touches (scratch, circuit tables, stage) stays at its address
@@ -2075,6 +2075,7 @@ int dev_prove(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, co
         s.graph_kind = hash_kind;
         s.graph_len = tstate_host[0];
         s.graph_hs = tstate_host[1];
+        s.graph_pre = tail_prefix_len;
     }
     GKRX_HIP_CHECK(hipGraphLaunch(s.graph, st));
     return GKRX_OK;
