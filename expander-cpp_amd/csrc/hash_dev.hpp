@@ -27,6 +27,12 @@ __constant__ static const uint32_t kSha256K[64] = {
     0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
     0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
 
+// a ^ b ^ c in one gfx950 v_bitop3_b32 (truth table 0x96)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // One SHA-256 compression of a block given as 16 big-endian words.
 __device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16])
 {
@@ -38,15 +44,15 @@ __device__ __forceinline__ void sha256_compress(uint32_t st[8], uint32_t w[16])
             wt = w[t];
         } else {
             uint32_t w15 = w[(t - 15) & 15], w2 = w[(t - 2) & 15];
-            uint32_t s0 = rotr32d(w15, 7) ^ rotr32d(w15, 18) ^ (w15 >> 3);
-            uint32_t s1 = rotr32d(w2, 17) ^ rotr32d(w2, 19) ^ (w2 >> 10);
+            uint32_t s0 = xor3(rotr32d(w15, 7), rotr32d(w15, 18), w15 >> 3);
+            uint32_t s1 = xor3(rotr32d(w2, 17), rotr32d(w2, 19), w2 >> 10);
             wt = w[t & 15] + s0 + w[(t - 7) & 15] + s1;
             w[t & 15] = wt;
         }
-        uint32_t S1 = rotr32d(e, 6) ^ rotr32d(e, 11) ^ rotr32d(e, 25);
+        uint32_t S1 = xor3(rotr32d(e, 6), rotr32d(e, 11), rotr32d(e, 25));
         uint32_t ch = (e & f) ^ (~e & g);
         uint32_t t1 = h + S1 + ch + kSha256K[t] + wt;
-        uint32_t S0 = rotr32d(a, 2) ^ rotr32d(a, 13) ^ rotr32d(a, 22);
+        uint32_t S0 = xor3(rotr32d(a, 2), rotr32d(a, 13), rotr32d(a, 22));
         uint32_t mj = (a & b) ^ (a & c) ^ (b & c);
         h = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + S0 + mj;
     }
