@@ -134,3 +134,29 @@ def test_profile_classes_times_only_the_selected_class(contexts):
     for cl, pr in (every, masked):
         assert np.array_equal(cl, plain[0]) and bytes(pr) == bytes(plain[1])
     circ.close()
+
+
+@pytest.mark.parametrize("lanes,fs", [(16, gkrx.FS_KECCAK256), (8, gkrx.FS_SHA256)])
+def test_heavy_gx_rows_match_oracle(lanes, fs, tmp_path, contexts):
+    """A g_x row with many entries (300 gates share first input 7 in every layer) puts many
+    LDS atomic adds on one row of one tile; the proof must still equal the oracle's."""
+    _say(f"[gpu] heavy g_x rows L={lanes} hash={fs}")
+    layers = gkrx.synth_circuit(51, 4, 11, 600, True)
+    heavy = []
+    for mul, add in layers:
+        mul = np.array(mul, copy=True).reshape(-1, 4)
+        add = np.array(add, copy=True).reshape(-1, 3)
+        mul[:150, 0] = 7
+        add[:150, 0] = 7
+        heavy.append((mul, add))
+    mp, ap = str(tmp_path / "mul.txt"), str(tmp_path / "add.txt")
+    gkrx.write_text(heavy, mp, ap)
+    wit = gkrx.synth_witness(51, 1 << 11, lanes)
+    ctx = _ctx(contexts, lanes, fs)
+    circ = gkrx.Circuit.load_extracted_gates(ctx, mp, ap)
+    circ.set_witness(wit)
+    claimed, proof = gkrx.Prover(ctx).prove(circ)
+    oclaimed, oproof = ol.OracleCircuit(mp, ap).prove(lanes, fs, wit)
+    assert np.array_equal(claimed, oclaimed)
+    assert bytes(proof) == oproof
+    circ.close()
