@@ -7,6 +7,7 @@
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -205,7 +206,7 @@ static int finish_circuit(gkrx_ctx *ctx, gkrx_circuit *c)
 extern "C" int gkrx_circuit_from_arrays(gkrx_ctx *ctx, uint32_t nb_layers, const uint32_t *n_mul,
                                         const uint32_t *const *mul, const uint32_t *n_add,
                                         const uint32_t *const *add, gkrx_circuit **out)
-{
+try {
     if (!ctx || !out || nb_layers == 0 || !n_mul || !mul || !n_add || !add) return GKRX_ERR_ARG;
     gkrx_circuit *c = new (std::nothrow) gkrx_circuit;
     if (!c) return GKRX_ERR_NOMEM;
@@ -224,12 +225,54 @@ extern "C" int gkrx_circuit_from_arrays(gkrx_ctx *ctx, uint32_t nb_layers, const
     }
     *out = c;
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
-// Circuit::load_extracted_gates (include/circuit/circuit.hpp:152-215)
-extern "C" int gkrx_circuit_load_text(gkrx_ctx *ctx, const char *mul_path, const char *add_path, gkrx_circuit **out)
+// Bytes from the current position of f to its end (0 when the size is unknown).
+static uint64_t bytes_left(FILE *f)
 {
-    if (!ctx || !mul_path || !add_path || !out) return GKRX_ERR_ARG;
+    const long here = ftell(f);
+    if (here < 0 || fseek(f, 0, SEEK_END) != 0) return 0;
+    const long end = ftell(f);
+    (void)fseek(f, here, SEEK_SET);
+    return end > here ? (uint64_t)(end - here) : 0;
+}
+
+// One layer line of the text format: `count` then count x `words` numbers.  The gate list
+// grows as gates parse, and the count is checked against what the rest of the file can
+// hold (a number and its separator take at least two bytes), so a corrupt count can
+// neither wrap a size computation nor reserve memory the file cannot fill.
+static const char *read_gate_line(FILE *f, uint32_t words, std::vector<uint32_t> &out)
+{
+    unsigned long long ng;
+    if (fscanf(f, "%llu", &ng) != 1) return "malformed gate count";
+    if (ng > 0xffffffffull) return "gate count above 2^32";
+    const uint64_t room = bytes_left(f);
+    if (ng > room / (2ull * words - 1) + 1) return "gate count larger than the file can hold";
+    out.clear();
+    out.reserve((size_t)ng * words);
+    for (unsigned long long g = 0; g < ng; g++) {
+        unsigned long v[4];
+        unsigned coef;
+        if (words == 4) {
+            if (fscanf(f, "%lu%lu%lu%u", &v[0], &v[1], &v[2], &coef) != 4) return "malformed mul gate";
+            if ((v[0] | v[1] | v[2]) >> 32) return "gate index above 2^32";
+        } else {
+            if (fscanf(f, "%lu%lu%u", &v[0], &v[1], &coef) != 3) return "malformed add gate";
+            if ((v[0] | v[1]) >> 32) return "gate index above 2^32";
+        }
+        for (uint32_t w = 0; w + 1 < words; w++) out.push_back((uint32_t)v[w]);
+        out.push_back(coef);
+    }
+    return nullptr;
+}
+
+// Parses the two files of the text format into c (D, mul, add); no validation of the
+// indices, no device work.
+static int parse_text(const char *mul_path, const char *add_path, gkrx_circuit &c)
+{
     FILE *f = fopen(mul_path, "r");
     if (!f) {
         set_error(std::string("cannot open ") + mul_path);
@@ -248,72 +291,89 @@ extern "C" int gkrx_circuit_load_text(gkrx_ctx *ctx, const char *mul_path, const
         set_error("empty mul file");
         return GKRX_ERR_CIRCUIT;
     }
-    gkrx_circuit *c = new (std::nothrow) gkrx_circuit;
-    if (!c) {
-        fclose(f);
-        return GKRX_ERR_NOMEM;
-    }
-    c->D = nb;
-    c->mul.resize(nb);
-    c->add.resize(nb);
-    auto fail = [&](const char *msg) {
-        if (f) fclose(f);
-        delete c;
-        set_error(msg);
-        return GKRX_ERR_CIRCUIT;
-    };
+    c.D = nb;
+    c.mul.assign(nb, {});
+    c.add.assign(nb, {});
     for (uint32_t i = 0; i < nb; i++) {
-        auto &mu = c->mul[nb - 1 - i];  // line i -> layers[D - 1 - i]
-        unsigned long ng;
-        if (fscanf(f, "%lu", &ng) != 1) return fail("malformed mul file");
-        mu.resize(4 * ng);
-        for (unsigned long g = 0; g < ng; g++) {
-            unsigned long i0, i1, o;
-            unsigned coef;
-            if (fscanf(f, "%lu%lu%lu%u", &i0, &i1, &o, &coef) != 4) return fail("malformed mul gate");
-            if ((i0 | i1 | o) >> 32) return fail("gate index above 2^32");
-            mu[4 * g] = (uint32_t)i0;
-            mu[4 * g + 1] = (uint32_t)i1;
-            mu[4 * g + 2] = (uint32_t)o;
-            mu[4 * g + 3] = coef;
+        // line i -> layers[D - 1 - i]
+        if (const char *msg = read_gate_line(f, 4, c.mul[nb - 1 - i])) {
+            fclose(f);
+            set_error(std::string("mul file: ") + msg);
+            return GKRX_ERR_CIRCUIT;
         }
     }
     fclose(f);
     f = fopen(add_path, "r");
     if (!f) {
-        delete c;
         set_error(std::string("cannot open ") + add_path);
         return GKRX_ERR_IO;
     }
     for (uint32_t i = 0; i < nb; i++) {
-        auto &ad = c->add[nb - 1 - i];
-        unsigned long ng;
-        if (fscanf(f, "%lu", &ng) != 1) return fail("malformed add file");
-        ad.resize(3 * ng);
-        for (unsigned long g = 0; g < ng; g++) {
-            unsigned long i0, o;
-            unsigned coef;
-            if (fscanf(f, "%lu%lu%u", &i0, &o, &coef) != 3) return fail("malformed add gate");
-            if ((i0 | o) >> 32) return fail("gate index above 2^32");
-            ad[3 * g] = (uint32_t)i0;
-            ad[3 * g + 1] = (uint32_t)o;
-            ad[3 * g + 2] = coef;
+        if (const char *msg = read_gate_line(f, 3, c.add[nb - 1 - i])) {
+            fclose(f);
+            set_error(std::string("add file: ") + msg);
+            return GKRX_ERR_CIRCUIT;
         }
     }
     fclose(f);
-    f = nullptr;
-    int rc = finish_circuit(ctx, c);
-    if (rc) {
-        dev_free_circuit(c->dev);
-        delete c;
-        return rc;
-    }
-    *out = c;
     return GKRX_OK;
 }
 
-extern "C" int gkrx_circuit_load_circuit2(gkrx_ctx *ctx, const char *path, int coef_mode, gkrx_circuit **out)
+static int load_text_impl(gkrx_ctx *ctx, const char *mul_path, const char *add_path, gkrx_circuit **out)
 {
+    std::unique_ptr<gkrx_circuit> c(new gkrx_circuit);
+    int rc = parse_text(mul_path, add_path, *c);
+    if (rc) return rc;
+    rc = finish_circuit(ctx, c.get());
+    if (rc) {
+        dev_free_circuit(c->dev);
+        return rc;
+    }
+    *out = c.release();
+    return GKRX_OK;
+}
+
+// Circuit::load_extracted_gates (include/circuit/circuit.hpp:152-215)
+extern "C" int gkrx_circuit_load_text(gkrx_ctx *ctx, const char *mul_path, const char *add_path, gkrx_circuit **out)
+{
+    if (!ctx || !mul_path || !add_path || !out) return GKRX_ERR_ARG;
+    try {
+        return load_text_impl(ctx, mul_path, add_path, out);
+    } catch (const std::bad_alloc &) {
+        set_error("out of host memory while loading the circuit");
+        return GKRX_ERR_NOMEM;
+    }
+}
+
+extern "C" int gkrx_circuit_parse_text(const char *mul_path, const char *add_path, int field_type,
+                                       uint32_t *nb_layers, uint32_t *nb_in_vars, uint32_t *nb_out_vars)
+{
+    if (!mul_path || !add_path || !nb_layers) return GKRX_ERR_ARG;
+    if (field_type != GKRX_FIELD_M31 && field_type != GKRX_FIELD_BN254) return GKRX_ERR_CONFIG;
+    try {
+        gkrx_circuit c;
+        int rc = parse_text(mul_path, add_path, c);
+        if (!rc) rc = validate_circuit(field_type == GKRX_FIELD_BN254, &c);
+        if (rc) return rc;
+        const uint32_t cap = *nb_layers;
+        *nb_layers = c.D;
+        if (c.D > cap && (nb_in_vars || nb_out_vars)) {
+            set_error("shape arrays hold fewer entries than the circuit has layers");
+            return GKRX_ERR_ARG;
+        }
+        for (uint32_t l = 0; l < c.D; l++) {
+            if (nb_in_vars) nb_in_vars[l] = c.n[l];
+            if (nb_out_vars) nb_out_vars[l] = c.m[l];
+        }
+        return GKRX_OK;
+    } catch (const std::bad_alloc &) {
+        set_error("out of host memory while reading the circuit");
+        return GKRX_ERR_NOMEM;
+    }
+}
+
+extern "C" int gkrx_circuit_load_circuit2(gkrx_ctx *ctx, const char *path, int coef_mode, gkrx_circuit **out)
+try {
     if (!ctx || !path || !out) return GKRX_ERR_ARG;
     auto *c = new (std::nothrow) gkrx_circuit;
     if (!c) return GKRX_ERR_NOMEM;
@@ -329,6 +389,9 @@ extern "C" int gkrx_circuit_load_circuit2(gkrx_ctx *ctx, const char *path, int c
     }
     *out = c;
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 extern "C" int gkrx_circuit_shape(const gkrx_circuit *c, uint32_t *nb_layers, uint32_t *nb_in_vars,
@@ -344,7 +407,7 @@ extern "C" int gkrx_circuit_shape(const gkrx_circuit *c, uint32_t *nb_layers, ui
 }
 
 extern "C" int gkrx_circuit_write_text(const gkrx_circuit *c, const char *mul_path, const char *add_path)
-{
+try {
     if (!c || !mul_path || !add_path) return GKRX_ERR_ARG;
     for (int which = 0; which < 2; which++) {
         FILE *f = fopen(which == 0 ? mul_path : add_path, "w");
@@ -366,10 +429,13 @@ extern "C" int gkrx_circuit_write_text(const gkrx_circuit *c, const char *mul_pa
         fclose(f);
     }
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 extern "C" int gkrx_circuit_set_witness(gkrx_circuit *c, const uint32_t *lanes, size_t n_u32)
-{
+try {
     if (!c || !lanes) return GKRX_ERR_ARG;
     if (c->in_server) {
         set_error("circuit has a server request in flight");
@@ -407,6 +473,9 @@ extern "C" int gkrx_circuit_set_witness(gkrx_circuit *c, const uint32_t *lanes, 
     c->has_witness = true;
     c->evaluated = false;
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 static int evaluate_locked(gkrx_circuit *c)
@@ -535,7 +604,7 @@ static int prove_bn254_locked(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, ui
 
 extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap, size_t *proof_len,
                           uint32_t *claimed_v)
-{
+try {
     if (!ctx || !c || !proof || !claimed_v || c->ctx != ctx) return GKRX_ERR_ARG;
     if (c->in_server) {
         set_error("circuit has a server request in flight");
@@ -593,6 +662,9 @@ extern "C" int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t
     ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 namespace {
@@ -714,7 +786,7 @@ class PrefixRunner final : public BatchPrefixes {
 
 extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, uint8_t *const *proofs,
                                 const size_t *proof_caps, size_t *proof_lens, uint32_t *const *claimed_v)
-{
+try {
     if (!ctx || (n && (!circuits || !proofs || !proof_caps || !proof_lens || !claimed_v))) return GKRX_ERR_ARG;
     for (uint32_t i = 0; i < n; i++) {
         gkrx_circuit *c = circuits[i];
@@ -785,6 +857,9 @@ extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *
     ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 extern "C" int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block)
@@ -853,7 +928,7 @@ extern "C" int gkrx_ctx_get_batch_lanes(const gkrx_ctx *ctx, uint32_t *lanes)
 
 extern "C" int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, int *engine,
                                    uint32_t *block, uint32_t *lanes, double *ms_each)
-{
+try {
     if (!ctx || n == 0 || !circuits) return GKRX_ERR_ARG;
     if (ctx->bn254) {
         set_error("the batch engines prove M31 circuits");
@@ -924,22 +999,28 @@ extern "C" int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *cons
     if (block) *block = pick.block;
     if (lanes) *lanes = pick.lanes;
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 extern "C" int gkrx_verify(gkrx_ctx *ctx, const gkrx_circuit *c, const uint32_t *claimed_v, const uint8_t *proof,
                            size_t proof_len)
-{
+try {
     if (!ctx || !c || !claimed_v || !proof) return GKRX_ERR_ARG;
     HostCircuitView v{c->D, c->n.data(), c->m.data(), c->mul, c->add};
     const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
     if (ctx->bn254) return host_verify_bn254(v, kind, (uint32_t)ctx->cfg.grinding_bits, claimed_v, proof, proof_len);
     return host_verify(v, ctx->L, ctx->R, kind, (uint32_t)ctx->cfg.grinding_bits, claimed_v, proof, proof_len);
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 extern "C" int gkrx_verify_arrays(const gkrx_config *cfg, uint32_t nb_layers, const uint32_t *n_mul,
                                   const uint32_t *const *mul, const uint32_t *n_add, const uint32_t *const *add,
                                   const uint32_t *claimed_v, const uint8_t *proof, size_t proof_len)
-{
+try {
     if (!cfg || nb_layers == 0 || !n_mul || !mul || !n_add || !add || !claimed_v || !proof) return GKRX_ERR_ARG;
     const bool bn = cfg->field_type == GKRX_FIELD_BN254;
     if ((cfg->field_type != GKRX_FIELD_M31 && !bn) || (cfg->fs_hash != GKRX_FS_SHA256 && cfg->fs_hash != GKRX_FS_KECCAK256) ||
@@ -964,6 +1045,9 @@ extern "C" int gkrx_verify_arrays(const gkrx_config *cfg, uint32_t nb_layers, co
     if (R < 1 || R > 3) return GKRX_ERR_CONFIG;
     return host_verify(v, (uint32_t)cfg->nb_parallel, (uint32_t)R, kind, (uint32_t)cfg->grinding_bits, claimed_v,
                        proof, proof_len);
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 // Proof::write_and_compress / read_proof file layout (include/fiat_shamir/transcript.hpp:54-85)
@@ -1072,7 +1156,7 @@ extern "C" int gkrx_hash(int fs_hash, const uint8_t *in, size_t len, uint8_t out
 }
 
 extern "C" int gkrx_hash_many(int fs_hash, uint32_t n, const uint8_t *const *in, size_t len, uint8_t *out)
-{
+try {
     if ((n && (!in || !out))) return GKRX_ERR_ARG;
     for (uint32_t i = 0; i < n; i++)
         if (!in[i] && len) return GKRX_ERR_ARG;
@@ -1083,4 +1167,7 @@ extern "C" int gkrx_hash_many(int fs_hash, uint32_t n, const uint8_t *const *in,
     if (fs_hash != GKRX_FS_SHA256) return GKRX_ERR_CONFIG;
     for (uint32_t i = 0; i < n; i++) host_sha256(in[i], len, out + 32 * (size_t)i);
     return 0;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -284,7 +285,7 @@ int circuit2_flatten(const char *path, int field, int mode, std::vector<std::vec
 
 extern "C" int gkrx_circuit2_to_text(const char *path, int field_type, int coef_mode, const char *mul_path,
                                      const char *add_path)
-{
+try {
     if (!path || !mul_path || !add_path) return GKRX_ERR_ARG;
     if (field_type != GKRX_FIELD_M31 && field_type != GKRX_FIELD_BN254) return GKRX_ERR_CONFIG;
     std::vector<std::vector<uint32_t>> mul, add;
@@ -307,4 +308,7 @@ extern "C" int gkrx_circuit2_to_text(const char *path, int field_type, int coef_
         fclose(f);
     }
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    gkrx::set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }

@@ -32,6 +32,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -351,7 +352,7 @@ void free_slot(Slot &sl)
 }  // namespace
 
 extern "C" int gkrx_server_create(gkrx_ctx *ctx, uint64_t hbm_budget_bytes, uint32_t slots, gkrx_server **out)
-{
+try {
     if (!ctx || !out || hbm_budget_bytes == 0 || slots == 0 || slots > 16) return GKRX_ERR_ARG;
     GKRX_HIP_CHECK(hipSetDevice(ctx->device));
     auto *S = new (std::nothrow) gkrx_server;
@@ -389,11 +390,14 @@ extern "C" int gkrx_server_create(gkrx_ctx *ctx, uint64_t hbm_budget_bytes, uint
     S->sched = std::thread(scheduler, S);
     *out = S;
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 extern "C" int gkrx_server_submit(gkrx_server *S, gkrx_circuit *c, uint8_t *proof, size_t proof_cap,
                                   uint32_t *claimed_v, uint64_t *ticket)
-{
+try {
     if (!S || !c || !proof || !claimed_v || !ticket || c->ctx != S->ctx) return GKRX_ERR_ARG;
     gkrx_ctx *ctx = S->ctx;
     if (!c->has_witness) {
@@ -447,6 +451,9 @@ extern "C" int gkrx_server_submit(gkrx_server *S, gkrx_circuit *c, uint8_t *proo
     }
     S->pcv.notify_one();
     return GKRX_OK;
+} catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return GKRX_ERR_NOMEM;
 }
 
 extern "C" int gkrx_server_wait(gkrx_server *S, uint64_t ticket, int64_t timeout_us, size_t *proof_len,

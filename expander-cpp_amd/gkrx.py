@@ -53,6 +53,7 @@ def lib():
     L.gkrx_ctx_create.argtypes = [C.POINTER(_Config), C.c_int, C.POINTER(vp)]
     L.gkrx_ctx_destroy.argtypes = [vp]
     L.gkrx_circuit_load_text.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(vp)]
+    L.gkrx_circuit_parse_text.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_uint32), vp, vp]
     L.gkrx_circuit_from_arrays.argtypes = [vp, C.c_uint32, _u32p, C.POINTER(C.c_void_p), _u32p,
                                            C.POINTER(C.c_void_p), C.POINTER(vp)]
     L.gkrx_circuit_shape.argtypes = [vp, C.POINTER(C.c_uint32), vp, vp]
@@ -199,6 +200,19 @@ def circuit2_to_text(path: str, mul_path: str, add_path: str, field_type: int = 
     """Flatten a CIRCUIT2 file into the text format with no GPU (gkrx_circuit2_to_text)."""
     _check(lib().gkrx_circuit2_to_text(path.encode(), field_type, coef_mode, mul_path.encode(), add_path.encode()),
            "gkrx_circuit2_to_text")
+
+
+def parse_text_shape(mul_path: str, add_path: str, field_type: int = FIELD_M31):
+    """Host-only read and check of a text-format circuit (gkrx_circuit_parse_text): returns
+    (nb_input_vars, nb_output_vars) per layer, layer 0 first."""
+    n = C.c_uint32(0)
+    _check(lib().gkrx_circuit_parse_text(mul_path.encode(), add_path.encode(), field_type, C.byref(n), None, None),
+           "gkrx_circuit_parse_text")
+    ni = np.zeros(n.value, dtype=np.uint32)
+    no = np.zeros(n.value, dtype=np.uint32)
+    _check(lib().gkrx_circuit_parse_text(mul_path.encode(), add_path.encode(), field_type, C.byref(n),
+                                         ni.ctypes.data, no.ctypes.data), "gkrx_circuit_parse_text")
+    return ni.tolist(), no.tolist()
 
 
 def read_text(mul_path: str, add_path: str):

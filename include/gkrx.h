@@ -65,6 +65,13 @@ int gkrx_ctx_destroy(gkrx_ctx *ctx);
 /* Circuit::load_extracted_gates (include/circuit/circuit.hpp:152-215): the text
  * format (mul file one line per layer, output layer first). */
 int gkrx_circuit_load_text(gkrx_ctx *ctx, const char *mul_path, const char *add_path, gkrx_circuit **out);
+/* The same reader and checks with no context and no device: reads and validates the two
+ * files on the host and reports the shape (Circuit::_compute_nb_vars, circuit.hpp:124-149),
+ * from which proof size and device memory follow.  *nb_layers: capacity of nb_in_vars /
+ * nb_out_vars on entry (either may be NULL), the layer count on return.  field_type is
+ * GKRX_FIELD_M31 (coefficients must fold into the field) or GKRX_FIELD_BN254. */
+int gkrx_circuit_parse_text(const char *mul_path, const char *add_path, int field_type, uint32_t *nb_layers,
+                            uint32_t *nb_in_vars, uint32_t *nb_out_vars);
 /* Circuit built from gate arrays, layer 0 = input layer as in Circuit::layers.
  * mul[l]: n_mul[l] x (in0, in1, out, coef); add[l]: n_add[l] x (in0, out, coef).
  * Variable counts follow Circuit::_compute_nb_vars (circuit.hpp:124-149). */
