@@ -843,7 +843,8 @@ try {
         memcpy(proofs[i], circuits[i]->witness.data(), ((size_t)1 << circuits[i]->n[0]) * S);
     });
     GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    dev_batch_collect(items, ctx->batch);
+    rc = dev_batch_collect(items, ctx->batch);
+    if (rc) return rc;
     const uint32_t *cl = ctx->batch.claims_host;
     if (!cl) return GKRX_ERR_NOMEM;
     for (uint32_t i = 0; i < n; i++) {

@@ -116,8 +116,9 @@ struct BatchScratch {
     std::vector<uint32_t> meta_steps;
     std::vector<uint32_t> meta_seed_step;   // lockstep: the step whose work seeds job i's transcript
     std::vector<uint32_t> meta_seed_order;  // the jobs in the order of their seed slots
-    uint32_t *claims_host = nullptr;
+    uint32_t *claims_host = nullptr;  // claims (item-major, R * L each), then each item's transcript length
     uint64_t claims_cap = 0;
+    uint64_t claims_words = 0;        // claim words of the last batch
     // pinned staging of the proof tails: the device copies land here (a copy into pageable
     // memory would block the launching thread), dev_batch_collect hands them out
     uint8_t *tails_host = nullptr;
@@ -225,14 +226,15 @@ struct BatchPrefixes {
 };
 
 // Proves every item with the engine bs.engine selects.  Enqueues the copies of the
-// tails and of the claims (item-major, R * L each, to bs.host_claims()) into pinned
-// staging; once the stream has reached them, dev_batch_collect copies each tail to
-// items[i].tail_out.  With `pre`, the items' prefixes may still be in progress: each is
+// tails, of the claims (item-major, R * L each, to bs.host_claims()) and of each item's
+// final transcript length into pinned staging; once the stream has reached them,
+// dev_batch_collect checks the lengths against the proof layout (GKRX_ERR_DEVICE if one
+// differs) and copies each tail to items[i].tail_out.  With `pre`, the items' prefixes may still be in progress: each is
 // awaited just before the first step that needs it (the lockstep engine seeds a proof's
 // transcript after its forward pass), otherwise they must be ready.
 int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_kind, BatchScratch &bs, hipStream_t st,
                     Profiler *pf, BatchPrefixes *pre = nullptr);
-void dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &bs);
+int dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &bs);
 // Device bytes the batch prover takes for one proof of dc (its tables and transcript).
 uint64_t dev_batch_job_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes);
 // Device bytes of a circuit's resident values and wiring.

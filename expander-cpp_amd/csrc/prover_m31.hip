@@ -33,6 +33,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "hash_dev.hpp"
 #include "internal.hpp"
@@ -1272,6 +1274,14 @@ __global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *_
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
+// The transcript length each job's device transcript reached (tstate[0]), gathered so that
+// one copy brings every job's back to the host.
+__global__ void k_batch_tlen(const BatchJob *__restrict__ jobs, uint32_t nj, uint32_t *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nj) out[i] = jobs[i].tstate[0];
+}
+
 template <int L>
 __device__ void prove_batch_job(const BatchJob &J, uint32_t R, int hash_kind, unsigned long long *tacc)
 {
@@ -1600,28 +1610,33 @@ __global__ void __launch_bounds__(BS) k_ls_pair(const LsTask *__restrict__ tasks
     }
 }
 
+// e0 / e1: when not null, the launch records the kernel's own start and end in them
 template <int L, int BS>
 void ls_launch_pair_v(uint32_t V, uint32_t blocks, hipStream_t st, const LsTask *tasks, const uint32_t *start,
-                      uint32_t n)
+                      uint32_t n, hipEvent_t e0, hipEvent_t e1)
 {
     const dim3 g(blocks);
+    auto go = [&](auto kernel) {
+        if (e0) hipExtLaunchKernelGGL(kernel, g, dim3(BS), 0, st, e0, e1, 0, tasks, start, n);
+        else hipLaunchKernelGGL(kernel, g, dim3(BS), 0, st, tasks, start, n);
+    };
     switch (V) {
-    case 1: hipLaunchKernelGGL((k_ls_pair<L, BS, 1>), g, dim3(BS), 0, st, tasks, start, n); break;
-    case 2: hipLaunchKernelGGL((k_ls_pair<L, BS, 2>), g, dim3(BS), 0, st, tasks, start, n); break;
-    default: hipLaunchKernelGGL((k_ls_pair<L, BS, 4>), g, dim3(BS), 0, st, tasks, start, n); break;
+    case 1: go(k_ls_pair<L, BS, 1>); break;
+    case 2: go(k_ls_pair<L, BS, 2>); break;
+    default: go(k_ls_pair<L, BS, 4>); break;
     }
 }
 
 // k_ls_pair for block size BS in {64, 128, 256, 512} and V in {1, 2, 4} lanes per thread
 template <int L>
 void ls_launch_pair(uint32_t BS, uint32_t V, uint32_t blocks, hipStream_t st, const LsTask *tasks,
-                    const uint32_t *start, uint32_t n)
+                    const uint32_t *start, uint32_t n, hipEvent_t e0, hipEvent_t e1)
 {
     switch (BS) {
-    case 64: ls_launch_pair_v<L, 64>(V, blocks, st, tasks, start, n); break;
-    case 128: ls_launch_pair_v<L, 128>(V, blocks, st, tasks, start, n); break;
-    case 512: ls_launch_pair_v<L, 512>(V, blocks, st, tasks, start, n); break;
-    default: ls_launch_pair_v<L, 256>(V, blocks, st, tasks, start, n); break;
+    case 64: ls_launch_pair_v<L, 64>(V, blocks, st, tasks, start, n, e0, e1); break;
+    case 128: ls_launch_pair_v<L, 128>(V, blocks, st, tasks, start, n, e0, e1); break;
+    case 512: ls_launch_pair_v<L, 512>(V, blocks, st, tasks, start, n, e0, e1); break;
+    default: ls_launch_pair_v<L, 256>(V, blocks, st, tasks, start, n, e0, e1); break;
     }
 }
 
@@ -2178,12 +2193,12 @@ void dev_free_circuit(DevCircuit &dc)
     dc.d_layers = nullptr;
 }
 
-int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t tail_bytes)
+namespace {
+
+// Carves the prover tables of circuits up to dc's shape out of `base` (nullptr: only
+// measures) and returns their size in u32; s's table pointers are set when base is given.
+uint64_t scratch_layout(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t tail_bytes, uint32_t *base)
 {
-    if (s.pool && s.max_n >= dc.max_n && s.max_m >= dc.max_m && s.L == dc.L && s.D >= dc.D &&
-        s.tail_cap >= tail_bytes)
-        return GKRX_OK;
-    dev_free_scratch(s);
     const uint32_t L = dc.L;
     const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
     const uint64_t N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4), N1 = scratch_rows(maxN);
@@ -2194,22 +2209,35 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
         {&s.sA, R * N2}, {&s.sB, R * N4}, {&s.partials, (uint64_t)R * kMaxBlocksPerRep * kPairVals * L},
         {&s.chal, (uint64_t)(dc.D + 1) * R * kChalStride}, {&s.ab, 2ull * (dc.D + 1)}, {&s.claims, R * L},
         {&s.vxy, 2ull * R * L}, {&s.tstate, 16}, {&s.tail, (tail_bytes + 3) / 4 + 64}};
-    uint64_t total = 0;
-    for (auto &pc : pieces) total += (pc.u32 + 63) & ~63ull;
-    GKRX_HIP_CHECK(hipMalloc(&s.pool, total * 4));
     uint64_t at = 0;
     for (auto &pc : pieces) {
-        *pc.p = reinterpret_cast<uint32_t *>(s.pool) + at;
-        at += (pc.u32 + 63) & ~63ull;
+        if (base) *pc.p = base + at;
+        at += (pc.u32 + 63) & ~63ull;  // 256-byte aligned pieces
     }
+    if (base) {
+        s.bytes = at * 4;
+        s.max_n = dc.max_n;
+        s.max_m = dc.max_m;
+        s.L = L;
+        s.D = dc.D;
+        s.tail_cap = tail_bytes;
+    }
+    return at;
+}
+
+}  // namespace
+
+int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t tail_bytes)
+{
+    if (s.pool && s.max_n >= dc.max_n && s.max_m >= dc.max_m && s.L == dc.L && s.D >= dc.D &&
+        s.tail_cap >= tail_bytes)
+        return GKRX_OK;
+    dev_free_scratch(s);
+    const uint64_t total = scratch_layout(s, dc, R, tail_bytes, nullptr);
+    GKRX_HIP_CHECK(hipMalloc(&s.pool, total * 4));
+    scratch_layout(s, dc, R, tail_bytes, reinterpret_cast<uint32_t *>(s.pool));
     s.stage_u32 = 1024;
     GKRX_HIP_CHECK(hipHostMalloc(&s.stage, s.stage_u32 * 4, hipHostMallocDefault));
-    s.bytes = total * 4;
-    s.max_n = dc.max_n;
-    s.max_m = dc.max_m;
-    s.L = L;
-    s.D = dc.D;
-    s.tail_cap = tail_bytes;
     return GKRX_OK;
 }
 
@@ -2653,7 +2681,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         job_off[i + 1] = job_off[i] + batch_tables_u32(dc, R);
         tail_off[i + 1] = tail_off[i] + ((items[i].tail_bytes + 3) / 4 + 63) / 64 * 64;
     }
-    const uint64_t need = job_off[nj] + tail_off[nj] + nj * (uint64_t)R * L + nj * seed_u32 + 64;
+    const uint64_t need = job_off[nj] + tail_off[nj] + nj * (uint64_t)R * L + nj * seed_u32 + nj + 64;
     GKRX_HIP_CHECK(hipStreamSynchronize(st));  // the previous batch may still use the buffers
     if (need > bs.pool_u32) {
         if (bs.pool) (void)hipFree(bs.pool);
@@ -2676,6 +2704,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     // seed records (rz1 in chal-slot layout, transcript state, grind bytes), one per job in
     // the order in which the schedule needs them
     uint32_t *seeds = claims + nj * (uint64_t)R * L;
+    uint32_t *tlens = seeds + nj * seed_u32;  // each job's final transcript length
     // The job table and the schedule depend only on the circuits' shapes, their device
     // addresses and the pool: a batch with the same key reuses the metadata already on the
     // device.
@@ -2816,7 +2845,10 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         hipStream_t ls_st[3] = {st, st, st};
         hipEvent_t ls_ev[3] = {nullptr, nullptr, nullptr};
         uint64_t rec[3] = {0, 0, 0}, seen[3][3] = {};
-        const bool multi = bs.chain_stream && !pf;
+        // timing any class but the round pairs (whose launches record their own start and
+        // end) puts every lane on one stream and times each op class of the work alone
+        const bool serial = pf && (pf->mask & ~(1ull << kKLsPair)) != 0;
+        const bool multi = bs.chain_stream && !serial;
         if (multi) {
             if (!bs.aux) {
                 GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux, hipStreamNonBlocking));
@@ -2885,10 +2917,10 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                     const LsList &ls = S.l[kLsPairs];
                     hipStream_t ps = ls_st[1];
                     if (int rc = join(1)) return rc;
-                    timed(pf, kKLsPair, ps, [&] {
-                        if (L == 8) ls_launch_pair<8>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n);
-                        else ls_launch_pair<16>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n);
-                    });
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    if (pf && !pf->kernel_events(kKLsPair, e0, e1)) e0 = e1 = nullptr;
+                    if (L == 8) ls_launch_pair<8>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n, e0, e1);
+                    else ls_launch_pair<16>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n, e0, e1);
                     if (S.l[kLsPairFin].n) fin(S.l[kLsPairFin], ps, kKPairFin);
                     launched[1] = true;
                 }
@@ -2904,7 +2936,7 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                         if (L == 8) ls_launch_work<8>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
                         else ls_launch_work<16>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
                     };
-                    if (W.n && !pf) {
+                    if (W.n && !serial) {
                         work(W.blocks, 0);
                     } else if (W.n) {
                         // profiling: one launch per run of tasks of one op class
@@ -2935,19 +2967,31 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         if (int rc = join(0)) return rc;
         if (seeded != nj) return GKRX_ERR_STATE;  // every proof has a SEED task
     }
+    hipLaunchKernelGGL(k_batch_tlen, dim3((uint32_t)((nj + 255) / 256)), dim3(256), 0, st, djobs, (uint32_t)nj, tlens);
     GKRX_HIP_CHECK(hipGetLastError());
     GKRX_HIP_CHECK(hipMemcpyAsync(bs.tails_host, tails, tail_off[nj] * 4, hipMemcpyDeviceToHost, st));
-    uint32_t *hc = bs.host_claims(nj * R * L);
+    uint32_t *hc = bs.host_claims(nj * R * L + nj);
     if (!hc) return GKRX_ERR_NOMEM;
     GKRX_HIP_CHECK(hipMemcpyAsync(hc, claims, nj * (uint64_t)R * L * 4, hipMemcpyDeviceToHost, st));
+    GKRX_HIP_CHECK(hipMemcpyAsync(hc + nj * (uint64_t)R * L, tlens, nj * 4, hipMemcpyDeviceToHost, st));
+    bs.claims_words = nj * (uint64_t)R * L;
     return GKRX_OK;
 }
 
-void dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &bs)
+int dev_batch_collect(const std::vector<BatchItem> &items, const BatchScratch &bs)
 {
+    // every job's device transcript must have ended where the proof layout says
+    const uint32_t *tlen = bs.claims_host + bs.claims_words;
+    for (size_t i = 0; i < items.size(); i++)
+        if (tlen[i] != items[i].tail_bytes) {
+            set_error("batch item " + std::to_string(i) + ": device transcript length " + std::to_string(tlen[i]) +
+                      " != " + std::to_string(items[i].tail_bytes));
+            return GKRX_ERR_DEVICE;
+        }
     parallel_for(items.size(), 16, [&](size_t i) {
         memcpy(items[i].tail_out, bs.tails_host + bs.tail_stage_off[i] * 4, items[i].tail_bytes);
     });
+    return GKRX_OK;
 }
 
 uint64_t dev_batch_job_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes)

@@ -59,11 +59,13 @@ struct SrvReq {
     uint32_t *claims = nullptr;
     uint64_t bytes = 0;   // admission footprint
     bool single = false;  // runs on a single slot (BN254, or M31 outside the batch engine)
-    Clock::time_point t_submit;
+    // submitted; host prefix started and done; launched; completion seen on the device
+    Clock::time_point t_submit, t_prefix0, t_prefix1, t_launch, t_seen;
     int state = 0;        // 0 queued, 1 on the GPU, 2 done
     int rc = 0;
     size_t len = 0;
     double lat_ms = 0;
+    double split_ms[6] = {};  // see gkrx_server_wait_split
     std::vector<uint32_t> rz1;
     uint32_t tstate[10] = {};
     uint8_t grind[32] = {};
@@ -144,7 +146,9 @@ void prefix_worker(gkrx_server *S)
         S->pq.pop_front();
         S->prefixing++;
         lk.unlock();
+        r->t_prefix0 = Clock::now();
         compute_prefix(S->ctx, r);
+        r->t_prefix1 = Clock::now();
         lk.lock();
         S->prefixing--;
         S->ready.push_back(r);
@@ -229,19 +233,29 @@ int collect(gkrx_server *S, Slot &sl)
         memcpy(r->proof + commit_bytes(ctx, r->c), sl.tail_host, tb);
         memcpy(r->claims, sl.small_host, cw * 4);
     } else {
-        dev_batch_collect(sl.items, sl.bs);
+        if (int rc = dev_batch_collect(sl.items, sl.bs)) return rc;
         for (size_t i = 0; i < sl.reqs.size(); i++) memcpy(sl.reqs[i]->claims, sl.bs.claims_host + i * cw, cw * 4);
     }
     return GKRX_OK;
 }
 
-void finish_reqs(gkrx_server *S, std::vector<SrvReq *> &rs, int rc)
+double ms_between(Clock::time_point a, Clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// seen: when the slot's completion was observed (before its results were collected)
+void finish_reqs(gkrx_server *S, std::vector<SrvReq *> &rs, int rc, Clock::time_point seen)
 {
     const auto now = Clock::now();
     for (SrvReq *r : rs) {
         r->rc = rc;
         r->len = rc ? 0 : commit_bytes(S->ctx, r->c) + tail_bytes(S->ctx, r->c);
-        r->lat_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+        r->lat_ms = ms_between(r->t_submit, now);
+        r->t_seen = seen;
+        const Clock::time_point marks[6] = {r->t_submit, r->t_prefix0, r->t_prefix1, r->t_launch, seen, now};
+        for (int i = 0; i < 5; i++) r->split_ms[i] = std::max(0.0, ms_between(marks[i], marks[i + 1]));
+        r->split_ms[5] = r->lat_ms;
         r->state = 2;
         if (!rc) r->c->evaluated = true;
         r->c->in_server = false;
@@ -285,21 +299,27 @@ void scheduler(gkrx_server *S)
         bool progressed = false;
         if (!busy.empty()) {
             lk.unlock();
-            std::vector<std::pair<Slot *, int>> finished;
+            struct Done {
+                Slot *sl;
+                int rc;
+                Clock::time_point seen;
+            };
+            std::vector<Done> finished;
             for (Slot *sl : busy) {
                 const hipError_t q = hipEventQuery(sl->ev);
+                const auto seen = Clock::now();
                 if (q == hipSuccess) {
-                    finished.push_back({sl, collect(S, *sl)});
+                    finished.push_back({sl, collect(S, *sl), seen});
                 } else if (q != hipErrorNotReady) {
                     (void)hipStreamSynchronize(sl->st);
-                    finished.push_back({sl, GKRX_ERR_DEVICE});
+                    finished.push_back({sl, GKRX_ERR_DEVICE, seen});
                 }
             }
             lk.lock();
             for (auto &f : finished) {
-                finish_reqs(S, f.first->reqs, f.second);
-                f.first->reqs.clear();
-                f.first->busy = false;
+                finish_reqs(S, f.sl->reqs, f.rc, f.seen);
+                f.sl->reqs.clear();
+                f.sl->busy = false;
                 progressed = true;
             }
         }
@@ -316,12 +336,14 @@ void scheduler(gkrx_server *S)
             S->max_batch = std::max<uint64_t>(S->max_batch, batch.size());
             sl.busy = true;
             sl.reqs = batch;
+            const auto t_launch = Clock::now();
+            for (SrvReq *r : batch) r->t_launch = t_launch;
             lk.unlock();
             int rc = launch(S, sl);
             if (rc) (void)hipStreamSynchronize(sl.st);  // no completion event: drain the slot
             lk.lock();
             if (rc) {
-                finish_reqs(S, sl.reqs, rc);
+                finish_reqs(S, sl.reqs, rc, Clock::now());
                 sl.reqs.clear();
                 sl.busy = false;
                 S->done_cv.notify_all();
@@ -456,10 +478,10 @@ try {
     return GKRX_ERR_NOMEM;
 }
 
-extern "C" int gkrx_server_wait(gkrx_server *S, uint64_t ticket, int64_t timeout_us, size_t *proof_len,
-                                double *latency_ms)
+extern "C" int gkrx_server_wait_split(gkrx_server *S, uint64_t ticket, int64_t timeout_us, size_t *proof_len,
+                                      double split_ms[6])
 {
-    if (!S) return GKRX_ERR_ARG;
+    if (!S || !split_ms) return GKRX_ERR_ARG;
     std::unique_lock<std::mutex> lk(S->m);
     auto it = S->reqs.find(ticket);
     if (it == S->reqs.end()) {
@@ -472,9 +494,18 @@ extern "C" int gkrx_server_wait(gkrx_server *S, uint64_t ticket, int64_t timeout
     else if (!S->done_cv.wait_for(lk, std::chrono::microseconds(timeout_us), ready)) return 1;
     const int rc = r->rc;
     if (proof_len) *proof_len = r->len;
-    if (latency_ms) *latency_ms = r->lat_ms;
+    for (int i = 0; i < 6; i++) split_ms[i] = r->split_ms[i];
     S->reqs.erase(it);
     delete r;
+    return rc;
+}
+
+extern "C" int gkrx_server_wait(gkrx_server *S, uint64_t ticket, int64_t timeout_us, size_t *proof_len,
+                                double *latency_ms)
+{
+    double split[6] = {};
+    const int rc = gkrx_server_wait_split(S, ticket, timeout_us, proof_len, split);
+    if (rc != 1 && latency_ms) *latency_ms = split[5];
     return rc;
 }
 

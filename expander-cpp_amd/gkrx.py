@@ -85,6 +85,7 @@ def lib():
     L.gkrx_server_create.argtypes = [vp, C.c_uint64, C.c_uint32, C.POINTER(vp)]
     L.gkrx_server_submit.argtypes = [vp, vp, _u8p, C.c_size_t, _u32p, C.POINTER(C.c_uint64)]
     L.gkrx_server_wait.argtypes = [vp, C.c_uint64, C.c_int64, C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
+    L.gkrx_server_wait_split.argtypes = [vp, C.c_uint64, C.c_int64, C.POINTER(C.c_size_t), C.POINTER(C.c_double)]
     L.gkrx_server_stats.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.gkrx_server_destroy.argtypes = [vp]
     L.gkrx_ctx_profile.argtypes = [vp, C.c_int]
@@ -535,16 +536,22 @@ class Server:
         self._pending[t.value] = (circuit, buf, claims)
         return t.value
 
-    def wait(self, ticket: int, timeout_s: float | None = None):
-        """(claimed[R, words], proof, latency_ms), or None if still pending at the timeout."""
-        n, lat = C.c_size_t(), C.c_double()
+    SPLIT_KEYS = ("prefix_queue_ms", "prefix_ms", "admission_ms", "device_ms", "collect_ms", "total_ms")
+
+    def wait(self, ticket: int, timeout_s: float | None = None, split: bool = False):
+        """(claimed[R, words], proof, latency_ms), or None if still pending at the timeout.
+        With split, a fourth item: the request's time split (SPLIT_KEYS, ms)."""
+        n, sp = C.c_size_t(), (C.c_double * 6)()
         us = -1 if timeout_s is None else int(timeout_s * 1e6)
-        rc = lib().gkrx_server_wait(self._h, ticket, us, C.byref(n), C.byref(lat))
+        rc = lib().gkrx_server_wait_split(self._h, ticket, us, C.byref(n), sp)
         if rc == 1:
             return None
         circuit, buf, claims = self._pending.pop(ticket)
         _check(rc, "gkrx_server_wait")
-        return claims.reshape(self.ctx.config.get_num_repetitions(), -1), buf[:n.value], lat.value
+        out = (claims.reshape(self.ctx.config.get_num_repetitions(), -1), buf[:n.value], sp[5])
+        if split:
+            out += (dict(zip(self.SPLIT_KEYS, list(sp))),)
+        return out
 
     def stats(self) -> dict:
         out = (C.c_uint64 * 6)()

@@ -183,6 +183,12 @@ int gkrx_server_submit(gkrx_server *s, gkrx_circuit *c, uint8_t *proof, size_t p
  * (proof_len and latency_ms = submit to completion are set; the ticket is released),
  * 1 if it is still pending, < 0 if it failed. */
 int gkrx_server_wait(gkrx_server *s, uint64_t ticket, int64_t timeout_us, size_t *proof_len, double *latency_ms);
+/* gkrx_server_wait with the request's time split (ms): [0] waiting for a prefix thread,
+ * [1] host transcript prefix, [2] waiting for admission and a free slot, [3] on the GPU
+ * (launch until the slot's completion is seen), [4] results handed out, [5] total
+ * (submit to completion, the latency gkrx_server_wait reports). */
+int gkrx_server_wait_split(gkrx_server *s, uint64_t ticket, int64_t timeout_us, size_t *proof_len,
+                           double split_ms[6]);
 /* out: [0] submitted, [1] completed, [2] batches launched, [3] largest batch,
  * [4] peak admitted bytes, [5] admitted bytes now. */
 int gkrx_server_stats(gkrx_server *s, uint64_t out[6]);

@@ -11,6 +11,11 @@
 //   ref_driver prove2  <circuit2> <witness.bin> <proof.out> <claimed.out>
 //   ref_driver proof_write <raw.bin> <proof.out> <compressed.out>  (Proof::write_and_compress)
 //   ref_driver proof_read  <proof.in> <raw.out>                     (Proof::read_proof)
+//   ref_driver throughput <threads> <add_permille> <locality> <seed:depth:log_width>...
+//       the reference harness's throughput mode (benchmarks/bench_keccak.cpp:32-42, 82-92):
+//       threads take the listed synthetic circuits (built in memory by the library's seeded
+//       generator, expander-cpp_amd/csrc/synth.cpp) one at a time, evaluate them, then time
+//       prepare_mem + prove; prints "<gates> <max busy seconds of a thread> <wall seconds>"
 //
 // Build variants (oracle/Makefile):
 //   * as shipped: VectorizedM31 = 8 lanes, SHA-256 transcript (include/hash/hashes.hpp:22-31);
@@ -19,8 +24,11 @@
 //     reference's own transcript framing runs with the Keccak-256 hash;
 //   * 16 lanes: the recipe rewrites `vectorize_size = 1` to 2 (include/field/M31.hpp:237)
 //     in the preprocessed translation unit, in a pipe, without copying any source.
+#include <atomic>
 #include <chrono>
 #include <fstream>
+#include <string>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -41,6 +49,8 @@ class OracleKeccak256Ctx {
 #endif
 
 #include "LinearGKR/LinearGKR.hpp"
+
+#include "gkrx.h"  // gkrx_synth_layer, gkrx_synth_witness_m31 (synth.cpp, linked in)
 
 using F = gkr::M31_field::VectorizedM31;
 using FP = gkr::M31_field::M31;
@@ -106,10 +116,93 @@ static gkr::Circuit<F, FP> load_circuit2(const char *path)
     return gkr::Circuit<F, FP>::from_circuit_raw(raw);
 }
 
+// A synthetic circuit built straight into the reference's data structures (the gate lists
+// load_extracted_gates fills, circuit.hpp:152-215) with its witness, evaluated.
+static gkr::Circuit<F, FP> synth_circuit(uint64_t seed, uint32_t depth, uint32_t logw, uint32_t addp, uint32_t loc,
+                                         uint64_t &gates)
+{
+    gkr::Circuit<F, FP> c;
+    c.layers.resize(depth);
+    const size_t W = (size_t)1 << logw;
+    std::vector<uint32_t> mul(4 * W), add(3 * W);
+    gates = 0;
+    for (uint32_t l = 0; l < depth; l++) {
+        uint32_t nm = 0, na = 0;
+        if (gkrx_synth_layer(seed, l, logw, addp, loc, mul.data(), &nm, add.data(), &na) != 0) exit(2);
+        auto &ly = c.layers[l];
+        for (uint32_t g = 0; g < nm; g++) {
+            uint32_t in[2] = {mul[4 * g], mul[4 * g + 1]};
+            ly.mul.sparse_evals.emplace_back(gkr::Gate<FP, 2>(mul[4 * g + 2], in, FP(mul[4 * g + 3])));
+        }
+        for (uint32_t g = 0; g < na; g++) {
+            uint32_t in[1] = {add[3 * g]};
+            ly.add.sparse_evals.emplace_back(gkr::Gate<FP, 1>(add[3 * g + 1], in, FP(add[3 * g + 2])));
+        }
+        gates += nm + na;
+    }
+    c._compute_nb_vars();
+    const uint32_t lanes = sizeof(F) / 4;
+    std::vector<uint32_t> w(W * lanes);
+    if (gkrx_synth_witness_m31(seed, W, lanes, w.data()) != 0) exit(2);
+    std::vector<unsigned char> wb(w.size() * 4);
+    memcpy(wb.data(), w.data(), wb.size());
+    set_witness(c, wb);
+    c.evaluate();
+    return c;
+}
+
+static int throughput(int argc, char **argv)
+{
+    const int nt = atoi(argv[2]);
+    const uint32_t addp = (uint32_t)atoi(argv[3]), loc = (uint32_t)atoi(argv[4]);
+    struct Spec {
+        uint64_t seed;
+        uint32_t depth, logw;
+    };
+    std::vector<Spec> specs;
+    for (int i = 5; i < argc; i++) {
+        unsigned long long s;
+        unsigned d, k;
+        if (sscanf(argv[i], "%llu:%u:%u", &s, &d, &k) != 3 || d == 0 || k == 0 || k > 24) return 2;
+        specs.push_back({s, d, k});
+    }
+    if (nt < 1 || specs.empty()) return 2;
+    std::atomic<size_t> next{0};
+    std::vector<double> busy(nt, 0.0);
+    std::vector<uint64_t> gates(nt, 0);
+    Config config;
+    const auto w0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+        pool.emplace_back([&, t] {
+            for (size_t i; (i = next++) < specs.size();) {
+                uint64_t g = 0;
+                auto circuit = synth_circuit(specs[i].seed, specs[i].depth, specs[i].logw, addp, loc, g);
+                gkr::Prover<F, FP> prover(config);
+                const auto t0 = std::chrono::steady_clock::now();
+                prover.prepare_mem(circuit);
+                prover.prove(circuit);
+                busy[t] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                gates[t] += g;
+            }
+        });
+    for (auto &th : pool) th.join();
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+    uint64_t total = 0;
+    double worst = 0;
+    for (int t = 0; t < nt; t++) {
+        total += gates[t];
+        worst = std::max(worst, busy[t]);
+    }
+    printf("%llu %.6f %.6f\n", (unsigned long long)total, worst, wall);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) return 2;
     std::string cmd = argv[1];
+    if (cmd == "throughput" && argc >= 6) return throughput(argc, argv);
     if (cmd == "proof_write" && argc >= 5) {
         gkr::Proof<F> p;
         p.bytes = read_file(argv[2]);
