@@ -99,6 +99,12 @@ extern "C" int gkrx_ctx_create(const gkrx_config *cfg, int device, gkrx_ctx **ou
         set_error("hipStreamCreate failed");
         return GKRX_ERR_DEVICE;
     }
+    // the allocation-free prover's table array on this device (gkrx_prove_noalloc)
+    if (int rc = dev_noalloc_init(device)) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return GKRX_OK;
 }
@@ -665,6 +671,60 @@ try {
 } catch (const std::bad_alloc &) {
     set_error("out of host memory");
     return GKRX_ERR_NOMEM;
+}
+
+extern "C" int gkrx_prove_noalloc(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap,
+                                  size_t *proof_len, uint32_t *claimed_v)
+{
+    if (!ctx || !c || !proof || !proof_len || !claimed_v || c->ctx != ctx) return GKRX_ERR_ARG;
+    if (ctx->bn254) {
+        set_error("gkrx_prove_noalloc proves M31 circuits");
+        return GKRX_ERR_CONFIG;
+    }
+    if (c->in_server) {
+        set_error("circuit has a server request in flight");
+        return GKRX_ERR_STATE;
+    }
+    const size_t total = gkrx_proof_size(ctx, c);
+    if (proof_cap < total) {
+        set_error("proof buffer too small");
+        return GKRX_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    GKRX_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t tb = tail_bytes(ctx, c);
+    if (dev_noalloc_bytes(c->dev, ctx->R, tb) > kNoAllocScratchBytes) {
+        set_error("circuit needs " + std::to_string(dev_noalloc_bytes(c->dev, ctx->R, tb)) +
+                  " bytes of prover tables, above the " + std::to_string(kNoAllocScratchBytes) +
+                  " bytes of gkrx_prove_noalloc (prove it with gkrx_prove)");
+        return GKRX_ERR_ARG;
+    }
+    if (!c->evaluated) {
+        int rc = evaluate_locked(c);  // asynchronous: overlaps the host transcript prefix below
+        if (rc) return rc;
+    }
+    const size_t commit_len = ((size_t)1 << c->n[0]) * ctx->L * 4;
+    const uint32_t mlast = c->m[c->D - 1];
+    uint32_t rz1[3 * kMaxVars], tstate[10], tfinal[10];
+    uint8_t grind[32];
+    const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
+    host_transcript_prefix(kind, reinterpret_cast<const uint8_t *>(c->witness.data()), commit_len, ctx->R, mlast,
+                           (uint32_t)ctx->cfg.grinding_bits, grind, rz1, tstate);
+    DevScratch view;
+    int rc = dev_prove_noalloc(c->dev, ctx->R, kind, rz1, tstate, grind, tb, ctx->device, ctx->stream, view);
+    if (rc) return rc;
+    memcpy(proof, c->witness.data(), commit_len);  // RAW commitment bytes (raw.hpp:19-27)
+    GKRX_HIP_CHECK(hipMemcpyAsync(proof + commit_len, view.tail, tb, hipMemcpyDeviceToHost, ctx->stream));
+    GKRX_HIP_CHECK(hipMemcpyAsync(claimed_v, view.claims, (size_t)ctx->R * ctx->L * 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    GKRX_HIP_CHECK(hipMemcpyAsync(tfinal, view.tstate, sizeof tfinal, hipMemcpyDeviceToHost, ctx->stream));
+    GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (tfinal[0] != tb) {
+        set_error("device transcript length mismatch: " + std::to_string(tfinal[0]) + " != " + std::to_string(tb));
+        return GKRX_ERR_DEVICE;
+    }
+    *proof_len = total;
+    return GKRX_OK;
 }
 
 namespace {

@@ -205,6 +205,19 @@ int dev_upload_circuit(DevCircuit &dc, uint32_t L, const std::vector<uint32_t> &
                        const std::vector<std::vector<uint32_t>> &mul, const std::vector<std::vector<uint32_t>> &add);
 void dev_free_circuit(DevCircuit &dc);
 int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t tail_bytes);
+// The allocation-free prove (gkrx_prove_noalloc): its prover tables live in a file-scope
+// device array of kNoAllocScratchBytes.  dev_noalloc_init (once per device, with the device
+// current; the context's creation calls it) looks up that array's device address.
+constexpr uint64_t kNoAllocScratchBytes = 64ull << 20;
+int dev_noalloc_init(int device);
+// Bytes of prover tables a prove of dc needs (the allocation-free prove's size check).
+uint64_t dev_noalloc_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes);
+// dev_prove on the device array: the seed goes in by a kernel argument, nothing is
+// allocated or copied from host staging; view receives the table addresses (tail, claims,
+// tstate) to copy back.  GKRX_ERR_ARG when dc needs more than kNoAllocScratchBytes.
+int dev_prove_noalloc(const DevCircuit &dc, uint32_t R, int hash_kind, const uint32_t *rz1_host,
+                      const uint32_t *tstate_host, const uint8_t *grind, uint64_t tail_bytes, int device,
+                      hipStream_t st, DevScratch &view);
 void dev_free_scratch(DevScratch &s);
 void dev_drop_graph(DevScratch &s);  // releases the cached prove graph, if any
 int dev_forward(const DevCircuit &dc, hipStream_t st, Profiler *pf);

@@ -2241,6 +2241,109 @@ int dev_ensure_scratch(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_t
     return GKRX_OK;
 }
 
+// ------------------------------------------------------------------ allocation-free prove
+// The prover tables of gkrx_prove_noalloc live in this file-scope device array (each GPU
+// the library runs on holds its own copy until the process ends) instead of memory the
+// library allocates.  Host code never uses the array's name as an address: dev_noalloc_init
+// looks up the array's device address on each device once (hipGetSymbolAddress, with the
+// device current) and keeps it; kernels get that address.  Every call on a GPU shares the
+// GPU's copy, so calls that overlap on one GPU race.
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+uint32_t *g_noalloc_ptr[kNoAllocDevices];
+
+// The transcript seed from the host prefix as a kernel argument: rz1 of every repetition
+// (R * mlast words, repetition-major), the transcript state and the 32 grind bytes.
+struct NoAllocSeed {
+    uint32_t rz1[3 * kMaxVars];
+    uint32_t tstate[10];
+    uint32_t grind[8];
+    uint32_t R, mlast;
+};
+
+// Writes the seed slot D (rz1 in the slot layout, zero elsewhere), alpha = 1, beta = 0, the
+// transcript state and the grind bytes at the start of the tail: every scratch word a
+// prove reads before writing it.
+__global__ void k_install_seed(NoAllocSeed a, uint32_t *__restrict__ chal_slot, uint32_t *__restrict__ ab,
+                               uint32_t *__restrict__ tstate, uint32_t *__restrict__ tail)
+{
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < a.R * kChalStride; i += blockDim.x) {
+        const uint32_t j = i / kChalStride, v = i % kChalStride;
+        chal_slot[i] = v < a.mlast ? a.rz1[j * a.mlast + v] : 0u;
+    }
+    if (t < 10) tstate[t] = a.tstate[t];
+    if (t < 8) tail[t] = a.grind[t];
+    if (t == 0) {
+        ab[0] = 1u;
+        ab[1] = 0u;
+    }
+}
+}  // namespace
+
+int dev_noalloc_init(int device)
+{
+    if (device < 0 || device >= kNoAllocDevices) {
+        set_error("device ordinal outside the allocation-free prover's table");
+        return GKRX_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return GKRX_OK;
+    void *p = nullptr;
+    GKRX_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+        set_error("hipGetSymbolAddress gave no address for the scratch array");
+        return GKRX_ERR_DEVICE;
+    }
+    g_noalloc_ptr[device] = reinterpret_cast<uint32_t *>(p);
+    return GKRX_OK;
+}
+
+uint64_t dev_noalloc_bytes(const DevCircuit &dc, uint32_t R, uint64_t tail_bytes)
+{
+    DevScratch s;
+    return 4 * scratch_layout(s, dc, R, tail_bytes, nullptr);
+}
+
+int dev_prove_noalloc(const DevCircuit &dc, uint32_t R, int hash_kind, const uint32_t *rz1_host,
+                      const uint32_t *tstate_host, const uint8_t *grind, uint64_t tail_bytes, int device,
+                      hipStream_t st, DevScratch &view)
+{
+    if (R > 3 || (dc.L != 8 && dc.L != 16)) return GKRX_ERR_CONFIG;
+    uint32_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        base = (device >= 0 && device < kNoAllocDevices) ? g_noalloc_ptr[device] : nullptr;
+    }
+    if (!base) {
+        set_error("the allocation-free prover was not set up on this device");
+        return GKRX_ERR_STATE;
+    }
+    const uint64_t need = dev_noalloc_bytes(dc, R, tail_bytes);
+    if (need > kNoAllocScratchBytes) {
+        set_error("circuit needs " + std::to_string(need) + " bytes of prover tables, above the " +
+                  std::to_string(kNoAllocScratchBytes) + " bytes of the allocation-free prover");
+        return GKRX_ERR_ARG;
+    }
+    view = DevScratch{};
+    scratch_layout(view, dc, R, tail_bytes, base);
+    const uint32_t D = dc.D, mlast = dc.layers[D - 1].m;
+    NoAllocSeed a{};
+    for (uint32_t i = 0; i < R * mlast; i++) a.rz1[i] = rz1_host[i];
+    memcpy(a.tstate, tstate_host, sizeof a.tstate);
+    memcpy(a.grind, grind, sizeof a.grind);
+    a.R = R;
+    a.mlast = mlast;
+    hipLaunchKernelGGL(k_install_seed, dim3(1), dim3(256), 0, st, a, view.chal + (uint64_t)D * R * kChalStride,
+                       view.ab + 2ull * D, view.tstate, view.tail);
+    GKRX_HIP_CHECK(hipGetLastError());
+    if (dc.L == 8) return prove_impl<8>(dc, view, R, hash_kind, tstate_host[0], tstate_host[1], st, nullptr);
+    return prove_impl<16>(dc, view, R, hash_kind, tstate_host[0], tstate_host[1], st, nullptr);
+}
+
 void dev_free_scratch(DevScratch &s)
 {
     dev_drop_graph(s);

@@ -70,6 +70,7 @@ def lib():
     L.gkrx_proof_size.argtypes = [vp, vp]
     L.gkrx_proof_size.restype = C.c_size_t
     L.gkrx_prove.argtypes = [vp, vp, _u8p, C.c_size_t, C.POINTER(C.c_size_t), _u32p]
+    L.gkrx_prove_noalloc.argtypes = [vp, vp, _u8p, C.c_size_t, C.POINTER(C.c_size_t), _u32p]
     L.gkrx_verify.argtypes = [vp, vp, _u32p, _u8p, C.c_size_t]
     L.gkrx_verify_arrays.argtypes = [C.POINTER(_Config), C.c_uint32, _u32p, C.POINTER(C.c_void_p), _u32p,
                                      C.POINTER(C.c_void_p), _u32p, _u8p, C.c_size_t]

@@ -110,6 +110,18 @@ size_t gkrx_proof_size(const gkrx_ctx *ctx, const gkrx_circuit *c);
  * of the tuple prove returns. */
 int gkrx_prove(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap, size_t *proof_len,
                uint32_t *claimed_v);
+/* gkrx_prove without allocating: no counterpart in the reference.  For callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc -- this variant calls no hipMalloc, hipFree, hipHostMalloc or other
+ * allocating routine: the prover tables live in a file-scope device array of 64 MiB (each
+ * GPU the library runs on holds it until the process ends; the context's creation looks up
+ * its address), and the transcript seed goes to the device as a kernel argument.  Same
+ * arguments, proof bytes and claims as gkrx_prove (M31 only).  Circuits whose tables need
+ * more than 64 MiB (about 2^16 rows per layer at 16 lanes) fail with GKRX_ERR_ARG.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (from two contexts)
+ * race; calls on one context are serialized by it. */
+int gkrx_prove_noalloc(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, size_t proof_cap, size_t *proof_len,
+                       uint32_t *claimed_v);
 /* n independent proofs at once (no reference counterpart: the reference proves one
  * proof per thread, benchmarks/bench_keccak.cpp:32-42).  Proof i is byte-identical to
  * gkrx_prove on circuits[i].  The transcript prefixes run on a pool of host threads, then
