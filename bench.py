@@ -44,7 +44,12 @@ TAIL_ROWS = 2048  # tables with at most this many rows end a phase in the small-
 SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 140514307}
 # k_ls_pair of the 512-proof batch (profiles/pmc_ls_pair_m31x16_512_uneven_keccak256.md):
 # {(proofs, block, lanes per thread): bytes per dispatch}
-BATCH_PAIR_TRAFFIC = {}
+BATCH_PAIR_TRAFFIC = {
+    (512, 64, 1): 1690935665, (512, 64, 2): 1554011197, (512, 64, 4): 1474593235,
+    (512, 128, 1): 1514966340, (512, 128, 2): 1406289529, (512, 128, 4): 1336712176,
+    (512, 256, 1): 1386175865, (512, 256, 2): 1310957996, (512, 256, 4): 1267886940,
+    (512, 512, 1): 1311674523, (512, 512, 2): 1258961629, (512, 512, 4): 1227457480,
+}
 
 
 def log(msg):
@@ -129,7 +134,7 @@ def build_m31(gkrx, ctx, specs):
     return [c for c, _ in made], [g for _, g in made]
 
 
-def batch_leg(gkrx, fs, count, steps, warmup):
+def batch_leg(gkrx, fs, count, steps, warmup, plain=True, fixed=None):
     """The headline: the uneven batch proved by the autotuned batch engine, K timed steps of
     one batched prove each, against the plain path (gkrx_prove once per proof) on the same
     proofs.  The round-pair kernel k_ls_pair (the batch's critical lane) is timed inside
@@ -144,9 +149,15 @@ def batch_leg(gkrx, fs, count, steps, warmup):
     prover = gkrx.Prover(ctx)
     R = ctx.config.get_num_repetitions()
     outs = [np.empty(prover.proof_size(c), dtype=np.uint8) for c in circs]
-    engine, block, lanes, tune_ms = ctx.batch_autotune(circs)
     names = {gkrx.BATCH_PER_PROOF: "per_proof", gkrx.BATCH_LOCKSTEP: "lockstep"}
-    tuned = {f"{names[e]}_b{b}_v{v}": round(ms, 3) for (e, b, v), ms in tune_ms.items()}
+    if fixed:  # (block, lanes) of the lockstep engine, no autotuning
+        engine, (block, lanes) = gkrx.BATCH_LOCKSTEP, fixed
+        ctx.set_batch_engine(engine, block)
+        ctx.set_batch_lanes(lanes)
+        tuned = None
+    else:
+        engine, block, lanes, tune_ms = ctx.batch_autotune(circs)
+        tuned = {f"{names[e]}_b{b}_v{v}": round(ms, 3) for (e, b, v), ms in tune_ms.items()}
     log(f"autotune: {tuned} -> {names[engine]} block {block} lanes {lanes}")
     for _ in range(warmup):
         prover.prove_batch(circs, outs)
@@ -163,18 +174,20 @@ def batch_leg(gkrx, fs, count, steps, warmup):
     # the plain path on the same proofs, one gkrx_prove per proof; its proofs must equal
     # the batched ones
     plain_lat = []
-    same = True
+    same = None
     buf = np.empty(max(o.size for o in outs), dtype=np.uint8)
     t0 = time.perf_counter()
-    for i, c in enumerate(circs):
+    for i, c in enumerate(circs if plain else []):
         t1 = time.perf_counter()
         c.evaluate()
         cl, pr = prover.prove(c, out=buf)
         plain_lat.append((time.perf_counter() - t1) * 1e3)
-        same &= bool(np.array_equal(cl, res[i][0])) and pr.size == res[i][1].size and bool(np.array_equal(pr, res[i][1]))
-    plain_s = time.perf_counter() - t0
+        same = (same is not False) and bool(np.array_equal(cl, res[i][0])) and pr.size == res[i][1].size and \
+            bool(np.array_equal(pr, res[i][1]))
+    plain_s = time.perf_counter() - t0 if plain else None
     ok = all(gkrx.Verifier(ctx).verify(circs[i], res[i][0], res[i][1]) for i in range(0, count, max(1, count // 8)))
-    log(f"batched {batched_s * 1e3:.1f} ms, plain {plain_s * 1e3:.1f} ms, identical={same}, verified={ok}")
+    plain_txt = f"{plain_s * 1e3:.1f} ms" if plain else "not run"
+    log(f"batched {batched_s * 1e3:.1f} ms, plain {plain_txt}, identical={same}, verified={ok}")
     roof = None
     if engine == gkrx.BATCH_LOCKSTEP and st["count"]:
         nv = [ctx_n for c in circs for ctx_n in c.nb_input_vars]
@@ -189,10 +202,10 @@ def batch_leg(gkrx, fs, count, steps, warmup):
                 "algorithmic_bytes_per_launch": int(algo), "avg_launch_us": round(avg_ms * 1e3, 2),
                 "launches_per_step": st["count"] // steps, "pair_ms_per_step": round(st["ms"] / steps, 3)}
     out = {"proofs": count, "gates": gates, "batched_gates_per_s": gates / batched_s,
-           "batched_ms": batched_s * 1e3, "plain_gates_per_s": gates / plain_s, "plain_ms": plain_s * 1e3,
-           "ratio": plain_s / batched_s,
+           "batched_ms": batched_s * 1e3, "plain_gates_per_s": gates / plain_s if plain else None,
+           "plain_ms": plain_s * 1e3 if plain else None, "ratio": plain_s / batched_s if plain else None,
            "plain_latency_ms": {"p50": round(percentile(plain_lat, 50), 3),
-                                "p99": round(percentile(plain_lat, 99), 3)},
+                                "p99": round(percentile(plain_lat, 99), 3)} if plain else None,
            "identical_to_plain": same, "verified_sample": ok, "engine": names[engine], "block": block,
            "lanes_per_thread": lanes, "autotune_device_ms": tuned,
            "batched_split_ms": {k: round(float(np.mean([s[k] for s in splits])), 3) for k in splits[0]},
@@ -436,6 +449,8 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="proofs in the uneven batch")
     ap.add_argument("--hash", choices=["keccak256", "sha256"], default="keccak256")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain (one proof per launch) pass")
+    ap.add_argument("--fixed-engine", type=str, default="",
+                    help="BLOCK,LANES: the lockstep engine with this block size and lanes per thread, no autotuning")
     ap.add_argument("--single-layers", type=int, default=24, help="layers of the single-proof leg (0 = skip)")
     ap.add_argument("--single-log-width", type=int, default=20)
     ap.add_argument("--single-steps", type=int, default=5)
@@ -453,9 +468,10 @@ def main():
         print("bench.py measures one GPU only: there is no multi-GPU path here (independent proofs would "
               "shard by proof id, one process per GPU, with no communication)", file=sys.stderr)
         sys.exit(2)
-    if args.batch < 1 or args.steps < 1:
-        print("--batch and --steps must be at least 1", file=sys.stderr)
+    if args.steps < 1:
+        print("--steps must be at least 1", file=sys.stderr)
         sys.exit(2)
+    fixed = tuple(int(x) for x in args.fixed_engine.split(",")) if args.fixed_engine else None
 
     import gkrx
 
@@ -463,12 +479,14 @@ def main():
         calibration_copy(args.calibration_copy_mb)
     fs = gkrx.FS_KECCAK256 if args.hash == "keccak256" else gkrx.FS_SHA256
 
-    batch, m31_state = batch_leg(gkrx, fs, args.batch, args.steps, args.warmup)
-    serve = None
-    if args.serve_seconds > 0:
-        serve = serve_leg(gkrx, fs, m31_state, args.serve_seconds, args.serve_budget_gb, args.serve_slots,
-                          args.serve_bn254_slots)
-    m31_state[0].close()
+    # --batch 0 leaves out the headline (profiling runs of the other legs)
+    batch = serve = None
+    if args.batch > 0:
+        batch, m31_state = batch_leg(gkrx, fs, args.batch, args.steps, args.warmup, not args.no_plain, fixed)
+        if args.serve_seconds > 0:
+            serve = serve_leg(gkrx, fs, m31_state, args.serve_seconds, args.serve_budget_gb, args.serve_slots,
+                              args.serve_bn254_slots)
+        m31_state[0].close()
     single = None
     if args.single_layers > 0:
         single = single_leg(gkrx, fs, args.hash, args.single_layers, args.single_log_width, args.single_steps, 2)
@@ -479,11 +497,15 @@ def main():
     if not args.no_cpu_baseline:
         try:
             shape = (args.single_layers, args.single_log_width) if single else None
-            cpu, cpu_single = cpu_baselines(fs, args.batch, shape)
+            cpu, cpu_single = cpu_baselines(fs, args.batch, shape) if args.batch > 0 else (None, None)
             if single is not None:
                 single["cpu_baseline"] = cpu_single
         except Exception as e:  # reported, never fatal
             log(f"cpu baseline failed: {e}")
+    if batch is None:
+        print(json.dumps({"metric": "proved gates/s (M31x16) batched vs one-proof-per-launch; p50/p99 proof latency",
+                          "value": None, "single_proof": single, "bn254": bn254}), flush=True)
+        return
     value = batch["batched_gates_per_s"]
     line = {
         "metric": "proved gates/s (M31x16) batched vs one-proof-per-launch; p50/p99 proof latency",
@@ -498,8 +520,8 @@ def main():
                    "method": f"batched ({batch['engine']} engine, block {batch['block']}, "
                              f"{batch['lanes_per_thread']} lanes per thread, autotuned)"},
         "lane_gates_per_s": round(value * 16, 1),
-        "plain_gates_per_s": round(batch["plain_gates_per_s"], 1),
-        "batched_vs_plain": round(batch["ratio"], 3),
+        "plain_gates_per_s": batch["plain_gates_per_s"] and round(batch["plain_gates_per_s"], 1),
+        "batched_vs_plain": batch["ratio"] and round(batch["ratio"], 3),
         "latency_ms": serve["latency_ms"] if serve else None,
         "roofline": batch["roofline"],
         "cpu_baseline": cpu,

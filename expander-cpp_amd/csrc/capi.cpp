@@ -121,6 +121,15 @@ extern "C" int gkrx_ctx_destroy(gkrx_ctx *ctx)
     dev_free_scratch(ctx->scratch);
     dev_free_batch(ctx->batch);
     dev_bn_free_scratch(ctx->bn);
+    for (BnLane &ln : ctx->bn_lanes) {
+        if (ln.st) {
+            (void)hipStreamSynchronize(ln.st);
+            (void)hipStreamDestroy(ln.st);
+        }
+        dev_bn_free_scratch(ln.bn);
+        if (ln.tail_host) (void)hipHostFree(ln.tail_host);
+        if (ln.small_host) (void)hipHostFree(ln.small_host);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GKRX_OK;
@@ -844,6 +853,96 @@ class PrefixRunner final : public BatchPrefixes {
 
 }  // namespace
 
+// BN254 batches: the host prefixes on host threads, then the proofs on up to kBnBatchLanes
+// streams at once (each proof's rounds are chains of short launches and hashes, so proofs
+// side by side fill the GPU that one leaves idle).  Proof i is gkrx_prove's bytes for
+// circuits[i].  ctx->mu is not held on entry.
+static int prove_batch_bn254(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, uint8_t *const *proofs,
+                             size_t *proof_lens, uint32_t *const *claimed_v)
+{
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    GKRX_HIP_CHECK(hipSetDevice(ctx->device));
+    const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
+    struct Pre {
+        std::vector<uint32_t> rz1;
+        uint32_t tstate[10];
+        uint8_t grind[32];
+    };
+    std::vector<Pre> pre(n);
+    parallel_for(n, 16, [&](size_t i) {
+        gkrx_circuit *c = circuits[i];
+        const uint32_t mlast = c->m[c->D - 1];
+        pre[i].rz1.resize((size_t)8 * mlast);
+        host_transcript_prefix_bn254(kind, reinterpret_cast<const uint8_t *>(c->witness.data()),
+                                     commit_bytes(ctx, c), mlast, (uint32_t)ctx->cfg.grinding_bits, pre[i].grind,
+                                     pre[i].rz1.data(), pre[i].tstate);
+    });
+    const auto t1 = std::chrono::steady_clock::now();
+    const uint32_t nl = std::min<uint32_t>(n, kBnBatchLanes);
+    if (ctx->bn_lanes.size() < nl) ctx->bn_lanes.resize(nl);
+    for (uint32_t q = 0; q < nl; q++) {
+        BnLane &ln = ctx->bn_lanes[q];
+        if (ln.st) GKRX_HIP_CHECK(hipStreamSynchronize(ln.st));  // a failed batch may have left work
+        ln.item = -1;
+        if (!ln.st) GKRX_HIP_CHECK(hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking));
+        if (!ln.small_host) GKRX_HIP_CHECK(hipHostMalloc(&ln.small_host, 32 * 4, hipHostMallocDefault));
+    }
+    // waits for the lane's proof and hands its tail and claims out
+    auto finish = [&](BnLane &ln) -> int {
+        if (ln.item < 0) return GKRX_OK;
+        const uint32_t i = (uint32_t)ln.item;
+        ln.item = -1;
+        GKRX_HIP_CHECK(hipStreamSynchronize(ln.st));
+        gkrx_circuit *c = circuits[i];
+        const size_t tb = tail_bytes(ctx, c);
+        if (ln.small_host[8] != tb) {
+            set_error("batch item " + std::to_string(i) + ": device transcript length " +
+                      std::to_string(ln.small_host[8]) + " != " + std::to_string(tb));
+            return GKRX_ERR_DEVICE;
+        }
+        memcpy(proofs[i], c->witness.data(), commit_bytes(ctx, c));  // RAW commitment (raw.hpp:19-27)
+        memcpy(proofs[i] + commit_bytes(ctx, c), ln.tail_host, tb);
+        memcpy(claimed_v[i], ln.small_host, 32);
+        proof_lens[i] = commit_bytes(ctx, c) + tb;
+        c->evaluated = true;
+        return GKRX_OK;
+    };
+    int rc = GKRX_OK;
+    for (uint32_t i = 0; i < n && rc == GKRX_OK; i++) {
+        BnLane &ln = ctx->bn_lanes[i % nl];
+        if ((rc = finish(ln))) break;
+        gkrx_circuit *c = circuits[i];
+        const size_t tb = tail_bytes(ctx, c);
+        if (tb > ln.tail_cap) {
+            if (ln.tail_host) (void)hipHostFree(ln.tail_host);
+            ln.tail_host = nullptr;
+            ln.tail_cap = 0;
+            GKRX_HIP_CHECK(hipHostMalloc(&ln.tail_host, tb, hipHostMallocDefault));
+            ln.tail_cap = tb;
+        }
+        if ((rc = dev_bn_ensure_scratch(ln.bn, c->dev, tb))) break;
+        if ((rc = dev_bn_forward(c->dev, ln.st, nullptr))) break;
+        if ((rc = dev_bn_prove(c->dev, ln.bn, kind, pre[i].rz1.data(), pre[i].tstate, pre[i].grind, 32, ln.st,
+                               nullptr)))
+            break;
+        GKRX_HIP_CHECK(hipMemcpyAsync(ln.tail_host, ln.bn.tail, tb, hipMemcpyDeviceToHost, ln.st));
+        GKRX_HIP_CHECK(hipMemcpyAsync(ln.small_host, ln.bn.claims, 32, hipMemcpyDeviceToHost, ln.st));
+        GKRX_HIP_CHECK(hipMemcpyAsync(ln.small_host + 8, ln.bn.tstate, 40, hipMemcpyDeviceToHost, ln.st));
+        ln.item = i;
+    }
+    for (uint32_t q = 0; q < nl; q++) {
+        const int r2 = finish(ctx->bn_lanes[q]);
+        if (rc == GKRX_OK) rc = r2;
+    }
+    if (rc) return rc;
+    const auto t2 = std::chrono::steady_clock::now();
+    ctx->last.host_prefix_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    return GKRX_OK;
+}
+
 extern "C" int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, uint8_t *const *proofs,
                                 const size_t *proof_caps, size_t *proof_lens, uint32_t *const *claimed_v)
 try {
@@ -869,14 +968,7 @@ try {
         }
     }
     if (n == 0) return GKRX_OK;
-    if (ctx->bn254) {
-        // BN254 proofs are compute-bound multi-block launches; a batch runs them in turn
-        for (uint32_t i = 0; i < n; i++) {
-            int rc = gkrx_prove(ctx, circuits[i], proofs[i], proof_caps[i], &proof_lens[i], claimed_v[i]);
-            if (rc) return rc;
-        }
-        return GKRX_OK;
-    }
+    if (ctx->bn254) return prove_batch_bn254(ctx, n, circuits, proofs, proof_lens, claimed_v);
     std::lock_guard<std::mutex> lk(ctx->mu);
     const auto t0 = std::chrono::steady_clock::now();
     GKRX_HIP_CHECK(hipSetDevice(ctx->device));

@@ -10,6 +10,18 @@
 #include "gkrx.h"
 #include "internal.hpp"
 
+// One stream of a BN254 batch (gkrx_prove_batch proves up to kBnBatchLanes BN254 proofs at
+// once, each on a lane with its own tables and pinned staging of its results).
+struct BnLane {
+    hipStream_t st = nullptr;
+    gkrx::BnScratch bn;
+    uint8_t *tail_host = nullptr;    // pinned: the proof tail of the lane's current proof
+    uint64_t tail_cap = 0;
+    uint32_t *small_host = nullptr;  // pinned: claims (8 words), then the transcript state (10 words)
+    int64_t item = -1;               // batch item in flight on the lane
+};
+constexpr uint32_t kBnBatchLanes = 4;
+
 struct gkrx_ctx {
     gkrx_config cfg;
     int device = 0;
@@ -23,6 +35,7 @@ struct gkrx_ctx {
     gkrx::ProveTimings last;
     gkrx::Profiler prof;
     gkrx::BatchScratch batch;
+    std::vector<BnLane> bn_lanes;  // created on the first BN254 batch
     bool graphs = false;  // gkrx_prove replays its device part as a captured HIP graph
     std::mutex mu;
     std::atomic<int> live_circuits{0};  // circuits (and servers) bound to this context

@@ -35,7 +35,7 @@ namespace {
 constexpr int kBnThreads = 256;
 constexpr uint32_t kBnMaxBlocks = 1024;
 constexpr uint32_t kBnTailThreads = 512;
-constexpr uint32_t kBnTailRows = 4096;  // rounds with at most this many rows run in k_bn_tail
+constexpr uint32_t kBnTailRows = 512;  // rounds with at most this many rows run in k_bn_tail
 constexpr uint32_t kBnChalStride = 64;  // Fr per layer slot: rx in [0, 32), ry in [32, 64)
 
 __device__ __forceinline__ Fr ld_fr(const uint32_t *p)
@@ -295,21 +295,22 @@ __device__ void bn_transcript_step(const uint32_t *sums, const BnTsArgs &a, Fr *
     const uint32_t base = ts[0], hs0 = ts[1];
     const uint32_t P = (base - hs0) / 4;
     if (t < P && P <= 8) hb[8 - P + t] = a.tail[hs0 / 4 + t];
-    if (t == 0) {
+    if (t < 3) {
+        // thread e forms message element e: p0, p1, or p2 = 6 p1 + 3 p0 - 2 p2, times vx in
+        // phase 2, as canonical bytes (three threads: the Montgomery products run side by side)
+        const uint32_t e = t;
         const Fr s0 = ld_fr(sums), s1 = ld_fr(sums + 8), s2 = ld_fr(sums + 16);
-        const Fr s1x2 = fr_add(s1, s1), s1x3 = fr_add(s1x2, s1), s1x6 = fr_add(s1x3, s1x3);
-        const Fr s0x3 = fr_add(fr_add(s0, s0), s0);
-        Fr q[3] = {s0, s1, fr_sub(fr_add(s1x6, s0x3), fr_add(s2, s2))};
-        if (a.phase == 2) {
-            const Fr vx = ld_fr(a.vx);
-            for (int e = 0; e < 3; e++) q[e] = fr_mont_mul(q[e], vx);
+        Fr q = e == 0 ? s0 : s1;
+        if (e == 2) {
+            const Fr s1x2 = fr_add(s1, s1), s1x3 = fr_add(s1x2, s1), s1x6 = fr_add(s1x3, s1x3);
+            const Fr s0x3 = fr_add(fr_add(s0, s0), s0);
+            q = fr_sub(fr_add(s1x6, s0x3), fr_add(s2, s2));
         }
-        for (int e = 0; e < 3; e++) {
-            const Fr c = fr_from_mont(q[e]);
-            for (int i = 0; i < 8; i++) {
-                a.tail[base / 4 + 8 * e + i] = c.w[i];
-                hb[8 + 8 * e + i] = c.w[i];
-            }
+        if (a.phase == 2) q = fr_mont_mul(q, ld_fr(a.vx));
+        const Fr c = fr_from_mont(q);
+        for (int i = 0; i < 8; i++) {
+            a.tail[base / 4 + 8 * e + i] = c.w[i];
+            hb[8 + 8 * e + i] = c.w[i];
         }
     }
     __syncthreads();
@@ -382,6 +383,7 @@ __device__ __forceinline__ void bn_reduce_partials(const uint32_t *__restrict__ 
 
 __global__ void __launch_bounds__(kBnThreads) k_bn_fin(const uint32_t *__restrict__ partials, uint32_t nb, BnTsArgs ts)
 {
+    __builtin_amdgcn_s_setprio(3);  // a hash chain: its waves issue first on a shared CU
     __shared__ uint32_t sums[24];
     __shared__ Fr rr;
     bn_reduce_partials(partials, nb, sums);
@@ -399,6 +401,7 @@ struct BnTailParams {
 
 __global__ void __launch_bounds__(kBnTailThreads) k_bn_tail(BnTailParams p)
 {
+    __builtin_amdgcn_s_setprio(3);  // a hash chain: its waves issue first on a shared CU
     __shared__ Fr red[3 * kBnTailThreads / 64];
     __shared__ uint32_t sums[24];
     __shared__ Fr rr;

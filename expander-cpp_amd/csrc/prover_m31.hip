@@ -49,6 +49,11 @@ constexpr int kThreads = 256;
 constexpr uint32_t kMaxBlocksPerRep = 256;
 constexpr uint32_t kChalStride = 64;  // per rep: rx in [0, 32), ry in [32, 64)
 
+// The transcript kernels are chains of dependent single-wave hashes: on a CU they share
+// with bandwidth kernels of other streams (batches, server slots), their waves take the
+// SIMDs' issue slots first.
+__device__ __forceinline__ void latency_critical() { __builtin_amdgcn_s_setprio(3); }
+
 __device__ __forceinline__ uint4 ld4(const uint32_t *p) { return *reinterpret_cast<const uint4 *>(p); }
 __device__ __forceinline__ void st4(uint32_t *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
 __device__ __forceinline__ uint4 add4(uint4 a, uint4 b)
@@ -932,6 +937,7 @@ template <int L>
 __global__ void __launch_bounds__(kFinThreads) k_pair_fin(const uint32_t *__restrict__ partials, uint32_t nb,
                                                           TsArgs ts, uint32_t len0, uint32_t hs0)
 {
+    latency_critical();
     pair_fin_rep_body<L>(partials, nb, ts, len0, hs0, blockIdx.x);
 }
 
@@ -1001,6 +1007,7 @@ __device__ void phase_rounds(const TailParams &p)
 template <int L, bool PH2>
 __global__ void __launch_bounds__(kTailThreads) k_tail(TailParams p)
 {
+    latency_critical();
     phase_rounds<L, PH2>(p);
 }
 
@@ -1109,6 +1116,7 @@ __device__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
 template <int L, bool PH2>
 __global__ void __launch_bounds__(kRepThreads) k_tail_rep(TailRepParams q)
 {
+    latency_critical();
     tail_rep_body<L, PH2>(q, blockIdx.x);
 }
 
@@ -1138,6 +1146,7 @@ template <int L>
 __global__ void __launch_bounds__(kFinThreads) k_dot_fin(const uint32_t *__restrict__ partials, uint32_t nb, uint32_t R,
                                                          uint32_t *__restrict__ claims)
 {
+    latency_critical();
     constexpr uint32_t Q = 3 * L;
     __shared__ uint32_t sums[3 * Q];
     reduce_partials<L>(partials, nb, R * Q, sums);
@@ -1701,6 +1710,7 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restri
                                                         const uint32_t *__restrict__ start, uint32_t ntasks,
                                                         uint32_t R, int hash_kind)
 {
+    latency_critical();
     constexpr uint32_t Q = 3 * L;
     __shared__ uint32_t sums[3 * Q];
     const uint2 f = ls_find(start, ntasks, blockIdx.x);
@@ -1765,6 +1775,7 @@ __global__ void __launch_bounds__(kChainThreads) k_ls_chain(const BatchJob *__re
                                                             const uint32_t *__restrict__ start, uint32_t ntasks,
                                                             uint32_t R, int hash_kind)
 {
+    latency_critical();
     const uint2 f = ls_find(start, ntasks, blockIdx.x);
     const LsTask &T = tasks[f.x];
     const BatchJob &J = jobs[T.job];
@@ -2750,9 +2761,32 @@ void ls_build_schedule(const std::vector<BatchItem> &items, const std::vector<Ba
         }
         list(kLsDotFin, [](const LsEntry &e, auto &&put) { if (e.nf && e.f.op == kOpDotFin) put(e.f); },
              fin_blocks);
-        list(kLsPairs, [](const LsEntry &e, auto &&put) {
-            for (uint32_t q = 0; q < e.nw; q++) if (e.w[q].op == kOpPair) put(e.w[q]);
-        }, work_blocks);
+        // round pairs: the tasks whose workgroups run longest (most items per thread) first, so
+        // that the launch's last workgroups are short ones (workgroups start in index order)
+        {
+            LsList &ls = S.l[kLsPairs];
+            ls.task = (uint32_t)tasks.size();
+            ls.start = (uint32_t)starts.size();
+            std::vector<const LsTask *> pt;
+            for (size_t i = 0; i < nj; i++) {
+                const LsEntry *e = at(i, s);
+                if (!e) continue;
+                for (uint32_t q = 0; q < e->nw; q++)
+                    if (e->w[q].op == kOpPair) pt.push_back(&e->w[q]);
+            }
+            auto per_block = [](const LsTask *t) { return (uint64_t)t->rp.items * 1024 / t->nb; };
+            std::stable_sort(pt.begin(), pt.end(),
+                             [&](const LsTask *a, const LsTask *b) { return per_block(a) > per_block(b); });
+            uint32_t blocks = 0;
+            for (const LsTask *t : pt) {
+                starts.push_back(blocks);
+                tasks.push_back(*t);
+                blocks += work_blocks(*t);
+            }
+            ls.n = (uint32_t)tasks.size() - ls.task;
+            ls.blocks = blocks;
+            starts.push_back(blocks);
+        }
         list(kLsPairFin, [](const LsEntry &e, auto &&put) { if (e.nf && e.f.op == kOpPairFin) put(e.f); },
              fin_blocks);
         list(kLsChain, [](const LsEntry &e, auto &&put) { if (e.nf && ls_is_chain(e.f.op)) put(e.f); },

@@ -2,7 +2,9 @@
 (oracle/gkr_generic.py, whose structure is pinned on the M31 fixtures), claims equal,
 accepted by the host verifier; the forward pass equals the oracle's evaluation; tampered
 proofs are rejected.  The shapes cover layers small enough for the single-workgroup
-rounds only and layers wide enough (2^13, 2^14) to take the grid rounds first."""
+rounds only (up to 512 rows) and layers wide enough (2^10, 2^13, 2^14) to take the grid
+rounds first.  A batch of BN254 proofs (several streams at once) gives each proof's bytes
+made alone."""
 import os
 import sys
 
@@ -22,6 +24,7 @@ CASES = [  # (seed, depth, log_width, add_permille, locality, hash)
     (22, 3, 4, 700, True, gkrx.FS_KECCAK256),
     (23, 4, 6, 0, True, gkrx.FS_SHA256),
     (24, 3, 8, 1000, False, gkrx.FS_KECCAK256),
+    (27, 3, 10, 700, True, gkrx.FS_SHA256),
     (25, 2, 13, 700, True, gkrx.FS_KECCAK256),
     (26, 2, 14, 300, True, gkrx.FS_SHA256),
 ]
@@ -96,3 +99,26 @@ def test_bn254_rejects_noncanonical_witness(ctxs):
     with pytest.raises(gkrx.GkrxError):
         circ.set_witness(w)
     circ.close()
+
+
+def test_bn254_batch_equals_alone(ctxs):
+    """More proofs than the batch has streams, of uneven sizes; every proof and claim equal to
+    the proof made alone, twice in a row (lanes and their tables reused)."""
+    print("[gpu] bn254 batch vs alone", flush=True)
+    ctx = ctxs[1]
+    specs = [(60 + i, 2 + i % 3, [4, 12, 9, 14, 6, 10, 11][i]) for i in range(7)]
+    circs = []
+    for s_, d, k in specs:
+        c = gkrx.Circuit.from_layers(ctx, gkrx.synth_circuit(s_, d, k, 700, True))
+        c.set_witness(gkrx.synth_witness_bn254(s_, 1 << k))
+        circs.append(c)
+    prover = gkrx.Prover(ctx)
+    alone = [prover.prove(c) for c in circs]
+    for _ in range(2):
+        batch = prover.prove_batch(circs)
+        for (cl, pr), (c1, p1) in zip(batch, alone):
+            assert np.array_equal(cl, c1)
+            assert bytes(pr) == bytes(p1)
+    assert gkrx.Verifier(ctx).verify(circs[3], batch[3][0], batch[3][1])
+    for c in circs:
+        c.close()
