@@ -43,12 +43,12 @@ TAIL_ROWS = 2048  # tables with at most this many rows end a phase in the small-
 # (profiles/pmc_traffic_m31x16_24x2e20_keccak256.md):
 SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 140514307}
 # k_ls_pair of the 512-proof batch (profiles/pmc_ls_pair_m31x16_512_uneven_keccak256.md):
-# {(proofs, block, lanes per thread): bytes per dispatch}
+# {(proofs, engine, block, lanes per thread): bytes per dispatch}
 BATCH_PAIR_TRAFFIC = {
-    (512, 64, 1): 1690935665, (512, 64, 2): 1554011197, (512, 64, 4): 1474593235,
-    (512, 128, 1): 1514966340, (512, 128, 2): 1406289529, (512, 128, 4): 1336712176,
-    (512, 256, 1): 1386175865, (512, 256, 2): 1310957996, (512, 256, 4): 1267886940,
-    (512, 512, 1): 1311674523, (512, 512, 2): 1258961629, (512, 512, 4): 1227457480,
+    (512, 2, 64, 1): 1690935665, (512, 2, 64, 2): 1554011197, (512, 2, 64, 4): 1474593235,
+    (512, 2, 128, 1): 1514966340, (512, 2, 128, 2): 1406289529, (512, 2, 128, 4): 1336712176,
+    (512, 2, 256, 1): 1386175865, (512, 2, 256, 2): 1310957996, (512, 2, 256, 4): 1267886940,
+    (512, 2, 512, 1): 1311674523, (512, 2, 512, 2): 1258961629, (512, 2, 512, 4): 1227457480,
 }
 
 
@@ -149,9 +149,10 @@ def batch_leg(gkrx, fs, count, steps, warmup, plain=True, fixed=None):
     prover = gkrx.Prover(ctx)
     R = ctx.config.get_num_repetitions()
     outs = [np.empty(prover.proof_size(c), dtype=np.uint8) for c in circs]
-    names = {gkrx.BATCH_PER_PROOF: "per_proof", gkrx.BATCH_LOCKSTEP: "lockstep"}
-    if fixed:  # (block, lanes) of the lockstep engine, no autotuning
-        engine, (block, lanes) = gkrx.BATCH_LOCKSTEP, fixed
+    names = {gkrx.BATCH_PER_PROOF: "per_proof", gkrx.BATCH_LOCKSTEP: "lockstep",
+             gkrx.BATCH_LOCKSTEP_GROUPS: "lockstep_groups"}
+    if fixed:  # (engine, block, lanes) of a lockstep engine, no autotuning
+        engine, block, lanes = fixed
         ctx.set_batch_engine(engine, block)
         ctx.set_batch_lanes(lanes)
         tuned = None
@@ -189,12 +190,12 @@ def batch_leg(gkrx, fs, count, steps, warmup, plain=True, fixed=None):
     plain_txt = f"{plain_s * 1e3:.1f} ms" if plain else "not run"
     log(f"batched {batched_s * 1e3:.1f} ms, plain {plain_txt}, identical={same}, verified={ok}")
     roof = None
-    if engine == gkrx.BATCH_LOCKSTEP and st["count"]:
+    if engine in (gkrx.BATCH_LOCKSTEP, gkrx.BATCH_LOCKSTEP_GROUPS) and st["count"]:
         nv = [ctx_n for c in circs for ctx_n in c.nb_input_vars]
         algo = circuit_pair_bytes(nv, R, 4 * L) * steps / st["count"]
         avg_ms = st["ms"] / st["count"]
         achieved = algo / (avg_ms * 1e-3) / 1e9
-        traffic = BATCH_PAIR_TRAFFIC.get((count, block, lanes))
+        traffic = BATCH_PAIR_TRAFFIC.get((count, engine, block, lanes))
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
@@ -450,7 +451,8 @@ def main():
     ap.add_argument("--hash", choices=["keccak256", "sha256"], default="keccak256")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain (one proof per launch) pass")
     ap.add_argument("--fixed-engine", type=str, default="",
-                    help="BLOCK,LANES: the lockstep engine with this block size and lanes per thread, no autotuning")
+                    help="ENGINE,BLOCK,LANES: this lockstep engine (2 lockstep, 4 lockstep with proof groups) with "
+                         "this block size and lanes per thread, no autotuning")
     ap.add_argument("--single-layers", type=int, default=24, help="layers of the single-proof leg (0 = skip)")
     ap.add_argument("--single-log-width", type=int, default=20)
     ap.add_argument("--single-steps", type=int, default=5)

@@ -1018,9 +1018,10 @@ try {
 extern "C" int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block)
 {
     if (!ctx) return GKRX_ERR_ARG;
-    if ((engine != GKRX_BATCH_PER_PROOF && engine != GKRX_BATCH_LOCKSTEP) ||
+    if ((engine != GKRX_BATCH_PER_PROOF && engine != GKRX_BATCH_LOCKSTEP && engine != GKRX_BATCH_LOCKSTEP_GROUPS) ||
         (block != 64 && block != 128 && block != 256 && block != 512)) {
-        set_error("batch engine must be GKRX_BATCH_PER_PROOF or GKRX_BATCH_LOCKSTEP with block 64, 128, 256 or 512");
+        set_error("batch engine must be GKRX_BATCH_PER_PROOF, GKRX_BATCH_LOCKSTEP or GKRX_BATCH_LOCKSTEP_GROUPS with "
+                  "block 64, 128, 256 or 512");
         return GKRX_ERR_ARG;
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1116,8 +1117,9 @@ try {
     };
     // the one-workgroup-per-proof engine, then the lockstep engine over block size x lanes
     std::vector<Cand> cands = {{GKRX_BATCH_PER_PROOF, 256, 4}};
-    for (uint32_t b : {64u, 128u, 256u, 512u})
-        for (uint32_t v : {1u, 2u, 4u}) cands.push_back({GKRX_BATCH_LOCKSTEP, b, v});
+    for (int e : {GKRX_BATCH_LOCKSTEP, GKRX_BATCH_LOCKSTEP_GROUPS})
+        for (uint32_t b : {64u, 128u, 256u, 512u})
+            for (uint32_t v : {1u, 2u, 4u}) cands.push_back({e, b, v});
     const int saved_engine = ctx->batch.engine;
     const uint32_t saved_block = ctx->batch.block, saved_lanes = ctx->batch.lanes;
     double best = 1e300;

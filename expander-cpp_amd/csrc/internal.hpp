@@ -90,7 +90,7 @@ struct BatchItem {
 // Batched prover engines: the whole proof in one workgroup (one launch per batch), or
 // the lockstep engine (one launch per step of every proof, each proof's step on as many
 // workgroups as its tables need).
-enum BatchEngine : int { kBatchPerProof = 1, kBatchLockstep = 2 };
+enum BatchEngine : int { kBatchPerProof = 1, kBatchLockstep = 2, kBatchLockstepGrouped = 4 };
 
 // Device scratch of the batched prover, grown to the largest batch seen.
 struct BatchScratch {
@@ -114,8 +114,11 @@ struct BatchScratch {
     std::vector<uint64_t> meta_key;
     uint64_t meta_used = 0, o_order = 0, o_tasks = 0, o_starts = 0;
     std::vector<uint32_t> meta_steps;
-    std::vector<uint32_t> meta_seed_step;   // lockstep: the step whose work seeds job i's transcript
+    std::vector<uint32_t> meta_seed_step;   // lockstep: the step (of its group) whose work seeds job i
     std::vector<uint32_t> meta_seed_order;  // the jobs in the order of their seed slots
+    std::vector<uint32_t> meta_need_order;  // the jobs in the order their seeds are needed
+    std::vector<uint32_t> meta_group_off;   // lockstep: group g's steps are [off[g], off[g + 1])
+    std::vector<uint32_t> meta_group_slot;  // group g's seed slots are [slot[g], slot[g + 1])
     uint32_t *claims_host = nullptr;  // claims (item-major, R * L each), then each item's transcript length
     uint64_t claims_cap = 0;
     uint64_t claims_words = 0;        // claim words of the last batch
@@ -152,7 +155,7 @@ struct DevScratch {
     uint32_t *s0 = nullptr;       // R * maxN        (phase-2 scalar h table)
     uint32_t *sA = nullptr;       // R * maxN/2
     uint32_t *sB = nullptr;       // R * maxN/4
-    uint32_t *partials = nullptr; // R * 3L * maxBlocks (value-major block partials)
+    uint32_t *partials = nullptr; // R * kPairVals L * maxBlocks (block-major block partials)
     uint32_t *chal = nullptr;     // D * R * 2 * 32 (rx then ry per layer and rep) + rz1 seed
     uint32_t *ab = nullptr;       // D * 2 (alpha, beta after each layer)
     uint32_t *claims = nullptr;   // R * L

@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(kThreads) k_forward(const uint32_t *__restrict
 // kEqChunk, both in LDS.
 constexpr int kEqChunk = 16;  // z values per thread
 template <int BS>
-__device__ void eq_block(uint32_t *__restrict__ o, const uint32_t *__restrict__ r1, const uint32_t *__restrict__ r2,
+__device__ __forceinline__ void eq_block(uint32_t *__restrict__ o, const uint32_t *__restrict__ r1, const uint32_t *__restrict__ r2,
                          const uint32_t *__restrict__ ab, uint32_t m, uint32_t chunk)
 {
     constexpr uint32_t LOG = BS == 64 ? 6 : BS == 128 ? 7 : BS == 256 ? 8 : 9;
@@ -247,7 +247,7 @@ constexpr uint32_t gx_rows() { return tile_lanes(BS) / L; }
 constexpr uint32_t kGxAccWords = 3 * 1024;
 
 template <int L, int BS>
-__device__ void gx_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ hg, uint64_t hg_rs,
+__device__ __forceinline__ void gx_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ hg, uint64_t hg_rs,
                         const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
                         const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N, uint32_t R,
                         uint32_t x0)
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint
 constexpr uint32_t kHyAccWords = 3 * 1024;
 
 template <int BS>
-__device__ void hy_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ s, uint64_t s_rs,
+__device__ __forceinline__ void hy_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ s, uint64_t s_rs,
                         const uint32_t *__restrict__ eqz, uint64_t eqz_rs, const uint32_t *__restrict__ eqx,
                         uint64_t eqx_rs, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
                         uint32_t N, uint32_t R, uint32_t y0)
@@ -608,7 +608,7 @@ __device__ __forceinline__ uint32_t msg_word(const uint32_t *sj, const uint32_t 
 // x claim); hb keeps L lead words in front of every message.  Every thread of the block
 // must call it; the block has at least 64 R threads.
 template <int L>
-__device__ void transcript_step(const uint32_t *sums, const TsArgs &a, uint32_t *r_out)
+__device__ __forceinline__ void transcript_step(const uint32_t *sums, const TsArgs &a, uint32_t *r_out)
 {
     constexpr uint32_t Q = 3 * L, S = L + Q;
     __shared__ uint32_t ts[10];
@@ -736,7 +736,7 @@ struct RoundParams {
     uint32_t *hdst;
     uint64_t hdst_rs;
     const uint32_t *rprev; // r_{k-2} of rep 0 (r_{k-1} follows it); rep stride kChalStride
-    uint32_t *partials;    // value-major: [rep][kPairVals L][blocks per rep]
+    uint32_t *partials;    // block-major: [rep][block][kPairVals L]
     uint32_t items;        // items of the pair: N_k / 4 (four rows of table k each)
     uint32_t R;
 };
@@ -823,45 +823,43 @@ __global__ void __launch_bounds__(kThreads) k_pair(RoundParams p)
         for (uint64_t w = (uint64_t)bx * kThreads + threadIdx.x; w < total; w += (uint64_t)nb * kThreads)
             pair_item<L, PH2, MODE>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
     }
-    block_reduce_vals<L, kThreads, kPairVals>(acc, p.partials + (uint64_t)rep * kPairVals * L * nb + bx, nb);
+    block_reduce_vals<L, kThreads, kPairVals>(acc, p.partials + ((uint64_t)rep * nb + bx) * kPairVals * L, 1);
 }
 
-// Adds the value-major block partials [R][V][nb] of k_pair (V = kPairVals L) or k_dot
-// (V = 3L) into sums[R][V] (LDS): wave w takes values w, w + 16, ..., kFinVals of them per
-// pass; lanes stride over the nb <= kMaxBlocksPerRep blocks.  Every load of a pass is issued
-// before the first add, so a pass costs one memory latency rather than one per value.
-// blockDim is kFinThreads.
+// Adds the block-major partials [nb][nvals] of k_pair (one repetition: nvals = kPairVals L)
+// or k_dot (nvals = R 3L; at most 144 values) into sums[nvals] (LDS).  Each block's partials
+// are one contiguous row, written whole by the block (a value-major layout would have every
+// block write one word of lines that neighbouring blocks on other XCDs also write, and each
+// XCD's L2 write its own partial copy of them back).  Thread t adds value t % nvals of
+// blocks t / nvals, t / nvals + G, ... (G = kFinThreads / nvals groups, at least 7): the
+// threads of a group read a row with coalesced loads, all of a thread's loads are issued
+// before the first add (one memory latency), and the G group sums meet in LDS.  blockDim is
+// kFinThreads; every thread must call it.
 constexpr uint32_t kFinThreads = 1024;
+constexpr uint32_t kFinMaxVals = 144;
 template <int L>
 __device__ __forceinline__ void reduce_partials(const uint32_t *__restrict__ partials, uint32_t nb, uint32_t nvals,
                                                 uint32_t *sums)
 {
-    constexpr uint32_t W = kFinThreads / 64;
-    constexpr uint32_t kFinVals = (3 * 3 * L + W - 1) / W;
-    constexpr uint32_t kBlk = kMaxBlocksPerRep / 64;
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (uint32_t base = 0; base < nvals; base += kFinVals * W) {
-        uint32_t x[kFinVals][kBlk];
+    constexpr uint32_t kGroupsMin = kFinThreads / kFinMaxVals;
+    constexpr uint32_t kPerThread = (kMaxBlocksPerRep + kGroupsMin - 1) / kGroupsMin;
+    __shared__ unsigned long long part[kFinThreads];
+    const uint32_t t = threadIdx.x, G = kFinThreads / nvals, v = t % nvals, grp = t / nvals;
+    uint32_t x[kPerThread];
 #pragma unroll
-        for (uint32_t i = 0; i < kFinVals; i++) {
-            const uint32_t v = base + wave + i * W;
+    for (uint32_t i = 0; i < kPerThread; i++) {
+        const uint32_t b = grp + i * G;
+        x[i] = (grp < G && b < nb) ? partials[(uint64_t)b * nvals + v] : 0u;
+    }
+    unsigned long long a = 0;
 #pragma unroll
-            for (uint32_t c = 0; c < kBlk; c++) {
-                const uint32_t b = lane + 64 * c;
-                x[i][c] = (v < nvals && b < nb) ? partials[(uint64_t)v * nb + b] : 0u;
-            }
-        }
-#pragma unroll
-        for (uint32_t i = 0; i < kFinVals; i++) {
-            uint64_t a = 0;
-#pragma unroll
-            for (uint32_t c = 0; c < kBlk; c++) a += x[i][c];
-            uint32_t y = m31_reduce64(a);
-#pragma unroll
-            for (uint32_t off = 1; off < 64; off <<= 1) y = m31_add(y, __shfl_xor(y, off, 64));
-            const uint32_t v = base + wave + i * W;
-            if (lane == 0 && v < nvals) sums[v] = y;
-        }
+    for (uint32_t i = 0; i < kPerThread; i++) a += x[i];
+    part[t] = a;
+    __syncthreads();
+    if (t < nvals) {
+        unsigned long long sum = 0;
+        for (uint32_t g = 0; g < G; g++) sum += part[g * nvals + t];
+        sums[t] = m31_reduce64(sum);
     }
     __syncthreads();
 }
@@ -875,7 +873,7 @@ __device__ __forceinline__ void reduce_partials(const uint32_t *__restrict__ par
 // transcript state: the host tracks len0 and hs0, and the workgroup of the last rep writes
 // the state after round k+1.  ts.k is round k.
 template <int L>
-__device__ void pair_fin_rep_body(const uint32_t *__restrict__ partials, uint32_t nb, const TsArgs &ts,
+__device__ __forceinline__ void pair_fin_rep_body(const uint32_t *__restrict__ partials, uint32_t nb, const TsArgs &ts,
                                   uint32_t len0, uint32_t hs0, uint32_t rep)
 {
     constexpr uint32_t Q = 3 * L, V = kPairVals * L;
@@ -961,7 +959,7 @@ struct TailParams {
 // Rounds p.k0 .. p.n - 1 of one phase, run by one workgroup (blockDim a multiple of 256,
 // at least 256 R): group j = threads [256 j, 256 j + 256) works on rep j.
 template <int L, bool PH2>
-__device__ void phase_rounds(const TailParams &p)
+__device__ __forceinline__ void phase_rounds(const TailParams &p)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
     __shared__ uint32_t red[4][4][3 * L];
@@ -1027,7 +1025,7 @@ struct TailRepParams {
 
 // The body of k_tail_rep for repetition `rep`; blockDim is NT (a multiple of 64, >= 3L).
 template <int L, bool PH2, uint32_t NT = kRepThreads>
-__device__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
+__device__ __forceinline__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L, W = NT / 64;
     static_assert(NT % 64 == 0 && NT >= Q, "tail workgroup shape");
@@ -1121,8 +1119,8 @@ __global__ void __launch_bounds__(kRepThreads) k_tail_rep(TailRepParams q)
 }
 
 // Dot product sum_z eq[rep][z] * vals[z] (lanes): claimed_v = eval_multilinear(output, rz1)
-// (include/LinearGKR/gkr.hpp:51-55).  Each block leaves value-major partials (the p0 slot
-// of the round layout; the other slots are zero) that k_dot_fin adds.
+// (include/LinearGKR/gkr.hpp:51-55).  Each block leaves one row of partials [block][rep][3L]
+// (the p0 slot of the round layout; the other slots are zero) that k_dot_fin adds.
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_dot(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ eq,
                                                   uint64_t eq_rs, uint32_t M, uint32_t *__restrict__ partials)
@@ -1139,7 +1137,7 @@ __global__ void __launch_bounds__(kThreads) k_dot(const uint32_t *__restrict__ v
         const uint4 t = mul4s(ld4(vals + z * L + 4 * g), e[z]);
         acc[0][0] += t.x; acc[0][1] += t.y; acc[0][2] += t.z; acc[0][3] += t.w;
     }
-    group_reduce<L>(acc, red, partials + (uint64_t)rep * Q * gridDim.x + blockIdx.x, gridDim.x);
+    group_reduce<L>(acc, red, partials + ((uint64_t)blockIdx.x * gridDim.y + rep) * Q, 1);
 }
 
 template <int L>
@@ -1489,7 +1487,7 @@ __device__ __forceinline__ void block_reduce_to(uint64_t (&acc)[3][4], uint32_t 
 // Workgroup lb of a round-pair task (rep = lb / nb): k_pair on a block range, V lanes per
 // thread.
 template <int L, int BS, bool PH2, int MODE, int V>
-__device__ void ls_pair(const RoundParams &p, uint32_t nb, uint32_t lb)
+__device__ __forceinline__ void ls_pair(const RoundParams &p, uint32_t nb, uint32_t lb)
 {
     constexpr uint32_t LG = L / V;
     const uint2 rb_ = pair_block(lb, p.R);  // pair tasks start on multiples of 8 workgroups
@@ -1524,7 +1522,7 @@ __device__ void ls_pair(const RoundParams &p, uint32_t nb, uint32_t lb)
         for (uint64_t w = (uint64_t)bx * BS + threadIdx.x; w < total; w += (uint64_t)nb * BS)
             pair_item<L, PH2, MODE, V>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
     }
-    block_reduce_vals<L, BS, kPairVals, V>(acc, p.partials + (uint64_t)rep * kPairVals * L * nb + bx, nb);
+    block_reduce_vals<L, BS, kPairVals, V>(acc, p.partials + ((uint64_t)rep * nb + bx) * kPairVals * L, 1);
 }
 
 template <int L, int BS>
@@ -1594,7 +1592,7 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
             const uint4 v = mul4s(ld4(a.vals + z * L + 4 * g), e[z]);
             acc[0][0] += v.x; acc[0][1] += v.y; acc[0][2] += v.z; acc[0][3] += v.w;
         }
-        block_reduce_to<L, BS>(acc, a.partials + (uint64_t)rep * Q * T.nb + bx, T.nb);
+        block_reduce_to<L, BS>(acc, a.partials + ((uint64_t)bx * R + rep) * Q, 1);
         break;
     }
     default:
@@ -1650,7 +1648,7 @@ void ls_launch_pair(uint32_t BS, uint32_t V, uint32_t blocks, hipStream_t st, co
 }
 
 template <int L>
-__device__ TsArgs ls_ts(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t R, int hash_kind)
+__device__ __forceinline__ TsArgs ls_ts(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t R, int hash_kind)
 {
     TsArgs ts{};
     ts.R = R;
@@ -1668,7 +1666,7 @@ __device__ TsArgs ls_ts(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t 
 // The small-table rounds of layer l, phase `phase`, from round k0 on (sources as in the
 // single-proof path: table k - 1 feeds round k, table 0 is the layer input and hg0 / s0).
 template <int L>
-__device__ TailParams ls_tail_params(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t R, int hash_kind,
+__device__ __forceinline__ TailParams ls_tail_params(const BatchJob &J, uint32_t l, uint32_t phase, uint32_t R, int hash_kind,
                                      uint32_t k0, uint32_t dbl)
 {
     const LayerDev &ly = J.layers[l];
@@ -1731,7 +1729,7 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restri
 // phase 2.  The same work as phase_rounds from k0 = n - 1, for any blockDim of at least
 // 64 R threads.
 template <int L, bool PH2>
-__device__ void last_round_body(const TailParams &p)
+__device__ __forceinline__ void last_round_body(const TailParams &p)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
     __shared__ uint32_t sums[3 * Q], rr[3];
@@ -2382,6 +2380,7 @@ uint64_t batch_tables_u32(const DevCircuit &dc, uint32_t R)
 
 constexpr uint32_t kLsGatherItems = 4;  // work items per thread of a gather task
 constexpr uint32_t kLsMaxJobs = 2048;   // at most two work tasks per proof in a step, 4096 per step
+constexpr uint32_t kLsGroups = 3;       // grouped lockstep engine: groups of proofs (one stream each)
 
 uint32_t ls_grid(uint64_t work, uint32_t bs)
 {
@@ -2618,9 +2617,20 @@ struct LsList {
 enum { kLsWork = 0, kLsDotFin, kLsPairs, kLsPairFin, kLsChain, kLsLists };
 struct LsStep {
     LsList l[kLsLists];
+    // wait[x]: the lanes (bit 0 main, 1 pairs, 2 chains) whose previous work lane x's ops of
+    // this step follow: a proof's op at step t depends only on its op at step t - 1
+    uint32_t wait[3];
 };
 
 bool ls_is_chain(uint32_t op) { return op == kOpTailRep || op == kOpLast; }
+
+// The lane an entry of a proof's schedule runs on: 0 main, 1 pairs, 2 chains.
+int ls_lane(const LsEntry &e)
+{
+    if (e.nw && e.w[0].op == kOpPair) return 1;
+    if (!e.nw && e.nf && ls_is_chain(e.f.op)) return 2;
+    return 0;
+}
 
 // The device addresses of every job's tables in the batch pool (job i's tables start at
 // pool + job_off[i]; its tail and claims sit in the tail and claim areas).
@@ -2662,15 +2672,18 @@ void batch_job_table(const std::vector<BatchItem> &items, uint32_t R, uint32_t *
 
 // The lockstep schedule of a batch: for every step its task lists (tasks and the prefix
 // arrays of their first workgroups) and, per job, the step that installs its transcript seed.
-void ls_build_schedule(const std::vector<BatchItem> &items, const std::vector<BatchJob> &hjobs, uint32_t R,
-                       uint32_t BS, uint32_t LPT, std::vector<LsTask> &tasks, std::vector<uint32_t> &starts,
-                       std::vector<LsStep> &steps, std::vector<uint32_t> &seed_step)
+void ls_build_schedule(const std::vector<BatchItem> &items, const std::vector<BatchJob> &hjobs,
+                       const std::vector<uint32_t> &members, uint32_t R, uint32_t BS, uint32_t LPT,
+                       std::vector<LsTask> &tasks, std::vector<uint32_t> &starts, std::vector<LsStep> &steps,
+                       std::vector<uint32_t> &seed_step)
 {
-    const size_t nj = items.size();
+    // the schedule of the jobs in `members` (indices into items / hjobs); index m below is a
+    // member, members[m] its job
+    const size_t nj = members.size();
     // The transcript position after the host prefix follows from the shapes alone: the 32
     // grind bytes are in the buffer, hashed once if output challenges were drawn.
     auto ts_hashed = [&](size_t i) -> uint64_t {
-        const DevCircuit &dc = *items[i].dc;
+        const DevCircuit &dc = *items[members[i]].dc;
         return (uint64_t)R * dc.layers[dc.D - 1].m ? 32 : 0;
     };
     // Every proof's forward pass first (steps [0, D)): it needs no transcript.  Then
@@ -2682,19 +2695,19 @@ void ls_build_schedule(const std::vector<BatchItem> &items, const std::vector<Ba
     std::vector<size_t> base(nj);
     size_t dmax = 0, smax = 0, ntask = 0;
     for (size_t i = 0; i < nj; i++) {
-        const uint32_t D = items[i].dc->D;
-        ops[i].reserve((size_t)D * (8 + 2 * items[i].dc->max_n));
-        ls_job_ops(*items[i].dc, hjobs[i], (uint32_t)i, R, BS, LPT, 32, ts_hashed(i), ops[i]);
+        const uint32_t D = items[members[i]].dc->D;
+        ops[i].reserve((size_t)D * (8 + 2 * items[members[i]].dc->max_n));
+        ls_job_ops(*items[members[i]].dc, hjobs[members[i]], members[i], R, BS, LPT, 32, ts_hashed(i), ops[i]);
         dmax = std::max<size_t>(dmax, D);
         smax = std::max(smax, ops[i].size() - D);
         ntask += 2 * ops[i].size() + 1;
     }
     const size_t nsteps = dmax + smax;
     for (size_t i = 0; i < nj; i++) {
-        const uint32_t D = items[i].dc->D;
+        const uint32_t D = items[members[i]].dc->D;
         base[i] = nsteps - (ops[i].size() - D);  // >= dmax >= D
-        seed_step[i] = (uint32_t)(base[i] - 1);
-        LsTask sd = ls_task(kOpSeed, (uint32_t)i, 0);
+        seed_step[members[i]] = (uint32_t)(base[i] - 1);
+        LsTask sd = ls_task(kOpSeed, members[i], 0);
         sd.nb = 1;
         if (base[i] == D) {  // the seed shares the last forward step
             LsEntry &e = ops[i][D - 1];
@@ -2706,7 +2719,7 @@ void ls_build_schedule(const std::vector<BatchItem> &items, const std::vector<Ba
     }
     // job i's entry at step s
     auto at = [&](size_t i, size_t s) -> const LsEntry * {
-        const uint32_t D = items[i].dc->D;
+        const uint32_t D = items[members[i]].dc->D;
         if (s < D) return &ops[i][s];
         if (s + 1 == base[i]) return &seed_entry[i];
         if (s >= base[i]) return &ops[i][D + (s - base[i])];
@@ -2791,6 +2804,10 @@ void ls_build_schedule(const std::vector<BatchItem> &items, const std::vector<Ba
              fin_blocks);
         list(kLsChain, [](const LsEntry &e, auto &&put) { if (e.nf && ls_is_chain(e.f.op)) put(e.f); },
              fin_blocks);
+        for (size_t i = 0; s > 0 && i < nj; i++) {
+            const LsEntry *e = at(i, s), *p = at(i, s - 1);
+            if (e && p) S.wait[ls_lane(*e)] |= 1u << ls_lane(*p);
+        }
         steps.push_back(S);
     }
 }
@@ -2807,6 +2824,9 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     if (R > 3) return GKRX_ERR_CONFIG;
     const uint32_t L = items[0].dc->L;
     const bool lockstep = bs.engine != kBatchPerProof && nj <= kLsMaxJobs;
+    // the grouped lockstep engine: the proofs in kLsGroups groups, each group's steps on a
+    // stream of its own (no events between them); otherwise one group on three op lanes
+    const bool grouped = lockstep && bs.engine == kBatchLockstepGrouped;
     const uint32_t BS = (bs.block == 64 || bs.block == 128 || bs.block == 512) ? bs.block : 256u;
     const uint32_t LPT = (bs.lanes == 1 || bs.lanes == 2) ? bs.lanes : 4u;  // round-pair lanes per thread
     // sizes of every job's scratch, one pool
@@ -2845,7 +2865,8 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     // The job table and the schedule depend only on the circuits' shapes, their device
     // addresses and the pool: a batch with the same key reuses the metadata already on the
     // device.
-    std::vector<uint64_t> key = {(uint64_t)lockstep, BS, LPT, L, R, nj, (uint64_t)(uintptr_t)bs.pool};
+    std::vector<uint64_t> key = {(uint64_t)lockstep + 2 * (uint64_t)grouped, BS, LPT, L, R, nj,
+                                 (uint64_t)(uintptr_t)bs.pool};
     for (size_t i = 0; i < nj; i++) {
         const DevCircuit &dc = *items[i].dc;
         key.insert(key.end(), {(uint64_t)(uintptr_t)dc.values, (uint64_t)(uintptr_t)dc.csr_pool,
@@ -2880,15 +2901,49 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             std::stable_sort(work.begin(), work.end(), [](auto &x, auto &y) { return x.first > y.first; });
             horder.resize(nj);
             for (size_t i = 0; i < nj; i++) horder[i] = work[i].second;
-        } else {
-            ls_build_schedule(items, hjobs, R, BS, LPT, tasks, starts, steps, bs.meta_seed_step);
         }
-        // seed slots in the order of need
+        // groups of proofs: the heaviest first, each to the group with the fewest table rows
+        const uint32_t G = grouped ? (uint32_t)std::min<size_t>(kLsGroups, nj) : 1u;
+        std::vector<uint32_t> group(nj, 0);
+        if (G > 1) {
+            std::vector<std::pair<uint64_t, uint32_t>> work(nj);
+            for (size_t i = 0; i < nj; i++) {
+                uint64_t w = 0;
+                for (uint32_t l = 0; l < items[i].dc->D; l++) w += 1ull << items[i].dc->layers[l].n;
+                work[i] = {w, (uint32_t)i};
+            }
+            std::stable_sort(work.begin(), work.end(), [](auto &x, auto &y) { return x.first > y.first; });
+            std::vector<uint64_t> load(G, 0);
+            for (auto &w : work) {
+                const uint32_t g = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+                group[w.second] = g;
+                load[g] += w.first;
+            }
+        }
+        bs.meta_group_off.assign(1, 0);
+        if (lockstep)
+            for (uint32_t g = 0; g < G; g++) {
+                std::vector<uint32_t> members;
+                for (size_t i = 0; i < nj; i++)
+                    if (group[i] == g) members.push_back((uint32_t)i);
+                ls_build_schedule(items, hjobs, members, R, BS, LPT, tasks, starts, steps, bs.meta_seed_step);
+                bs.meta_group_off.push_back((uint32_t)steps.size());
+            }
+        // seed slots: by group, each group's in the order of need (a group uploads its own
+        // slots, a contiguous range); the prefixes are computed in the order of need overall
         std::vector<uint32_t> &sorder = bs.meta_seed_order;
         sorder.resize(nj);
         for (size_t i = 0; i < nj; i++) sorder[i] = (uint32_t)i;
-        std::stable_sort(sorder.begin(), sorder.end(),
+        std::vector<uint32_t> &norder = bs.meta_need_order;
+        norder = sorder;
+        std::stable_sort(norder.begin(), norder.end(),
                          [&](uint32_t a, uint32_t b) { return bs.meta_seed_step[a] < bs.meta_seed_step[b]; });
+        std::stable_sort(sorder.begin(), sorder.end(), [&](uint32_t a, uint32_t b) {
+            return group[a] != group[b] ? group[a] < group[b] : bs.meta_seed_step[a] < bs.meta_seed_step[b];
+        });
+        bs.meta_group_slot.assign(G + 1, 0);
+        for (size_t i = 0; i < nj; i++) bs.meta_group_slot[group[i] + 1]++;
+        for (uint32_t g = 0; g < G; g++) bs.meta_group_slot[g + 1] += bs.meta_group_slot[g];
         for (size_t q = 0; q < nj; q++) hjobs[sorder[q]].seed = seeds + q * seed_u32;
         // device metadata: [jobs][order][tasks][starts]; the pinned staging adds the seeds
         bs.o_order = align256(nj * sizeof(BatchJob));
@@ -2923,13 +2978,14 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     static_assert(sizeof(LsStep) % 4 == 0, "step records are whole words");
     std::vector<LsStep> steps(bs.meta_steps.size() / (sizeof(LsStep) / 4));
     if (!steps.empty()) memcpy(steps.data(), bs.meta_steps.data(), steps.size() * sizeof(LsStep));
-    // The seeds change with every batch.  upload(q) copies the records of slots [seeded, q)
-    // (waiting for their prefixes first) through the pinned staging after the metadata.
+    // The seeds change with every batch.  upload(seeded, q, sx) copies the records of slots
+    // [seeded, q) (waiting for their prefixes first) through the pinned staging after the
+    // metadata, on stream sx.
     const std::vector<uint32_t> &sorder = bs.meta_seed_order;
+    const std::vector<uint32_t> &gslot = bs.meta_group_slot;
     uint32_t *stage = reinterpret_cast<uint32_t *>(bs.host_meta + bs.meta_used);
-    size_t seeded = 0;
-    if (pre) pre->start(sorder);
-    auto upload = [&](size_t q) -> int {
+    if (pre) pre->start(bs.meta_need_order);
+    auto upload = [&](size_t &seeded, size_t q, hipStream_t sx) -> int {
         if (q <= seeded) return GKRX_OK;
         for (size_t k = seeded; k < q; k++) {
             const size_t i = sorder[k];
@@ -2944,13 +3000,15 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
             memcpy(sd + R * kChalStride + 10, items[i].grind, 32);
         }
         GKRX_HIP_CHECK(hipMemcpyAsync(seeds + seeded * seed_u32, stage + seeded * seed_u32,
-                                      (q - seeded) * seed_u32 * 4, hipMemcpyHostToDevice, st));
+                                      (q - seeded) * seed_u32 * 4, hipMemcpyHostToDevice, sx));
         seeded = q;
         return GKRX_OK;
     };
     // with the prefixes still in progress, the lockstep engine uploads them step by step
-    if (!(pre && lockstep))
-        if (int rc = upload(nj)) return rc;
+    size_t seeded_all = 0;
+    const bool seeds_now = !(pre && lockstep);
+    if (seeds_now)
+        if (int rc = upload(seeded_all, nj, st)) return rc;
     const BatchJob *djobs = reinterpret_cast<const BatchJob *>(bs.meta);
     if (!lockstep) {
         const uint32_t *dorder = reinterpret_cast<const uint32_t *>(bs.meta + bs.o_order);
@@ -2972,52 +3030,28 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
     } else {
         const LsTask *dtasks = reinterpret_cast<const LsTask *>(bs.meta + bs.o_tasks);
         const uint32_t *dstarts = reinterpret_cast<const uint32_t *>(bs.meta + bs.o_starts);
-        // Three lanes: 0 main (wide work, claims), 1 pairs (+ their transcript ends), 2 chains.
-        // With chain_stream they run on three streams: at step t each lane first waits for
-        // what the other lanes recorded at the end of steps before t (a proof's op at step t
-        // depends on its op at step t - 1, on whichever lane that ran), and every lane that
-        // launched records its event after all lanes of step t are enqueued.  Otherwise (and
-        // with profiling on, so that each op class is timed alone) every lane is the main
-        // stream.
-        hipStream_t ls_st[3] = {st, st, st};
-        hipEvent_t ls_ev[3] = {nullptr, nullptr, nullptr};
-        uint64_t rec[3] = {0, 0, 0}, seen[3][3] = {};
         // timing any class but the round pairs (whose launches record their own start and
         // end) puts every lane on one stream and times each op class of the work alone
         const bool serial = pf && (pf->mask & ~(1ull << kKLsPair)) != 0;
         const bool multi = bs.chain_stream && !serial;
-        if (multi) {
-            if (!bs.aux) {
-                GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux, hipStreamNonBlocking));
-                GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux2, hipStreamNonBlocking));
-                GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_main, hipEventDisableTiming));
-                GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_aux, hipEventDisableTiming));
-                GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_aux2, hipEventDisableTiming));
-            }
-            ls_st[1] = bs.aux2;
-            ls_st[2] = bs.aux;
-            ls_ev[0] = bs.ev_main;
-            ls_ev[1] = bs.ev_aux2;
-            ls_ev[2] = bs.ev_aux;
+        if (multi && !bs.aux) {
+            GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux, hipStreamNonBlocking));
+            GKRX_HIP_CHECK(hipStreamCreateWithFlags(&bs.aux2, hipStreamNonBlocking));
+            GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_main, hipEventDisableTiming));
+            GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_aux, hipEventDisableTiming));
+            GKRX_HIP_CHECK(hipEventCreateWithFlags(&bs.ev_aux2, hipEventDisableTiming));
         }
-        // the lanes' fork point: every lane's first wait is for the main stream's work so far
-        auto fork = [&]() -> int {
-            if (!multi) return GKRX_OK;
-            for (int x = 0; x < 3; x++) {
-                rec[x] = x == 0;
-                for (int o = 0; o < 3; o++) seen[x][o] = 0;
-            }
-            GKRX_HIP_CHECK(hipEventRecord(ls_ev[0], st));
-            return GKRX_OK;
-        };
-        auto join = [&](int x) -> int {  // lane x waits for the other lanes' latest records
-            if (!multi) return GKRX_OK;
-            for (int o = 0; o < 3; o++)
-                if (o != x && rec[o] > seen[x][o]) {
-                    GKRX_HIP_CHECK(hipStreamWaitEvent(ls_st[x], ls_ev[o], 0));
-                    seen[x][o] = rec[o];
-                }
-            return GKRX_OK;
+        hipStream_t streams[3] = {st, multi ? bs.aux2 : st, multi ? bs.aux : st};
+        hipEvent_t events[3] = {bs.ev_main, bs.ev_aux2, bs.ev_aux};
+        const std::vector<uint32_t> &goff = bs.meta_group_off;
+        const uint32_t G = (uint32_t)goff.size() - 1;
+        // next slot to upload, per group (all uploaded already when the prefixes were ready)
+        std::vector<size_t> gseeded(gslot.begin() + (seeds_now ? 1 : 0), gslot.end() - (seeds_now ? 0 : 1));
+        // the seeds of the SEED tasks of group g's step t, on stream sx
+        auto seed_step_t = [&](uint32_t g, size_t t, hipStream_t sx) -> int {
+            size_t q = gseeded[g];
+            while (q < gslot[g + 1] && bs.meta_seed_step[sorder[q]] <= t) q++;
+            return upload(gseeded[g], q, sx);
         };
         auto fin = [&](const LsList &ls, hipStream_t sx, int cls) {
             timed(pf, cls, sx, [&] {
@@ -3029,80 +3063,124 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
                                        dstarts + ls.start, ls.n, R, hash_kind);
             });
         };
-        // enqueues steps [s0, s1), each step's seeds uploaded just before it
-        auto run_steps = [&](size_t s0, size_t s1) -> int {
-            for (size_t s = s0; s < s1; s++) {
-                const LsStep &S = steps[s];
+        auto chain = [&](const LsList &ls, hipStream_t cs) {
+            timed(pf, kKLsChain, cs, [&] {
+                if (L == 8)
+                    hipLaunchKernelGGL(k_ls_chain<8>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
+                                       dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
+                else
+                    hipLaunchKernelGGL(k_ls_chain<16>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
+                                       dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
+            });
+        };
+        // the round pairs of a step, then their transcript ends
+        auto pairs = [&](const LsStep &S, hipStream_t ps) {
+            const LsList &ls = S.l[kLsPairs];
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (pf && !pf->kernel_events(kKLsPair, e0, e1)) e0 = e1 = nullptr;
+            if (L == 8) ls_launch_pair<8>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n, e0, e1);
+            else ls_launch_pair<16>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n, e0, e1);
+            if (S.l[kLsPairFin].n) fin(S.l[kLsPairFin], ps, kKPairFin);
+        };
+        // the wide work of a step (one launch; profiling: one per run of tasks of an op
+        // class), then the claims
+        auto work = [&](const LsStep &S, hipStream_t ws) {
+            const LsList &W = S.l[kLsWork];
+            auto go = [&](uint32_t blocks, uint32_t base) {
+                if (L == 8) ls_launch_work<8>(BS, blocks, ws, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
+                else ls_launch_work<16>(BS, blocks, ws, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
+            };
+            if (W.n && !serial) {
+                go(W.blocks, 0);
+            } else if (W.n) {
+                const LsTask *ht = reinterpret_cast<const LsTask *>(bs.host_meta + bs.o_tasks) + W.task;
+                const uint32_t *hs = reinterpret_cast<const uint32_t *>(bs.host_meta + bs.o_starts) + W.start;
+                for (uint32_t a = 0; a < W.n;) {
+                    uint32_t b = a + 1;
+                    while (b < W.n && ls_op_class(ht[b].op) == ls_op_class(ht[a].op)) b++;
+                    timed(pf, ls_op_class(ht[a].op), ws, [&] { go(hs[b] - hs[a], hs[a]); });
+                    a = b;
+                }
+            }
+            if (S.l[kLsDotFin].n) fin(S.l[kLsDotFin], ws, kKLsFin);
+        };
+        if (grouped) {
+            // group g's steps in order on streams[g], every op of a step after the step
+            // before: the groups share no tables, so nothing orders one group after another
+            // except the fork from the main stream (the metadata copies) and the final join
+            if (multi)
+                for (uint32_t g = 1; g < G; g++) {
+                    if (g == 1) GKRX_HIP_CHECK(hipEventRecord(events[0], st));
+                    GKRX_HIP_CHECK(hipStreamWaitEvent(streams[g], events[0], 0));
+                }
+            size_t tmax = 0;
+            for (uint32_t g = 0; g < G; g++) tmax = std::max<size_t>(tmax, goff[g + 1] - goff[g]);
+            for (size_t t = 0; t < tmax; t++)
+                for (uint32_t g = 0; g < G; g++) {
+                    if (t >= goff[g + 1] - goff[g]) continue;
+                    const LsStep &S = steps[goff[g] + t];
+                    hipStream_t sx = streams[g];
+                    if (int rc = seed_step_t(g, t, sx)) return rc;
+                    work(S, sx);
+                    if (S.l[kLsPairs].n) pairs(S, sx);
+                    if (S.l[kLsChain].n) chain(S.l[kLsChain], sx);
+                }
+            if (multi)
+                for (uint32_t g = 1; g < G; g++) {
+                    GKRX_HIP_CHECK(hipEventRecord(events[g], streams[g]));
+                    GKRX_HIP_CHECK(hipStreamWaitEvent(st, events[g], 0));
+                }
+        } else {
+            // Three lanes: 0 main (wide work, claims), 1 pairs (+ their transcript ends), 2
+            // chains.  With chain_stream they run on three streams: at step t each lane first
+            // waits for what the lanes its ops follow (LsStep::wait) recorded at the end of
+            // steps before t, and every lane that launched records its event after all lanes
+            // of step t are enqueued.  Otherwise (and with profiling on, so that each op class
+            // is timed alone) every lane is the main stream.
+            uint64_t rec[3] = {1, 0, 0}, seen[3][3] = {};
+            if (multi) GKRX_HIP_CHECK(hipEventRecord(events[0], st));  // the fork: st's work so far
+            auto join = [&](int x, uint32_t mask) -> int {
+                if (!multi) return GKRX_OK;
+                for (int o = 0; o < 3; o++)
+                    if (o != x && ((mask >> o) & 1) && rec[o] > seen[x][o]) {
+                        GKRX_HIP_CHECK(hipStreamWaitEvent(streams[x], events[o], 0));
+                        seen[x][o] = rec[o];
+                    }
+                return GKRX_OK;
+            };
+            const size_t nsteps = goff[1];
+            for (size_t s2 = 0; s2 < nsteps; s2++) {
+                const LsStep &S = steps[s2];
                 bool launched[3] = {false, false, false};
-                // chains
                 if (S.l[kLsChain].n) {
-                    const LsList &ls = S.l[kLsChain];
-                    hipStream_t cs = ls_st[2];
-                    if (int rc = join(2)) return rc;
-                    timed(pf, kKLsChain, cs, [&] {
-                        if (L == 8)
-                            hipLaunchKernelGGL(k_ls_chain<8>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
-                                               dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
-                        else
-                            hipLaunchKernelGGL(k_ls_chain<16>, dim3(ls.blocks), dim3(kChainThreads), 0, cs, djobs,
-                                               dtasks + ls.task, dstarts + ls.start, ls.n, R, hash_kind);
-                    });
+                    if (int rc = join(2, S.wait[2])) return rc;
+                    chain(S.l[kLsChain], streams[2]);
                     launched[2] = true;
                 }
-                // round pairs and their transcript ends
                 if (S.l[kLsPairs].n) {
-                    const LsList &ls = S.l[kLsPairs];
-                    hipStream_t ps = ls_st[1];
-                    if (int rc = join(1)) return rc;
-                    hipEvent_t e0 = nullptr, e1 = nullptr;
-                    if (pf && !pf->kernel_events(kKLsPair, e0, e1)) e0 = e1 = nullptr;
-                    if (L == 8) ls_launch_pair<8>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n, e0, e1);
-                    else ls_launch_pair<16>(BS, LPT, ls.blocks, ps, dtasks + ls.task, dstarts + ls.start, ls.n, e0, e1);
-                    if (S.l[kLsPairFin].n) fin(S.l[kLsPairFin], ps, kKPairFin);
+                    if (int rc = join(1, S.wait[1])) return rc;
+                    pairs(S, streams[1]);
                     launched[1] = true;
                 }
-                // wide work and claims
-                const LsList &W = S.l[kLsWork];
-                if (W.n || S.l[kLsDotFin].n) {
+                if (S.l[kLsWork].n || S.l[kLsDotFin].n) {
                     // the seeds of this step's SEED tasks (the other lanes are enqueued already)
-                    size_t q = seeded;
-                    while (q < nj && bs.meta_seed_step[sorder[q]] <= s) q++;
-                    if (int rc = upload(q)) return rc;
-                    if (int rc = join(0)) return rc;
-                    auto work = [&](uint32_t blocks, uint32_t base) {
-                        if (L == 8) ls_launch_work<8>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
-                        else ls_launch_work<16>(BS, blocks, st, djobs, dtasks + W.task, dstarts + W.start, W.n, R, base);
-                    };
-                    if (W.n && !serial) {
-                        work(W.blocks, 0);
-                    } else if (W.n) {
-                        // profiling: one launch per run of tasks of one op class
-                        const LsTask *ht = reinterpret_cast<const LsTask *>(bs.host_meta + bs.o_tasks) + W.task;
-                        const uint32_t *hs = reinterpret_cast<const uint32_t *>(bs.host_meta + bs.o_starts) + W.start;
-                        for (uint32_t a = 0; a < W.n;) {
-                            uint32_t b = a + 1;
-                            while (b < W.n && ls_op_class(ht[b].op) == ls_op_class(ht[a].op)) b++;
-                            timed(pf, ls_op_class(ht[a].op), st, [&] { work(hs[b] - hs[a], hs[a]); });
-                            a = b;
-                        }
-                    }
-                    if (S.l[kLsDotFin].n) fin(S.l[kLsDotFin], st, kKLsFin);
+                    if (int rc = seed_step_t(0, s2, st)) return rc;
+                    if (int rc = join(0, S.wait[0])) return rc;
+                    work(S, st);
                     launched[0] = true;
                 }
                 if (multi)
                     for (int x = 0; x < 3; x++)
                         if (launched[x]) {
-                            GKRX_HIP_CHECK(hipEventRecord(ls_ev[x], ls_st[x]));
+                            GKRX_HIP_CHECK(hipEventRecord(events[x], streams[x]));
                             rec[x]++;
                         }
             }
-            return GKRX_OK;
-        };
-        if (int rc = fork()) return rc;
-        if (int rc = run_steps(0, steps.size())) return rc;
-        // the copies below (and the next batch, which synchronizes st) follow every lane
-        if (int rc = join(0)) return rc;
-        if (seeded != nj) return GKRX_ERR_STATE;  // every proof has a SEED task
+            // the copies below (and the next batch, which synchronizes st) follow every lane
+            if (int rc = join(0, 7u)) return rc;
+        }
+        for (uint32_t g = 0; g < G; g++)
+            if (gseeded[g] != gslot[g + 1]) return GKRX_ERR_STATE;  // every proof has a SEED task
     }
     hipLaunchKernelGGL(k_batch_tlen, dim3((uint32_t)((nj + 255) / 256)), dim3(256), 0, st, djobs, (uint32_t)nj, tlens);
     GKRX_HIP_CHECK(hipGetLastError());

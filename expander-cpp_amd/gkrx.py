@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libgkrx.so")
 
 FIELD_M31, FIELD_BABYBEAR, FIELD_BN254 = 0, 1, 2
 FS_SHA256, FS_KECCAK256 = 0, 1
-BATCH_PER_PROOF, BATCH_LOCKSTEP = 1, 2
+BATCH_PER_PROOF, BATCH_LOCKSTEP, BATCH_LOCKSTEP_GROUPS = 1, 2, 4
 PC_RAW = 0
 P31 = (1 << 31) - 1
 
@@ -331,7 +331,7 @@ class Context:
             pass
 
     def set_batch_engine(self, engine: int, block: int = 256) -> None:
-        """BATCH_PER_PROOF or BATCH_LOCKSTEP (block 64, 128, 256 or 512) for prove_batch."""
+        """BATCH_PER_PROOF, BATCH_LOCKSTEP or BATCH_LOCKSTEP_GROUPS (block 64, 128, 256 or 512) for prove_batch."""
         _check(lib().gkrx_ctx_set_batch_engine(self._h, engine, block), "gkrx_ctx_set_batch_engine")
 
     def batch_engine(self):
@@ -345,10 +345,11 @@ class Context:
         n = len(circuits)
         hs = (C.c_void_p * n)(*[c._h.value for c in circuits])
         e, b, v = C.c_int(), C.c_uint32(), C.c_uint32()
-        ms = (C.c_double * 13)()
+        ms = (C.c_double * 25)()
         _check(lib().gkrx_batch_autotune(self._h, n, hs, C.byref(e), C.byref(b), C.byref(v), ms),
                "gkrx_batch_autotune")
-        keys = [(BATCH_PER_PROOF, 256, 4)] + [(BATCH_LOCKSTEP, x, y) for x in (64, 128, 256, 512) for y in (1, 2, 4)]
+        keys = [(BATCH_PER_PROOF, 256, 4)] + [(e, x, y) for e in (BATCH_LOCKSTEP, BATCH_LOCKSTEP_GROUPS)
+                                              for x in (64, 128, 256, 512) for y in (1, 2, 4)]
         return e.value, b.value, v.value, {k: ms[i] for i, k in enumerate(keys)}
 
     def set_batch_lanes(self, lanes: int) -> None:

@@ -135,10 +135,14 @@ int gkrx_prove_batch(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, u
  * whole proof.  LOCKSTEP (the default): the batch advances one op of every proof per
  * step; a step is one launch of the wide ops (each proof's op on as many workgroups of
  * `block` threads as its tables need), one of the round pairs, one of the serial chains
- * and the launches of the transcript ends; batches of more than 2048 proofs run on the
- * PER_PROOF engine.  block is 64, 128, 256 or 512.  Results do not depend on the choice. */
+ * and the launches of the transcript ends, the three kinds on three streams.
+ * LOCKSTEP_GROUPS: the same steps for three groups of proofs, each group on a stream of
+ * its own, so that one group's transcript ends and chains overlap the other groups'
+ * passes.  Batches of more than 2048 proofs run on the PER_PROOF engine.  block is 64,
+ * 128, 256 or 512.  Results do not depend on the choice. */
 #define GKRX_BATCH_PER_PROOF 1
 #define GKRX_BATCH_LOCKSTEP 2
+#define GKRX_BATCH_LOCKSTEP_GROUPS 4
 int gkrx_ctx_set_batch_engine(gkrx_ctx *ctx, int engine, uint32_t block);
 int gkrx_ctx_get_batch_engine(const gkrx_ctx *ctx, int *engine, uint32_t *block);
 /* HIP graphs for gkrx_prove (0 off, the default; 1 on): the device part of a prove (the
@@ -154,8 +158,9 @@ int gkrx_ctx_set_batch_lanes(gkrx_ctx *ctx, uint32_t lanes);
 int gkrx_ctx_get_batch_lanes(const gkrx_ctx *ctx, uint32_t *lanes);
 /* Autotuner: times the device part of a batch of these circuits (witnesses set) under
  * every engine, block size and lanes per thread, keeps the fastest in the context, and
- * reports it (lanes is 4 for PER_PROOF).  ms_each (optional, 13 entries): PER_PROOF, then
- * LOCKSTEP with block 64, 128, 256, 512, each with lanes 1, 2, 4 (block-major). */
+ * reports it (lanes is 4 for PER_PROOF).  ms_each (optional, 25 entries): PER_PROOF, then
+ * LOCKSTEP with block 64, 128, 256, 512, each with lanes 1, 2, 4 (block-major), then
+ * LOCKSTEP_GROUPS in the same order. */
 int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, int *engine, uint32_t *block,
                         uint32_t *lanes, double *ms_each);
 /* Verifier::verify (LinearGKR.hpp:104-149), on the host.  Returns 1 = accepted,

@@ -96,7 +96,8 @@ def test_result_independent_of_batch_mix(ctx):
         c.close()
 
 
-ENGINES = [(gkrx.BATCH_PER_PROOF, 256)] + [(gkrx.BATCH_LOCKSTEP, b) for b in (64, 128, 256, 512)]
+ENGINES = [(gkrx.BATCH_PER_PROOF, 256)] + [(e, b) for e in (gkrx.BATCH_LOCKSTEP, gkrx.BATCH_LOCKSTEP_GROUPS)
+                                           for b in (64, 128, 256, 512)]
 
 
 @pytest.fixture(scope="module")
@@ -135,7 +136,7 @@ def test_autotune_picks_an_engine_and_keeps_results(ctx, mixed_plain):
     try:
         engine, block, lanes, ms = ctx.batch_autotune(circs)
         assert (engine, block) in ENGINES and ctx.batch_engine() == (engine, block) and ctx.batch_lanes() == lanes
-        assert len(ms) == 13 and all(v > 0 for v in ms.values()) and ms[(engine, block, lanes)] == min(ms.values())
+        assert len(ms) == 25 and all(v > 0 for v in ms.values()) and ms[(engine, block, lanes)] == min(ms.values())
         batch = gkrx.Prover(ctx).prove_batch(circs)
     finally:
         ctx.set_batch_engine(*saved)
@@ -144,12 +145,15 @@ def test_autotune_picks_an_engine_and_keeps_results(ctx, mixed_plain):
         assert np.array_equal(claimed, c1) and bytes(proof) == bytes(p1)
 
 
-@pytest.mark.parametrize("block,lanes", [(64, 1), (128, 2), (256, 1), (512, 2)])
-def test_lanes_per_thread_equal_plain(ctx, mixed_plain, block, lanes):
-    print(f"[gpu] lockstep block {block} lanes {lanes} vs plain proofs", flush=True)
+@pytest.mark.parametrize("engine,block,lanes", [(gkrx.BATCH_LOCKSTEP, 64, 1), (gkrx.BATCH_LOCKSTEP, 128, 2),
+                                               (gkrx.BATCH_LOCKSTEP, 256, 1), (gkrx.BATCH_LOCKSTEP, 512, 2),
+                                               (gkrx.BATCH_LOCKSTEP_GROUPS, 64, 4), (gkrx.BATCH_LOCKSTEP_GROUPS, 256, 1),
+                                               (gkrx.BATCH_LOCKSTEP_GROUPS, 512, 2)])
+def test_lanes_per_thread_equal_plain(ctx, mixed_plain, engine, block, lanes):
+    print(f"[gpu] lockstep engine {engine} block {block} lanes {lanes} vs plain proofs", flush=True)
     circs, plain = mixed_plain
     saved = ctx.batch_engine()
-    ctx.set_batch_engine(gkrx.BATCH_LOCKSTEP, block)
+    ctx.set_batch_engine(engine, block)
     ctx.set_batch_lanes(lanes)
     try:
         batch = gkrx.Prover(ctx).prove_batch(circs)
