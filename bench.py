@@ -41,14 +41,19 @@ TAIL_ROWS = 2048  # tables with at most this many rows end a phase in the small-
 # Keys: the workload and the batch engine setting the profiled run used.
 # k_pair<16, phase 1, double-fold> of the single 24 x 2^20 proof
 # (profiles/pmc_traffic_m31x16_24x2e20_keccak256.md):
-SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 140514307}
+SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 138502048}
 # k_ls_pair of the 512-proof batch (profiles/pmc_ls_pair_m31x16_512_uneven_keccak256.md):
-# {(proofs, engine, block, lanes per thread): bytes per dispatch}
+# {(proofs, engine, block, lanes per thread): bytes per dispatch}; engine 4 (proof groups on
+# their own streams) launches about three times as many, smaller, round-pair kernels
 BATCH_PAIR_TRAFFIC = {
-    (512, 2, 64, 1): 1690935665, (512, 2, 64, 2): 1554011197, (512, 2, 64, 4): 1474593235,
-    (512, 2, 128, 1): 1514966340, (512, 2, 128, 2): 1406289529, (512, 2, 128, 4): 1336712176,
-    (512, 2, 256, 1): 1386175865, (512, 2, 256, 2): 1310957996, (512, 2, 256, 4): 1267886940,
-    (512, 2, 512, 1): 1311674523, (512, 2, 512, 2): 1258961629, (512, 2, 512, 4): 1227457480,
+    (512, 2, 64, 1): 1252555197, (512, 2, 64, 2): 1216195945, (512, 2, 64, 4): 1214197048,
+    (512, 2, 128, 1): 1215793649, (512, 2, 128, 2): 1189116668, (512, 2, 128, 4): 1180100159,
+    (512, 2, 256, 1): 1184688706, (512, 2, 256, 2): 1170093911, (512, 2, 256, 4): 1167838135,
+    (512, 2, 512, 1): 1171091412, (512, 2, 512, 2): 1163174580, (512, 2, 512, 4): 1164073971,
+    (512, 4, 64, 1): 422135469, (512, 4, 64, 2): 411990796, (512, 4, 64, 4): 411182914,
+    (512, 4, 128, 1): 411578860, (512, 4, 128, 2): 404555489, (512, 4, 128, 4): 402206055,
+    (512, 4, 256, 1): 402501658, (512, 4, 256, 2): 398169350, (512, 4, 256, 4): 397925391,
+    (512, 4, 512, 1): 398258982, (512, 4, 512, 2): 395817737, (512, 4, 512, 4): 397104514,
 }
 
 

@@ -5,9 +5,9 @@
 // field_size 254 at :54); its field header is absent, so the representation here is the
 // build's own: elements are 8 little-endian 32-bit limbs; on the device they are kept in
 // Montgomery form (R = 2^256) and multiplied with CIOS, each 32 x 32 -> 64-bit step a
-// v_mad_u64_u32.  Serialization (transcript, proof, claims) is the canonical value as 32
-// little-endian bytes; a challenge is the 32 digest bytes read as a little-endian integer
-// reduced mod r.
+// v_mad_u64_u32 and the carries one v_addc_co_u32 chain per row.  Serialization
+// (transcript, proof, claims) is the canonical value as 32 little-endian bytes; a challenge
+// is the 32 digest bytes read as a little-endian integer reduced mod r.
 #pragma once
 
 #include <cstdint>
@@ -48,28 +48,37 @@ GKRX_HD bool fr_geq(const uint32_t a[8], const uint32_t b[8])
     return true;
 }
 
+// One word of a carry chain: a + b + carry (carry 0 or 1 in and out), and a - b - borrow.
+// On gfx950 these are single v_addc_co_u32 / v_subb_co_u32 instructions.
+GKRX_HD uint32_t adc32(uint32_t a, uint32_t b, uint32_t &carry)
+{
+    uint32_t out;
+    const uint32_t r = __builtin_addc(a, b, carry, &out);
+    carry = out;
+    return r;
+}
+GKRX_HD uint32_t sbb32(uint32_t a, uint32_t b, uint32_t &borrow)
+{
+    uint32_t out;
+    const uint32_t r = __builtin_subc(a, b, borrow, &out);
+    borrow = out;
+    return r;
+}
+
 // a - b, returns the borrow
 GKRX_HD uint32_t sub256(uint32_t out[8], const uint32_t a[8], const uint32_t b[8])
 {
-    uint64_t borrow = 0;
-    for (int i = 0; i < 8; i++) {
-        const uint64_t d = (uint64_t)a[i] - b[i] - borrow;
-        out[i] = (uint32_t)d;
-        borrow = (d >> 63) & 1u;
-    }
-    return (uint32_t)borrow;
+    uint32_t borrow = 0;
+    for (int i = 0; i < 8; i++) out[i] = sbb32(a[i], b[i], borrow);
+    return borrow;
 }
 
 // a + b, returns the carry
 GKRX_HD uint32_t add256(uint32_t out[8], const uint32_t a[8], const uint32_t b[8])
 {
-    uint64_t carry = 0;
-    for (int i = 0; i < 8; i++) {
-        const uint64_t s = (uint64_t)a[i] + b[i] + carry;
-        out[i] = (uint32_t)s;
-        carry = s >> 32;
-    }
-    return (uint32_t)carry;
+    uint32_t carry = 0;
+    for (int i = 0; i < 8; i++) out[i] = adc32(a[i], b[i], carry);
+    return carry;
 }
 
 // (a + b) mod r for a, b < r (r < 2^254, so a + b has no carry out of 256 bits)
@@ -79,53 +88,55 @@ GKRX_HD Fr fr_add(const Fr &a, const Fr &b)
     Fr s, t;
     add256(s.w, a.w, b.w);
     const uint32_t borrow = sub256(t.w, s.w, R);
-    return borrow ? s : t;
+    for (int i = 0; i < 8; i++) t.w[i] = borrow ? s.w[i] : t.w[i];
+    return t;
 }
 
 // (a - b) mod r for a, b < r
 GKRX_HD Fr fr_sub(const Fr &a, const Fr &b)
 {
     const uint32_t R[8] = GKRX_BN_R;
-    Fr d;
-    if (sub256(d.w, a.w, b.w)) add256(d.w, d.w, R);
-    return d;
+    Fr d, e;
+    const uint32_t mask = 0u - sub256(d.w, a.w, b.w);  // all ones on borrow: add r back
+    uint32_t carry = 0;
+    for (int i = 0; i < 8; i++) e.w[i] = adc32(d.w[i], R[i] & mask, carry);
+    return e;
 }
 
 // Montgomery product a * b * 2^-256 mod r (CIOS, 32-bit words); a, b < r give a result < r.
+// Each row of a word product adds the eight products a_j b_i + t_j (each below 2^64, and
+// independent of one another) and then joins their halves with one carry chain, so that
+// the 64-bit multiply-adds of a row do not wait on each other's carries.
 GKRX_HD Fr fr_mont_mul(const Fr &a, const Fr &b)
 {
     const uint32_t R[8] = GKRX_BN_R;
     uint32_t t[10];
     for (int i = 0; i < 10; i++) t[i] = 0;
     for (int i = 0; i < 8; i++) {
-        uint64_t c = 0;
-        for (int j = 0; j < 8; j++) {
-            const uint64_t p = (uint64_t)a.w[j] * b.w[i] + t[j] + c;
-            t[j] = (uint32_t)p;
-            c = p >> 32;
-        }
-        uint64_t s = (uint64_t)t[8] + c;
-        t[8] = (uint32_t)s;
-        t[9] = (uint32_t)(s >> 32);
+        // t += a * b_i
+        uint64_t p[8];
+        for (int j = 0; j < 8; j++) p[j] = (uint64_t)a.w[j] * b.w[i] + t[j];
+        uint32_t c = 0;
+        t[0] = (uint32_t)p[0];
+        for (int j = 1; j < 8; j++) t[j] = adc32((uint32_t)p[j], (uint32_t)(p[j - 1] >> 32), c);
+        t[8] = adc32(t[8], (uint32_t)(p[7] >> 32), c);
+        t[9] += c;
+        // t = (t + m r) / 2^32 with m = -t_0 / r mod 2^32 (the low word of t + m r is zero)
         const uint32_t m = t[0] * kBnN0Inv;
-        uint64_t p = (uint64_t)m * R[0] + t[0];
-        c = p >> 32;
-        for (int j = 1; j < 8; j++) {
-            p = (uint64_t)m * R[j] + t[j] + c;
-            t[j - 1] = (uint32_t)p;
-            c = p >> 32;
-        }
-        s = (uint64_t)t[8] + c;
-        t[7] = (uint32_t)s;
-        t[8] = t[9] + (uint32_t)(s >> 32);
+        uint64_t q[8];
+        for (int j = 0; j < 8; j++) q[j] = (uint64_t)m * R[j] + t[j];
+        c = 0;
+        for (int j = 1; j < 8; j++) t[j - 1] = adc32((uint32_t)q[j], (uint32_t)(q[j - 1] >> 32), c);
+        t[7] = adc32(t[8], (uint32_t)(q[7] >> 32), c);
+        t[8] = t[9] + c;
+        t[9] = 0;
     }
     Fr out, d;
     for (int i = 0; i < 8; i++) out.w[i] = t[i];
-    if (t[8] != 0 || fr_geq(out.w, R)) {
-        sub256(d.w, out.w, R);
-        return d;
-    }
-    return out;
+    const uint32_t borrow = sub256(d.w, out.w, R);
+    const bool keep = t[8] == 0 && borrow;  // t < r: no subtraction
+    for (int i = 0; i < 8; i++) d.w[i] = keep ? out.w[i] : d.w[i];
+    return d;
 }
 
 GKRX_HD Fr fr_to_mont(const Fr &a)

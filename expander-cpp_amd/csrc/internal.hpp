@@ -187,6 +187,7 @@ struct BnScratch {
     uint32_t *fA = nullptr;
     uint32_t *fB = nullptr;
     uint32_t *partials = nullptr; // [block][3] Fr
+    uint32_t *eq_parts = nullptr; // the halves of two eq tables (2 x (2^ceil(m/2) + 2^floor(m/2)) Fr)
     uint32_t *chal = nullptr;     // (D + 1) slots of 64 Fr: rx then ry
     uint32_t *ab = nullptr;       // (D + 1) x (alpha, beta)
     uint32_t *claims = nullptr;   // canonical claimed_v

@@ -35,7 +35,7 @@ namespace {
 constexpr int kBnThreads = 256;
 constexpr uint32_t kBnMaxBlocks = 1024;
 constexpr uint32_t kBnTailThreads = 512;
-constexpr uint32_t kBnTailRows = 512;  // rounds with at most this many rows run in k_bn_tail
+constexpr uint32_t kBnTailRows = 1024;  // rounds with at most this many rows run in k_bn_tail
 constexpr uint32_t kBnChalStride = 64;  // Fr per layer slot: rx in [0, 32), ry in [32, 64)
 
 __device__ __forceinline__ Fr ld_fr(const uint32_t *p)
@@ -98,100 +98,114 @@ __global__ void __launch_bounds__(kBnThreads) k_bn_forward(const uint32_t *__res
 }
 
 // ------------------------------------------------------------------ eq tables
-// out[z] = f1 eq(r1, z) + f2 eq(r2, z), z < 2^m (f1 = ab[0], f2 = ab[1]; without ab,
-// f1 = 1 and no second term).  Block: 256 threads cover kBnEqChunk * 256 consecutive z;
-// the low 8 bits come from an LDS table of 256 partial products, the high bits from
-// kBnEqChunk per-block values: one multiply per table per z.
-constexpr uint32_t kBnEqChunk = 4;
-__global__ void __launch_bounds__(kBnThreads) k_bn_eq(uint32_t *__restrict__ out, const uint32_t *__restrict__ r1,
-                                                      const uint32_t *__restrict__ r2, const uint32_t *__restrict__ ab,
-                                                      uint32_t m)
+// out[z] = f1 eq(r1, z) + f2 eq(r2, z), z < 2^m (f1 = ab[0], f2 = ab[1]; without ab, f1 = 1
+// and no second term), by halves: eq(r, z) = lo[z mod 2^h] hi[z >> h] with h = ceil(m / 2),
+// lo[i] = prod_{b < h} (bit_b(i) ? r_b : 1 - r_b) and hi[j] = f prod_{b < m - h} (bit_b(j) ?
+// r_{h+b} : 1 - r_{h+b}).  k_bn_eq_parts forms the small tables (lo1, hi1, then lo2, hi2), one
+// entry per thread with a chain of at most ceil(m / 2) products; k_bn_eq takes one product per
+// term per z.
+__host__ __device__ inline uint32_t bn_eq_lo_bits(uint32_t m) { return (m + 1) / 2; }
+__host__ __device__ inline uint64_t bn_eq_parts_fr(uint32_t m)  // Fr of one term's two tables
 {
-    __shared__ Fr lo1[256], lo2[256], hi1[kBnEqChunk], hi2[kBnEqChunk];
-    const bool two = ab != nullptr;
-    const uint32_t h = m < 8 ? m : 8, hb = m - h;
-    const uint32_t t = threadIdx.x;
-    const uint64_t base = (uint64_t)blockIdx.x * (kBnEqChunk * 256);
-    if (t < (1u << h)) {
-        Fr a = two ? ld_fr(ab) : fr_one(), b = two ? ld_fr(ab + 8) : fr_zero();
-        for (uint32_t i = 0; i < h; i++) {
-            const bool bit = (t >> i) & 1u;
-            const Fr x1 = ld_fr(r1 + 8 * i);
-            a = fr_mont_mul(a, bit ? x1 : fr_sub(fr_one(), x1));
-            if (two) {
-                const Fr x2 = ld_fr(r2 + 8 * i);
-                b = fr_mont_mul(b, bit ? x2 : fr_sub(fr_one(), x2));
-            }
-        }
-        lo1[t] = a;
-        lo2[t] = b;
+    return (1ull << bn_eq_lo_bits(m)) + (1ull << (m - bn_eq_lo_bits(m)));
+}
+
+__global__ void __launch_bounds__(kBnThreads) k_bn_eq_parts(uint32_t *__restrict__ parts, const uint32_t *__restrict__ r1,
+                                                            const uint32_t *__restrict__ r2,
+                                                            const uint32_t *__restrict__ ab, uint32_t m)
+{
+    const uint32_t h = bn_eq_lo_bits(m);
+    const uint64_t per = bn_eq_parts_fr(m), nlo = 1ull << h;
+    const uint64_t i = (uint64_t)blockIdx.x * kBnThreads + threadIdx.x;
+    if (i >= per * (ab ? 2u : 1u)) return;
+    const uint32_t term = (uint32_t)(i / per);
+    const uint64_t j = i % per;
+    const uint32_t *r = term ? r2 : r1;
+    const bool lo = j < nlo;
+    const uint64_t idx = lo ? j : j - nlo;
+    const uint32_t b0 = lo ? 0u : h, nb = lo ? h : m - h;
+    Fr a = (!lo && ab) ? ld_fr(ab + 8 * term) : fr_one();
+    for (uint32_t b = 0; b < nb; b++) {
+        const Fr x = ld_fr(r + 8 * (b0 + b));
+        a = fr_mont_mul(a, (idx >> b) & 1u ? x : fr_sub(fr_one(), x));
     }
-    if (t < kBnEqChunk) {
-        const uint64_t hi = (base >> h) + t;
-        Fr a = fr_one(), b = fr_one();
-        for (uint32_t i = 0; i < hb; i++) {
-            const bool bit = (hi >> i) & 1u;
-            const Fr x1 = ld_fr(r1 + 8 * (h + i));
-            a = fr_mont_mul(a, bit ? x1 : fr_sub(fr_one(), x1));
-            if (two) {
-                const Fr x2 = ld_fr(r2 + 8 * (h + i));
-                b = fr_mont_mul(b, bit ? x2 : fr_sub(fr_one(), x2));
-            }
-        }
-        hi1[t] = a;
-        hi2[t] = b;
-    }
-    __syncthreads();
-    const uint64_t M = 1ull << m;
-    const uint32_t mask = (1u << h) - 1u;
-    for (uint32_t c = 0; c < kBnEqChunk; c++) {
-        const uint64_t z = base + (uint64_t)c * 256 + t;
-        if (z < M) {
-            const uint32_t li = (uint32_t)z & mask, hidx = (uint32_t)((z >> h) - (base >> h));
-            Fr v = fr_mont_mul(lo1[li], hi1[hidx]);
-            if (two) v = fr_add(v, fr_mont_mul(lo2[li], hi2[hidx]));
-            st_fr(out + 8 * z, v);
-        }
+    st_fr(parts + 8 * i, a);
+}
+
+__global__ void __launch_bounds__(kBnThreads) k_bn_eq(uint32_t *__restrict__ out, const uint32_t *__restrict__ parts,
+                                                      uint32_t m, uint32_t two)
+{
+    const uint32_t h = bn_eq_lo_bits(m);
+    const uint64_t per = bn_eq_parts_fr(m), nlo = 1ull << h, M = 1ull << m, mask = nlo - 1;
+    for (uint64_t z = (uint64_t)blockIdx.x * kBnThreads + threadIdx.x; z < M; z += (uint64_t)gridDim.x * kBnThreads) {
+        const uint64_t li = z & mask, hi = nlo + (z >> h);
+        Fr v = fr_mont_mul(ld_fr(parts + 8 * li), ld_fr(parts + 8 * hi));
+        if (two) v = fr_add(v, fr_mont_mul(ld_fr(parts + 8 * (per + li)), ld_fr(parts + 8 * (per + hi))));
+        st_fr(out + 8 * z, v);
     }
 }
 
 // ------------------------------------------------------------------ g_x and h_y
-// hg[x] = sum over the gates with first input x of c eqZ[z] v[y] (mul) or c eqZ[z] (add)
+// out[x] = the sum of term(e) over the entries e of CSR row x, for the kBnThreads rows from
+// x0 on.  The rows' entries are contiguous in the CSR; they are spread over the block's
+// threads kBnThreads at a time, one term per thread (its gathers and products do not wait
+// on the other entries of its row), staged in LDS, and thread t adds the staged terms of row
+// x0 + t.  Every thread of the block calls it.
+template <typename Term>
+__device__ __forceinline__ void bn_row_sums(uint32_t *__restrict__ out, const uint32_t *__restrict__ ptr,
+                                            const uint32_t *__restrict__ ent, uint64_t x0, uint32_t N, Term term)
+{
+    __shared__ Fr staged[kBnThreads];
+    __shared__ uint32_t rp[kBnThreads + 1];
+    const uint32_t t = threadIdx.x;
+    const uint32_t rows = (uint32_t)(N - x0 < kBnThreads ? N - x0 : kBnThreads);
+    if (t <= rows) rp[t] = ptr[x0 + t];
+    if (t == 0) rp[rows] = ptr[x0 + rows];
+    __syncthreads();
+    const uint32_t e0 = rp[0], e1 = rp[rows];
+    const uint32_t r0 = t < rows ? rp[t] : 0u, r1 = t < rows ? rp[t + 1] : 0u;
+    Fr acc = fr_zero();
+    for (uint32_t c = e0; c < e1; c += kBnThreads) {
+        const uint32_t e = c + t;
+        if (e < e1) staged[t] = term(ent + 4ull * e);
+        __syncthreads();
+        const uint32_t lo = r0 > c ? r0 : c, hi = r1 < c + kBnThreads ? r1 : c + kBnThreads;
+        for (uint32_t j = lo; j < hi; j++) acc = fr_add(acc, staged[j - c]);
+        __syncthreads();
+    }
+    if (t < rows) st_fr(out + 8 * (x0 + t), acc);
+    __syncthreads();
+}
+
+// hg[x] = sum over the gates with first input x of c eqZ[z] v[y] (mul) or c eqZ[z] (add);
+// CSR entry {y or kAddMark, z, c, x}
 __global__ void __launch_bounds__(kBnThreads) k_bn_gx(uint32_t *__restrict__ hg, const uint32_t *__restrict__ v,
                                                       const uint32_t *__restrict__ eqz, const uint32_t *__restrict__ ptr,
                                                       const uint32_t *__restrict__ ent, uint32_t N)
 {
-    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    for (uint64_t x = tile * kBnThreads + threadIdx.x; x < N; x += (uint64_t)gridDim.x * kBnThreads) {
-        Fr acc = fr_zero();
-        const uint32_t e1 = ptr[x + 1];
-        for (uint32_t e = ptr[x]; e < e1; e++) {
-            const uint32_t b = ent[kGxWords * (uint64_t)e], z = ent[kGxWords * (uint64_t)e + 1],
-                           c = ent[kGxWords * (uint64_t)e + 2];
+    static_assert(kGxWords == 4, "g_x entry");
+    const uint32_t ntiles = (N + kBnThreads - 1) / kBnThreads;
+    for (uint32_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < ntiles; tile += gridDim.x)
+        bn_row_sums(hg, ptr, ent, (uint64_t)tile * kBnThreads, N, [&](const uint32_t *en) {
+            const uint32_t b = en[0], z = en[1], c = en[2];
             Fr term = ld_fr(eqz + 8 * (uint64_t)z);
             if (b != kAddMark) term = fr_mont_mul(term, ld_fr(v + 8 * (uint64_t)b));
-            acc = fr_add(acc, coef_mul(term, c));
-        }
-        st_fr(hg + 8 * x, acc);
-    }
+            return coef_mul(term, c);
+        });
 }
 
-// s[y] = sum over the mul gates with second input y of c eqZ[z] eqX[x]
+// s[y] = sum over the mul gates with second input y of c eqZ[z] eqX[x]; CSR entry {x, z, c, y}
 __global__ void __launch_bounds__(kBnThreads) k_bn_hy(uint32_t *__restrict__ s, const uint32_t *__restrict__ eqz,
                                                       const uint32_t *__restrict__ eqx, const uint32_t *__restrict__ ptr,
                                                       const uint32_t *__restrict__ ent, uint32_t N)
 {
-    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    for (uint64_t y = tile * kBnThreads + threadIdx.x; y < N; y += (uint64_t)gridDim.x * kBnThreads) {
-        Fr acc = fr_zero();
-        const uint32_t e1 = ptr[y + 1];
-        for (uint32_t e = ptr[y]; e < e1; e++) {
-            const uint32_t x = ent[kHyWords * (uint64_t)e], z = ent[kHyWords * (uint64_t)e + 1],
-                           c = ent[kHyWords * (uint64_t)e + 2];
-            acc = fr_add(acc, coef_mul(fr_mont_mul(ld_fr(eqz + 8 * (uint64_t)z), ld_fr(eqx + 8 * (uint64_t)x)), c));
-        }
-        st_fr(s + 8 * y, acc);
-    }
+    static_assert(kHyWords == 4, "h_y entry");
+    const uint32_t ntiles = (N + kBnThreads - 1) / kBnThreads;
+    for (uint32_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < ntiles; tile += gridDim.x)
+        bn_row_sums(s, ptr, ent, (uint64_t)tile * kBnThreads, N, [&](const uint32_t *en) {
+            const uint32_t x = en[0], z = en[1], c = en[2];
+            return coef_mul(fr_mont_mul(ld_fr(eqz + 8 * (uint64_t)z), ld_fr(eqx + 8 * (uint64_t)x)), c);
+        });
 }
 
 // ------------------------------------------------------------------ rounds
@@ -286,7 +300,7 @@ __device__ __forceinline__ Fr bn_challenge(const uint32_t d[8]) { return fr_to_m
 // bytes, one challenge, then the x claim (phase 1) or the y claim, alpha and beta (phase
 // 2) after the last round (sumcheck.hpp:90-103, gkr.hpp:65-66).  Hash inputs are staged
 // in LDS; at most one element (the pending x claim) precedes a message.
-__device__ void bn_transcript_step(const uint32_t *sums, const BnTsArgs &a, Fr *r_out)
+__device__ __forceinline__ void bn_transcript_step(const uint32_t *sums, const BnTsArgs &a, Fr *r_out)
 {
     __shared__ uint32_t ts[10], hb[8 + 24], dig[8];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
@@ -465,6 +479,18 @@ uint32_t bn_grid(uint64_t work, uint64_t cap)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + kBnThreads - 1) / kBnThreads, cap));
 }
 
+// eq over m variables into out (two terms when ab is given; see k_bn_eq_parts)
+void bn_launch_eq(uint32_t *out, uint32_t *parts, const uint32_t *r1, const uint32_t *r2, const uint32_t *ab, uint32_t m,
+                  hipStream_t st)
+{
+    const uint64_t nparts = bn_eq_parts_fr(m) * (ab ? 2u : 1u);
+    hipLaunchKernelGGL(k_bn_eq_parts, dim3((uint32_t)((nparts + kBnThreads - 1) / kBnThreads)), dim3(kBnThreads), 0, st,
+                       parts, r1, r2, ab, m);
+    hipLaunchKernelGGL(k_bn_eq, dim3(bn_grid(1ull << m, 65536)), dim3(kBnThreads), 0, st, out,
+                       (const uint32_t *)parts, m, ab ? 1u : 0u);
+}
+
+
 }  // namespace
 
 // ------------------------------------------------------------------ host side
@@ -500,8 +526,9 @@ uint64_t dev_bn_scratch_bytes(const DevCircuit &dc, uint64_t tail_bytes)
 {
     const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
     const uint64_t N2 = std::max<uint64_t>(2, maxN / 2), N4 = std::max<uint64_t>(2, maxN / 4);
-    return 4 * (8 * maxM + 16 * maxN + 16 * N2 + 16 * N4 + 24ull * kBnMaxBlocks + 8ull * (dc.D + 1) * kBnChalStride +
-                16ull * (dc.D + 1) + 40 + tail_bytes / 4 + 64 + 14 * 64);
+    return 4 * (8 * maxM + 16 * maxN + 16 * N2 + 16 * N4 + 24ull * kBnMaxBlocks +
+                16 * bn_eq_parts_fr(std::max(dc.max_n, dc.max_m)) + 8ull * (dc.D + 1) * kBnChalStride +
+                16ull * (dc.D + 1) + 40 + tail_bytes / 4 + 64 + 15 * 64);
 }
 
 int dev_bn_ensure_scratch(BnScratch &s, const DevCircuit &dc, uint64_t tail_bytes)
@@ -515,6 +542,7 @@ int dev_bn_ensure_scratch(BnScratch &s, const DevCircuit &dc, uint64_t tail_byte
     Piece pieces[] = {
         {&s.eqz, 8 * maxM}, {&s.eqx, 8 * maxN}, {&s.h0, 8 * maxN}, {&s.hA, 8 * N2}, {&s.hB, 8 * N4},
         {&s.fA, 8 * N2}, {&s.fB, 8 * N4}, {&s.partials, 24ull * kBnMaxBlocks},
+        {&s.eq_parts, 16 * bn_eq_parts_fr(std::max(dc.max_n, dc.max_m))},
         {&s.chal, 8ull * (dc.D + 1) * kBnChalStride}, {&s.ab, 16ull * (dc.D + 1)}, {&s.claims, 8},
         {&s.vxy, 16}, {&s.tstate, 16}, {&s.tail, (tail_bytes + 3) / 4 + 64}};
     uint64_t total = 0;
@@ -571,9 +599,7 @@ int dev_bn_prove(const DevCircuit &dc, BnScratch &s, int hash_kind, const uint32
         uint32_t *chal_l = chal_slot(l);
         const uint32_t *chal_p = chal_slot(l + 1);
         timed(pf, kKEq, st, [&] {
-            hipLaunchKernelGGL(k_bn_eq, dim3((uint32_t)((M + kBnEqChunk * 256 - 1) / (kBnEqChunk * 256))),
-                               dim3(kBnThreads), 0, st, s.eqz, chal_p, chal_p + 8 * 32, (const uint32_t *)ab_slot(l + 1),
-                               m);
+            bn_launch_eq(s.eqz, s.eq_parts, chal_p, chal_p + 8 * 32, ab_slot(l + 1), m, st);
         });
         if (l == (int)D - 1) {
             const uint32_t nb = bn_grid(M, kBnMaxBlocks);
@@ -592,9 +618,7 @@ int dev_bn_prove(const DevCircuit &dc, BnScratch &s, int hash_kind, const uint32
             if (phase == 2) {
                 // h0 (the phase-1 table 0) is no longer read: the phase-2 table s replaces it
                 timed(pf, kKEq, st, [&] {
-                    hipLaunchKernelGGL(k_bn_eq, dim3((uint32_t)((N + kBnEqChunk * 256 - 1) / (kBnEqChunk * 256))),
-                                       dim3(kBnThreads), 0, st, s.eqx, (const uint32_t *)chal_l,
-                                       (const uint32_t *)nullptr, (const uint32_t *)nullptr, n);
+                    bn_launch_eq(s.eqx, s.eq_parts, chal_l, nullptr, nullptr, n, st);
                 });
                 timed(pf, kKHy, st, [&] {
                     hipLaunchKernelGGL(k_bn_hy, dim3(bn_grid(N, 65536)), dim3(kBnThreads), 0, st, s.h0,

@@ -864,6 +864,55 @@ __device__ __forceinline__ void reduce_partials(const uint32_t *__restrict__ par
     __syncthreads();
 }
 
+// Round k's three sums per lane from the nine product sums vals[q L + lane] of a round
+// pair: u = P0 + P3, v = P1 + P4, and the sum at 2 = 2u + 2v - (P2 + P5).
+template <int L>
+__device__ __forceinline__ void pair_sums_first(const uint32_t *vals, uint32_t *sums)
+{
+    for (uint32_t c = threadIdx.x; c < L; c += blockDim.x) {
+        const uint32_t *P = vals + c;  // P[q L] = product q of lane c
+        const uint32_t u = m31_add(P[0], P[3 * L]), v = m31_add(P[L], P[4 * L]), w = m31_add(P[2 * L], P[5 * L]);
+        sums[c] = u;                                                       // sum f0 h0
+        sums[L + c] = v;                                                   // sum f1 h1
+        sums[2 * L + c] = m31_sub(m31_add(m31_add(u, u), m31_add(v, v)), w);  // sum (f0 + f1)(h0 + h1)
+    }
+}
+
+// Round k+1's sums of a round pair: each a quadratic in r = r_k over three products.
+template <int L>
+__device__ __forceinline__ void pair_sums_second(const uint32_t *vals, uint32_t r, uint32_t *sums)
+{
+    constexpr uint32_t Q = 3 * L;
+    for (uint32_t q = threadIdx.x; q < Q; q += blockDim.x) {
+        const uint32_t s = q / L, c = q % L;
+        const uint32_t *P = vals + 3 * s * L + c;  // the three products of s_s
+        const uint32_t c1 = m31_sub(P[L], m31_add(P[0], P[2 * L]));
+        sums[q] = m31_add(P[0], m31_mul(r, m31_add(c1, m31_mul(r, P[2 * L]))));
+    }
+}
+
+// One transcript step of repetition rep, run by one wave: the round's message (from sums,
+// times vx in phase 2) goes to the proof tail at byte `base` and to hb (behind at most L
+// pending words), then the bytes from `start` (below `base` only for the first message of
+// rep 0 after other appends) to the end of the message are hashed.  Returns the challenge;
+// d receives the digest.
+template <int L>
+__device__ __forceinline__ uint32_t rep_step(const uint32_t *sums, const TsArgs &ts, int phase, uint32_t rep,
+                                             uint32_t base, uint32_t start, uint32_t *hb, uint32_t (&d)[8])
+{
+    constexpr uint32_t Q = 3 * L;
+    const uint32_t lane = threadIdx.x & 63, lead = (base - start) / 4;
+    for (uint32_t w = lane; w < Q; w += 64) {
+        const uint32_t v = msg_word<L>(sums, ts.vx + rep * L, phase, w);
+        ts.tail[base / 4 + w] = v;
+        hb[L + w] = v;
+    }
+    if (lane < lead && lead <= L) hb[L - lead + lane] = ts.tail[start / 4 + lane];
+    __threadfence_block();
+    hash_words_wave(ts.hash_kind, lead <= L ? hb + L - lead : ts.tail + start / 4, lead + Q, d);
+    return challenge_from(d);
+}
+
 // The end of a pair of grid rounds for repetition `rep` (one workgroup of kFinThreads per
 // repetition): adds rep's partials (the nine products P0..P8 of pair_item per lane), forms
 // round k's sums and draws r_k, evaluates round k+1's sums at r_k and draws r_{k+1}.  Before
@@ -880,46 +929,22 @@ __device__ __forceinline__ void pair_fin_rep_body(const uint32_t *__restrict__ p
     __shared__ uint32_t vals[V], sums[Q], hb[L + Q], rr;
     const uint32_t R = ts.R, t = threadIdx.x, wave = t >> 6, lane = t & 63;
     reduce_partials<L>(partials + (uint64_t)rep * V * nb, nb, V, vals);
-    for (uint32_t c = t; c < L; c += blockDim.x) {
-        const uint32_t *P = vals + c;  // P[q L] = product q of lane c
-        const uint32_t u = m31_add(P[0], P[3 * L]), v = m31_add(P[L], P[4 * L]), w = m31_add(P[2 * L], P[5 * L]);
-        sums[c] = u;                                                       // sum f0 h0
-        sums[L + c] = v;                                                   // sum f1 h1
-        sums[2 * L + c] = m31_sub(m31_add(m31_add(u, u), m31_add(v, v)), w);  // sum (f0 + f1)(h0 + h1)
-    }
+    pair_sums_first<L>(vals, sums);
     __syncthreads();
     uint32_t *chal = ts.chal + rep * kChalStride + (ts.phase == 2 ? 32u : 0u);
     uint32_t d[8] = {};
     for (uint32_t step = 0; step < 2; step++) {
         if (wave == 0) {
-            // message words to the proof tail and to hb (behind `lead` pending words), then the hash
             const uint32_t base = len0 + step * 4 * Q * R + 4 * Q * rep;
-            const uint32_t start = (rep == 0 && step == 0) ? hs0 : base;
-            const uint32_t lead = (base - start) / 4;
-            for (uint32_t w = lane; w < Q; w += 64) {
-                const uint32_t v = msg_word<L>(sums, ts.vx + rep * L, ts.phase, w);
-                ts.tail[base / 4 + w] = v;
-                hb[L + w] = v;
-            }
-            if (lane < lead && lead <= L) hb[L - lead + lane] = ts.tail[start / 4 + lane];
-            __threadfence_block();
-            hash_words_wave(ts.hash_kind, lead <= L ? hb + L - lead : ts.tail + start / 4, lead + Q, d);
+            const uint32_t c = rep_step<L>(sums, ts, ts.phase, rep, base, (rep == 0 && step == 0) ? hs0 : base, hb, d);
             if (lane == 0) {
-                const uint32_t c = challenge_from(d);
                 chal[ts.k + step] = c;
                 rr = c;
             }
         }
         __syncthreads();
         if (step == 0) {
-            // round k+1: the three sums of each lane as quadratics in r_k, at r_k
-            const uint32_t r = rr;
-            for (uint32_t q = t; q < Q; q += blockDim.x) {
-                const uint32_t s = q / L, c = q % L;
-                const uint32_t *P = vals + 3 * s * L + c;  // the three products of s_s
-                const uint32_t c1 = m31_sub(P[L], m31_add(P[0], P[2 * L]));
-                sums[q] = m31_add(P[0], m31_mul(r, m31_add(c1, m31_mul(r, P[2 * L]))));
-            }
+            pair_sums_second<L>(vals, rr, sums);
             __syncthreads();
         }
     }
@@ -1024,6 +1049,9 @@ struct TailRepParams {
 };
 
 // The body of k_tail_rep for repetition `rep`; blockDim is NT (a multiple of 64, >= 3L).
+// One pass per round: with tables this small a round pair's pass (double folds, nine
+// products, a 9L-value reduction) costs about two single passes, measured on the 24 x 2^20
+// proof (20.63 against 20.31 ms of device time), so the rounds here run one at a time.
 template <int L, bool PH2, uint32_t NT = kRepThreads>
 __device__ __forceinline__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
 {
@@ -1083,20 +1111,10 @@ __device__ __forceinline__ void tail_rep_body(const TailRepParams &q, uint32_t r
         }
         __syncthreads();
         if (wave == 0) {
-            // message words to the proof tail and to hb (behind `lead` pending words)
             const uint32_t base = q.len0 + (k - p.k0) * 4 * Q * R + 4 * Q * rep;
             const uint32_t start = (rep == 0 && k == p.k0) ? q.hs0 : base;
-            const uint32_t lead = (base - start) / 4;
-            for (uint32_t w = lane; w < Q; w += 64) {
-                const uint32_t v = msg_word<L>(sums, p.ts.vx + rep * L, PH2 ? 2 : 1, w);
-                p.ts.tail[base / 4 + w] = v;
-                hb[L + w] = v;
-            }
-            if (lane < lead && lead <= L) hb[L - lead + lane] = p.ts.tail[start / 4 + lane];
-            __threadfence_block();
-            hash_words_wave(p.ts.hash_kind, lead <= L ? hb + L - lead : p.ts.tail + start / 4, lead + Q, d);
+            const uint32_t c = rep_step<L>(sums, p.ts, PH2 ? 2 : 1, rep, base, start, hb, d);
             if (lane == 0) {
-                const uint32_t c = challenge_from(d);
                 chal[k] = c;
                 rr = c;
             }
