@@ -41,19 +41,27 @@ TAIL_ROWS = 2048  # tables with at most this many rows end a phase in the small-
 # Keys: the workload and the batch engine setting the profiled run used.
 # k_pair<16, phase 1, double-fold> of the single 24 x 2^20 proof
 # (profiles/pmc_traffic_m31x16_24x2e20_keccak256.md):
-SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 138502048}
+SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 138368531}
+# BN254 fused round k_bn_round<true> of the 12 x 2^18 proof: int32 VALU wave instructions per
+# dispatch (rocprofv3 --pmc SQ_INSTS_VALU_INT32, averaged over the proof's dispatches of both
+# phases; profiles/valu_bn254_12x2e18_keccak256.md).  The int32 VALU peak is derived from the
+# published vector FP32 rate: 157.3 TFLOP/s counts a packed FMA (2 lanes x 2 flops) per lane
+# per cycle, so unpacked 32-bit lane operations peak at 157.3 / 4 = 39.3 T/s (256 CUs x 64
+# lanes x 2.4 GHz).
+BN254_ROUND_INT32 = {(12, 18): 459232}
+INT32_VALU_PEAK_TOPS = 39.32
 # k_ls_pair of the 512-proof batch (profiles/pmc_ls_pair_m31x16_512_uneven_keccak256.md):
 # {(proofs, engine, block, lanes per thread): bytes per dispatch}; engine 4 (proof groups on
 # their own streams) launches about three times as many, smaller, round-pair kernels
 BATCH_PAIR_TRAFFIC = {
-    (512, 2, 64, 1): 1252555197, (512, 2, 64, 2): 1216195945, (512, 2, 64, 4): 1214197048,
-    (512, 2, 128, 1): 1215793649, (512, 2, 128, 2): 1189116668, (512, 2, 128, 4): 1180100159,
-    (512, 2, 256, 1): 1184688706, (512, 2, 256, 2): 1170093911, (512, 2, 256, 4): 1167838135,
-    (512, 2, 512, 1): 1171091412, (512, 2, 512, 2): 1163174580, (512, 2, 512, 4): 1164073971,
-    (512, 4, 64, 1): 422135469, (512, 4, 64, 2): 411990796, (512, 4, 64, 4): 411182914,
-    (512, 4, 128, 1): 411578860, (512, 4, 128, 2): 404555489, (512, 4, 128, 4): 402206055,
-    (512, 4, 256, 1): 402501658, (512, 4, 256, 2): 398169350, (512, 4, 256, 4): 397925391,
-    (512, 4, 512, 1): 398258982, (512, 4, 512, 2): 395817737, (512, 4, 512, 4): 397104514,
+    (512, 2, 64, 1): 1252691336, (512, 2, 64, 2): 1216063907, (512, 2, 64, 4): 1214141397,
+    (512, 2, 128, 1): 1215803902, (512, 2, 128, 2): 1189142499, (512, 2, 128, 4): 1180104679,
+    (512, 2, 256, 1): 1184597465, (512, 2, 256, 2): 1170121267, (512, 2, 256, 4): 1167875802,
+    (512, 2, 512, 1): 1171106264, (512, 2, 512, 2): 1163164573, (512, 2, 512, 4): 1164080179,
+    (512, 4, 64, 1): 422220552, (512, 4, 64, 2): 412015620, (512, 4, 64, 4): 411162970,
+    (512, 4, 128, 1): 411678381, (512, 4, 128, 2): 404511076, (512, 4, 128, 4): 402241307,
+    (512, 4, 256, 1): 402472958, (512, 4, 256, 2): 398202361, (512, 4, 256, 4): 397999671,
+    (512, 4, 512, 1): 398269810, (512, 4, 512, 2): 395821151, (512, 4, 512, 4): 397116025,
 }
 
 
@@ -362,6 +370,25 @@ def bn254_leg(gkrx, fs, depth, log_width, steps, warmup):
         claimed, proof = prover.prove(circ, out=buf)
         lat.append((time.perf_counter() - t0) * 1e3)
         parts.append(prover.last_timing())
+    # the fused rounds k_bn_round<true> of phase 1, timed by events around their launches in
+    # proofs of their own (the events lengthen a proof, so the steps above run without them)
+    dom = "k_round_ph1"
+    gkrx.lib().gkrx_ctx_profile_classes(ctx._h, 1 << gkrx.kernel_class_index(ctx, dom))
+    for _ in range(steps):
+        circ.evaluate()
+        prover.prove(circ, out=buf)
+    st = gkrx.kernel_stats(ctx).get(dom, {"count": 0, "ms": 0.0})
+    gkrx.lib().gkrx_ctx_profile_classes(ctx._h, 0)
+    roof = None
+    insts = BN254_ROUND_INT32.get((depth, log_width))
+    if st["count"] and insts:
+        avg_s = st["ms"] / st["count"] * 1e-3
+        achieved = insts * 64 / avg_s / 1e12
+        roof = {"bound": "valu (int32)", "achieved": round(achieved, 3), "peak": INT32_VALU_PEAK_TOPS,
+                "unit": "T int32 lane-ops/s", "frac": round(achieved / INT32_VALU_PEAK_TOPS, 4),
+                "kernel": "k_bn_round<true> (fused BN254 round: 4 folds and 3 products, 8x8-limb CIOS each)",
+                "int32_wave_insts_per_launch": insts, "avg_launch_us": round(avg_s * 1e6, 2),
+                "launches": st["count"]}
     # per-kernel breakdown from one more proof with every launch timed
     gkrx.lib().gkrx_ctx_profile(ctx._h, 1)
     circ.evaluate()
@@ -378,7 +405,7 @@ def bn254_leg(gkrx, fs, depth, log_width, steps, warmup):
             "proof_bytes": int(proof.size),
             "breakdown_ms": {k2: round(float(np.mean([p[k2] for p in parts])), 3) for k2 in parts[0]},
             "kernels_ms_per_proof": {k2: round(v["ms"], 3) for k2, v in sorted(stats.items())},
-            "verified": bool(ok)}
+            "roofline": roof, "verified": bool(ok)}
 
 
 # ----------------------------------------------------------------- CPU baseline

@@ -606,7 +606,7 @@ static int prove_bn254_locked(gkrx_ctx *ctx, gkrx_circuit *c, uint8_t *proof, ui
     GKRX_HIP_CHECK(hipMemcpyAsync(tfinal, ctx->bn.tstate, sizeof tfinal, hipMemcpyDeviceToHost, ctx->stream));
     GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     const auto t2 = std::chrono::steady_clock::now();
-    if (ctx->prof.on) ctx->prof.collect();
+    if (ctx->prof.on) ctx->prof.settle();
     if (tfinal[0] != tb) {
         set_error("device transcript length mismatch: " + std::to_string(tfinal[0]) + " != " + std::to_string(tb));
         return GKRX_ERR_DEVICE;
@@ -667,7 +667,7 @@ try {
     GKRX_HIP_CHECK(hipMemcpyAsync(tfinal, ctx->scratch.tstate, sizeof tfinal, hipMemcpyDeviceToHost, ctx->stream));
     GKRX_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     const auto t2 = std::chrono::steady_clock::now();
-    if (ctx->prof.on) ctx->prof.collect();
+    if (ctx->prof.on) ctx->prof.settle();
     if (tfinal[0] != tb) {
         set_error("device transcript length mismatch: " + std::to_string(tfinal[0]) + " != " + std::to_string(tb));
         return GKRX_ERR_DEVICE;
@@ -1004,7 +1004,7 @@ try {
         proof_lens[i] = gkrx_proof_size(ctx, circuits[i]);
         circuits[i]->evaluated = true;
     }
-    if (ctx->prof.on) ctx->prof.collect();
+    if (ctx->prof.on) ctx->prof.settle();
     const auto t2 = std::chrono::steady_clock::now();
     ctx->last.host_prefix_ms = prefix_ms;
     ctx->last.device_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
@@ -1296,6 +1296,8 @@ extern "C" int gkrx_ctx_profile_classes(gkrx_ctx *ctx, uint64_t mask)
 extern "C" int gkrx_ctx_kernel_stat(gkrx_ctx *ctx, int idx, const char **name, uint64_t *launches, double *ms)
 {
     if (!ctx || idx < 0 || idx >= kKClassCount) return GKRX_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->prof.collect();  // every prove that recorded events has synchronized its streams
     if (name) *name = kernel_class_name(idx);
     if (launches) *launches = ctx->prof.count[idx];
     if (ms) *ms = ctx->prof.ms[idx];

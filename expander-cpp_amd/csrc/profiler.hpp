@@ -107,7 +107,15 @@ struct Profiler {
         cls.push_back(c);
         return true;
     }
-    // after the stream has been synchronized
+    // After a prove (its streams synchronized): the timings are read only once many event
+    // pairs are pending, or when the statistics are asked for, so that a timed loop does
+    // not read hundreds of events between its proves.
+    static constexpr size_t kPendingEvents = 8192;
+    void settle()
+    {
+        if (used >= kPendingEvents) collect();
+    }
+    // after the streams that recorded the pending events have been synchronized
     void collect()
     {
         for (size_t i = 0; i < cls.size() && 2 * i + 1 < used; i++) {
