@@ -255,27 +255,45 @@ __device__ __forceinline__ void gx_tile(unsigned long long *__restrict__ acc, ui
     constexpr uint32_t LG = L / 4, ROWS = gx_rows<L, BS>();
     const uint32_t t = threadIdx.x;
     const uint32_t rows = N - x0 < ROWS ? N - x0 : ROWS;
-    for (uint32_t i = t; i < R * ROWS * L; i += BS) acc[i] = 0;
-    __syncthreads();
     const uint32_t e0 = ptr[x0], e1 = ptr[x0 + rows];
-    for (uint64_t w = t; w < (uint64_t)(e1 - e0) * LG; w += BS) {
+    const uint64_t work = (uint64_t)(e1 - e0) * LG;
+    // one work item: entry e's R contributions to lanes 4g .. 4g+3 of its row
+    struct Item {
+        uint4 en, vb;  // {in1 or kAddMark, z, coef, x}; v[in1] (lanes), or ones for an add gate
+        uint32_t c[3];  // coef * eqZ_j[z]
+    };
+    auto load = [&](uint64_t w) {
+        Item it;
         const uint64_t e = e0 + w / LG;
         const uint32_t g = (uint32_t)(w % LG);
-        const uint4 en = ld4(ent + kGxWords * e);  // {in1 or kAddMark, z, coef, x}
-        const uint4 vb = en.x != kAddMark ? ld4(v + (uint64_t)en.x * L + 4 * g) : make_uint4(1u, 1u, 1u, 1u);
-        unsigned long long *a = acc + (uint64_t)(en.w - x0) * L + 4 * g;
+        it.en = ld4(ent + kGxWords * e);
+        it.vb = it.en.x != kAddMark ? ld4(v + (uint64_t)it.en.x * L + 4 * g) : make_uint4(1u, 1u, 1u, 1u);
+#pragma unroll
+        for (uint32_t j = 0; j < 3; j++) it.c[j] = j < R ? m31_mul(it.en.z, eqz[j * eqz_rs + it.en.y]) : 0u;
+        return it;
+    };
+    auto add = [&](const Item &it, uint64_t w) {
+        const uint32_t g = (uint32_t)(w % LG);
+        unsigned long long *a = acc + (uint64_t)(it.en.w - x0) * L + 4 * g;
 #pragma unroll
         for (uint32_t j = 0; j < 3; j++)
             if (j < R) {
                 // products below 2^32, reduced once when the row is written
-                const uint32_t c = m31_mul(en.z, eqz[j * eqz_rs + en.y]);
                 unsigned long long *aj = a + j * ROWS * L;
-                atomicAdd(aj, (unsigned long long)m31_mul_wide(vb.x, c));
-                atomicAdd(aj + 1, (unsigned long long)m31_mul_wide(vb.y, c));
-                atomicAdd(aj + 2, (unsigned long long)m31_mul_wide(vb.z, c));
-                atomicAdd(aj + 3, (unsigned long long)m31_mul_wide(vb.w, c));
+                atomicAdd(aj, (unsigned long long)m31_mul_wide(it.vb.x, it.c[j]));
+                atomicAdd(aj + 1, (unsigned long long)m31_mul_wide(it.vb.y, it.c[j]));
+                atomicAdd(aj + 2, (unsigned long long)m31_mul_wide(it.vb.z, it.c[j]));
+                atomicAdd(aj + 3, (unsigned long long)m31_mul_wide(it.vb.w, it.c[j]));
             }
-    }
+    };
+    // the thread's first item is gathered before the accumulators are cleared: its chain of
+    // dependent loads (entry, then the v row and eqZ) overlaps the clear and the barrier
+    Item first{};
+    if (t < work) first = load(t);
+    for (uint32_t i = t; i < R * ROWS * L; i += BS) acc[i] = 0;
+    __syncthreads();
+    if (t < work) add(first, t);
+    for (uint64_t w = t + BS; w < work; w += BS) add(load(w), w);
     __syncthreads();
     for (uint32_t i = t; i < R * rows * LG; i += BS) {
         const uint32_t j = i / (rows * LG), rem = i % (rows * LG), r = rem / LG, g = rem % LG;
@@ -312,17 +330,30 @@ __device__ __forceinline__ void hy_tile(unsigned long long *__restrict__ acc, ui
     constexpr uint32_t kHyRows = tile_lanes(BS);
     const uint32_t t = threadIdx.x;
     const uint32_t rows = N - y0 < kHyRows ? N - y0 : kHyRows;
-    for (uint32_t i = t; i < R * kHyRows; i += BS) acc[i] = 0;
-    __syncthreads();
     const uint32_t e0 = ptr[y0], e1 = ptr[y0 + rows];
-    for (uint32_t w = t; w < e1 - e0; w += BS) {
-        const uint4 en = ld4(ent + kHyWords * (uint64_t)(e0 + w));  // {x, z, coef, y}
+    // entry e's R products (coef kept apart: multiplied when added)
+    auto load = [&](uint32_t w, uint4 &en, uint32_t (&q)[3]) {
+        en = ld4(ent + kHyWords * (uint64_t)(e0 + w));  // {x, z, coef, y}
+#pragma unroll
+        for (uint32_t j = 0; j < 3; j++) q[j] = j < R ? m31_mul(eqz[j * eqz_rs + en.y], eqx[j * eqx_rs + en.x]) : 0u;
+    };
+    auto add = [&](const uint4 &en, const uint32_t (&q)[3]) {
 #pragma unroll
         for (uint32_t j = 0; j < 3; j++)
-            if (j < R)
-                atomicAdd(acc + j * kHyRows + (en.w - y0),
-                          (unsigned long long)m31_mul_wide(m31_mul(eqz[j * eqz_rs + en.y], eqx[j * eqx_rs + en.x]),
-                                                           en.z));
+            if (j < R) atomicAdd(acc + j * kHyRows + (en.w - y0), (unsigned long long)m31_mul_wide(q[j], en.z));
+    };
+    // the first entry's gathers overlap the clear of the accumulators and the barrier
+    uint4 en0 = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t q0[3] = {};
+    if (t < e1 - e0) load(t, en0, q0);
+    for (uint32_t i = t; i < R * kHyRows; i += BS) acc[i] = 0;
+    __syncthreads();
+    if (t < e1 - e0) add(en0, q0);
+    for (uint32_t w = t + BS; w < e1 - e0; w += BS) {
+        uint4 en;
+        uint32_t q[3];
+        load(w, en, q);
+        add(en, q);
     }
     __syncthreads();
     for (uint32_t i = t; i < R * rows; i += BS) {
