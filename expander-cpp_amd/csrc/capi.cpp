@@ -762,7 +762,7 @@ class PrefixRunner final : public BatchPrefixes {
     {
         t0_ = std::chrono::steady_clock::now();
         const bool x8 = kind_ == 1 && host_keccak_x8();
-        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const unsigned nt = std::max(1u, std::min(16u, usable_cpus()));
         std::vector<uint8_t> taken(n_, 0);
         for (size_t k = 0; k < order.size(); k++) {
             const uint32_t lead = order[k];

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <atomic>
@@ -32,11 +33,24 @@ constexpr uint32_t kAddMark = 0xffffffffu;  // "no second input" in a CSR entry
 // can spread its entries evenly over a workgroup; 16 bytes load in one instruction.
 constexpr uint32_t kFwWords = 3, kGxWords = 4, kHyWords = 4;
 
+// Host CPUs this process may run on (its affinity mask; hardware_concurrency counts every
+// online CPU, which can be many more than a container or a job scheduler grants).
+inline unsigned usable_cpus()
+{
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof set, &set) == 0) {
+        const int c = CPU_COUNT(&set);
+        if (c > 0) return (unsigned)c;
+    }
+    return std::max(1u, std::thread::hardware_concurrency());
+}
+
 // Runs fn(i) for i < n on up to max_threads host threads (the caller is one of them).
 template <typename F>
 void parallel_for(size_t n, unsigned max_threads, F &&fn)
 {
-    const unsigned hw = std::max(1u, std::min(max_threads, std::thread::hardware_concurrency()));
+    const unsigned hw = std::max(1u, std::min(max_threads, usable_cpus()));
     const unsigned nt = (unsigned)std::min<size_t>(hw, n);
     std::atomic<size_t> next{0};
     auto work = [&] {

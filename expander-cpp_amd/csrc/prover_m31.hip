@@ -858,41 +858,84 @@ __global__ void __launch_bounds__(kThreads) k_pair(RoundParams p)
 }
 
 // Adds the block-major partials [nb][nvals] of k_pair (one repetition: nvals = kPairVals L)
-// or k_dot (nvals = R 3L; at most 144 values) into sums[nvals] (LDS).  Each block's partials
-// are one contiguous row, written whole by the block (a value-major layout would have every
-// block write one word of lines that neighbouring blocks on other XCDs also write, and each
-// XCD's L2 write its own partial copy of them back).  Thread t adds value t % nvals of
-// blocks t / nvals, t / nvals + G, ... (G = kFinThreads / nvals groups, at least 7): the
-// threads of a group read a row with coalesced loads, all of a thread's loads are issued
-// before the first add (one memory latency), and the G group sums meet in LDS.  blockDim is
-// kFinThreads; every thread must call it.
-constexpr uint32_t kFinThreads = 1024;
+// or k_dot (nvals = R 3L) into sums[nvals] (LDS), with NT threads.  nvals is a multiple of 4
+// and at most kFinMaxVals; nb is at most kMaxBlocksPerRep.  Each block's partials are one
+// contiguous, 16-byte aligned row, written whole by the block (a value-major layout would
+// have every block write one word of lines that neighbouring blocks on other XCDs also
+// write, and each XCD's L2 write its own partial copy of them back).  The sums are u64 (at
+// most 256 canonical values: exact) and the group sums meet in LDS.  Every thread must call
+// it.
+//   NT = kFinThreads: thread t adds value t % nvals of rows t / nvals, t / nvals + G, ...
+//     (G = NT / nvals groups, at least 7); all of a thread's loads are issued before the
+//     first add, so the partials cost one memory latency.
+//   smaller NT: a row is W = nvals / 4 16-byte words; G = NT / W groups of W threads read
+//     rows grp, grp + G, ... with 16-byte loads, kFinBatch rows per thread in flight.
+constexpr uint32_t kFinThreads = 1024;   // single-proof ends: one workgroup per repetition, alone
+constexpr uint32_t kLsFinThreads = 256;  // batch ends: beside the other lanes' kernels (see k_ls_fin)
 constexpr uint32_t kFinMaxVals = 144;
-template <int L>
+constexpr uint32_t kFinBatch = 16;
+template <uint32_t NT>
 __device__ __forceinline__ void reduce_partials(const uint32_t *__restrict__ partials, uint32_t nb, uint32_t nvals,
                                                 uint32_t *sums)
 {
-    constexpr uint32_t kGroupsMin = kFinThreads / kFinMaxVals;
-    constexpr uint32_t kPerThread = (kMaxBlocksPerRep + kGroupsMin - 1) / kGroupsMin;
-    __shared__ unsigned long long part[kFinThreads];
-    const uint32_t t = threadIdx.x, G = kFinThreads / nvals, v = t % nvals, grp = t / nvals;
-    uint32_t x[kPerThread];
+    static_assert(NT >= kFinMaxVals, "a workgroup must cover one row of partials");
+    if constexpr (NT >= kFinThreads) {
+        constexpr uint32_t kGroupsMin = NT / kFinMaxVals;
+        constexpr uint32_t kPerThread = (kMaxBlocksPerRep + kGroupsMin - 1) / kGroupsMin;
+        __shared__ unsigned long long part1[NT];
+        const uint32_t t = threadIdx.x, G = NT / nvals, v = t % nvals, grp = t / nvals;
+        uint32_t x[kPerThread];
 #pragma unroll
-    for (uint32_t i = 0; i < kPerThread; i++) {
-        const uint32_t b = grp + i * G;
-        x[i] = (grp < G && b < nb) ? partials[(uint64_t)b * nvals + v] : 0u;
-    }
-    unsigned long long a = 0;
+        for (uint32_t i = 0; i < kPerThread; i++) {
+            const uint32_t b = grp + i * G;
+            x[i] = (grp < G && b < nb) ? partials[(uint64_t)b * nvals + v] : 0u;
+        }
+        unsigned long long a = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < kPerThread; i++) a += x[i];
-    part[t] = a;
-    __syncthreads();
-    if (t < nvals) {
-        unsigned long long sum = 0;
-        for (uint32_t g = 0; g < G; g++) sum += part[g * nvals + t];
-        sums[t] = m31_reduce64(sum);
+        for (uint32_t i = 0; i < kPerThread; i++) a += x[i];
+        part1[t] = a;
+        __syncthreads();
+        if (t < nvals) {
+            unsigned long long sum = 0;
+            for (uint32_t g = 0; g < G; g++) sum += part1[g * nvals + t];
+            sums[t] = m31_reduce64(sum);
+        }
+        __syncthreads();
+    } else {
+        __shared__ unsigned long long part[4 * NT];  // [group][value]: G nvals <= 4 NT
+        const uint32_t t = threadIdx.x, W = nvals / 4, G = NT / W, grp = t / W, w = t % W;
+        if (grp < G) {
+            const uint4 *p = reinterpret_cast<const uint4 *>(partials) + w;
+            unsigned long long a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            for (uint32_t b0 = grp; b0 < nb; b0 += kFinBatch * G) {
+                uint4 x[kFinBatch];
+#pragma unroll
+                for (uint32_t i = 0; i < kFinBatch; i++) {
+                    const uint32_t b = b0 + i * G;
+                    x[i] = b < nb ? p[(uint64_t)b * W] : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < kFinBatch; i++) {
+                    a0 += x[i].x;
+                    a1 += x[i].y;
+                    a2 += x[i].z;
+                    a3 += x[i].w;
+                }
+            }
+            unsigned long long *q = part + grp * nvals + 4 * w;
+            q[0] = a0;
+            q[1] = a1;
+            q[2] = a2;
+            q[3] = a3;
+        }
+        __syncthreads();
+        for (uint32_t v = t; v < nvals; v += NT) {
+            unsigned long long sum = 0;
+            for (uint32_t g = 0; g < G; g++) sum += part[g * nvals + v];
+            sums[v] = m31_reduce64(sum);
+        }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // Round k's three sums per lane from the nine product sums vals[q L + lane] of a round
@@ -944,7 +987,7 @@ __device__ __forceinline__ uint32_t rep_step(const uint32_t *sums, const TsArgs 
     return challenge_from(d);
 }
 
-// The end of a pair of grid rounds for repetition `rep` (one workgroup of kFinThreads per
+// The end of a pair of grid rounds for repetition `rep` (one workgroup of NT threads per
 // repetition): adds rep's partials (the nine products P0..P8 of pair_item per lane), forms
 // round k's sums and draws r_k, evaluates round k+1's sums at r_k and draws r_{k+1}.  Before
 // the last x round each repetition's challenge hashes only its own message (rep j's message
@@ -952,14 +995,14 @@ __device__ __forceinline__ uint32_t rep_step(const uint32_t *sums, const TsArgs 
 // hashes the unhashed bytes from hs0 on), so the R workgroups run at once and read no
 // transcript state: the host tracks len0 and hs0, and the workgroup of the last rep writes
 // the state after round k+1.  ts.k is round k.
-template <int L>
+template <int L, uint32_t NT>
 __device__ __forceinline__ void pair_fin_rep_body(const uint32_t *__restrict__ partials, uint32_t nb, const TsArgs &ts,
                                   uint32_t len0, uint32_t hs0, uint32_t rep)
 {
     constexpr uint32_t Q = 3 * L, V = kPairVals * L;
     __shared__ uint32_t vals[V], sums[Q], hb[L + Q], rr;
     const uint32_t R = ts.R, t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    reduce_partials<L>(partials + (uint64_t)rep * V * nb, nb, V, vals);
+    reduce_partials<NT>(partials + (uint64_t)rep * V * nb, nb, V, vals);
     pair_sums_first<L>(vals, sums);
     __syncthreads();
     uint32_t *chal = ts.chal + rep * kChalStride + (ts.phase == 2 ? 32u : 0u);
@@ -992,7 +1035,7 @@ __global__ void __launch_bounds__(kFinThreads) k_pair_fin(const uint32_t *__rest
                                                           TsArgs ts, uint32_t len0, uint32_t hs0)
 {
     latency_critical();
-    pair_fin_rep_body<L>(partials, nb, ts, len0, hs0, blockIdx.x);
+    pair_fin_rep_body<L, kFinThreads>(partials, nb, ts, len0, hs0, blockIdx.x);
 }
 
 // Rounds k0 .. n-1 of one phase in a single workgroup of 768 threads: the tables are
@@ -1196,7 +1239,7 @@ __global__ void __launch_bounds__(kFinThreads) k_dot_fin(const uint32_t *__restr
     latency_critical();
     constexpr uint32_t Q = 3 * L;
     __shared__ uint32_t sums[3 * Q];
-    reduce_partials<L>(partials, nb, R * Q, sums);
+    reduce_partials<kFinThreads>(partials, nb, R * Q, sums);
     if (threadIdx.x < R * L) claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
 }
 
@@ -1750,9 +1793,11 @@ __device__ __forceinline__ TailParams ls_tail_params(const BatchJob &J, uint32_t
 }
 
 // The short ends of a step (one workgroup each): the partials and the two transcript
-// steps after a pair of grid rounds, and the claims.
+// steps after a pair of grid rounds, and the claims.  Workgroups of kLsFinThreads (one wave
+// per SIMD): they start as soon as one slot per SIMD is free beside the other lanes'
+// kernels, where a workgroup of 1 024 threads at this register count needs a whole CU.
 template <int L>
-__global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restrict__ jobs,
+__global__ void __launch_bounds__(kLsFinThreads) k_ls_fin(const BatchJob *__restrict__ jobs,
                                                         const LsTask *__restrict__ tasks,
                                                         const uint32_t *__restrict__ start, uint32_t ntasks,
                                                         uint32_t R, int hash_kind)
@@ -1766,9 +1811,9 @@ __global__ void __launch_bounds__(kFinThreads) k_ls_fin(const BatchJob *__restri
     if (T.op == kOpPairFin) {
         TsArgs ts = ls_ts<L>(J, T.l, T.phase, R, hash_kind);
         ts.k = T.k;
-        pair_fin_rep_body<L>(J.partials, T.nb, ts, T.len0, T.hs0, f.y);  // one workgroup per rep
+        pair_fin_rep_body<L, kLsFinThreads>(J.partials, T.nb, ts, T.len0, T.hs0, f.y);  // one workgroup per rep
     } else {
-        reduce_partials<L>(J.partials, T.nb, R * Q, sums);
+        reduce_partials<kLsFinThreads>(J.partials, T.nb, R * Q, sums);
         if (threadIdx.x < R * L) J.claims[threadIdx.x] = sums[(threadIdx.x / L) * Q + threadIdx.x % L];
     }
 }
@@ -3105,10 +3150,10 @@ int dev_prove_batch(const std::vector<BatchItem> &items, uint32_t R, int hash_ki
         auto fin = [&](const LsList &ls, hipStream_t sx, int cls) {
             timed(pf, cls, sx, [&] {
                 if (L == 8)
-                    hipLaunchKernelGGL(k_ls_fin<8>, dim3(ls.blocks), dim3(kFinThreads), 0, sx, djobs, dtasks + ls.task,
+                    hipLaunchKernelGGL(k_ls_fin<8>, dim3(ls.blocks), dim3(kLsFinThreads), 0, sx, djobs, dtasks + ls.task,
                                        dstarts + ls.start, ls.n, R, hash_kind);
                 else
-                    hipLaunchKernelGGL(k_ls_fin<16>, dim3(ls.blocks), dim3(kFinThreads), 0, sx, djobs, dtasks + ls.task,
+                    hipLaunchKernelGGL(k_ls_fin<16>, dim3(ls.blocks), dim3(kLsFinThreads), 0, sx, djobs, dtasks + ls.task,
                                        dstarts + ls.start, ls.n, R, hash_kind);
             });
         };

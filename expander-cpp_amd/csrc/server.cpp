@@ -407,7 +407,7 @@ try {
         S->slots.push_back(std::move(sl));
     }
     ctx->live_circuits++;  // the context outlives the server
-    const unsigned nw = std::max(1u, std::min(12u, std::thread::hardware_concurrency()));
+    const unsigned nw = std::max(1u, std::min(12u, usable_cpus()));
     for (unsigned i = 0; i < nw; i++) S->workers.emplace_back(prefix_worker, S);
     S->sched = std::thread(scheduler, S);
     *out = S;
