@@ -56,6 +56,19 @@ __device__ __forceinline__ void latency_critical() { __builtin_amdgcn_s_setprio(
 
 __device__ __forceinline__ uint4 ld4(const uint32_t *p) { return *reinterpret_cast<const uint4 *>(p); }
 __device__ __forceinline__ void st4(uint32_t *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
+// One entry of an interleaved eq table (kEqWords words: rep 0, 1, 2), moved as one 12-byte
+// access (global_load_dwordx3 / global_store_dwordx3: 4-byte alignment is enough).
+constexpr uint32_t kEqWords = 3;
+struct Eq3 {
+    uint32_t r0, r1, r2;
+};
+__device__ __forceinline__ Eq3 ld3(const uint32_t *p)
+{
+    Eq3 v;
+    __builtin_memcpy(&v, p, sizeof v);
+    return v;
+}
+__device__ __forceinline__ void st3(uint32_t *p, const Eq3 &v) { __builtin_memcpy(p, &v, sizeof v); }
 __device__ __forceinline__ uint4 add4(uint4 a, uint4 b)
 {
     return make_uint4(m31_add(a.x, b.x), m31_add(a.y, b.y), m31_add(a.z, b.z), m31_add(a.w, b.w));
@@ -164,68 +177,92 @@ __global__ void __launch_bounds__(kThreads) k_forward(const uint32_t *__restrict
 }
 
 // ------------------------------------------------------------------ eq tables
-// out[rep][z] = f1 * eq(r1_rep, z) + f2 * eq(r2_rep, z), z < 2^m, with
-// eq(r, z) = prod_i (z_i ? r_i : 1 - r_i) (the reference's _eq at y = 0 / 1).
-// f1, f2 come from ab[0], ab[1] when ab is given, else f1 = 1 and the second term is absent.
-// A block of BS threads writes z in [chunk * kEqChunk * BS, (chunk + 1) * kEqChunk * BS):
-// the low log2(BS) bits of z index a table of BS partial products, the high bits one of
-// kEqChunk, both in LDS.
-constexpr int kEqChunk = 16;  // z values per thread
-template <int BS>
-__device__ __forceinline__ void eq_block(uint32_t *__restrict__ o, const uint32_t *__restrict__ r1, const uint32_t *__restrict__ r2,
-                         const uint32_t *__restrict__ ab, uint32_t m, uint32_t chunk)
+// The eq tables of the R <= 3 repetitions are interleaved: entry z is kEqWords words,
+// out[3 z + j] = f1 * eq(r1_j, z) + f2 * eq(r2_j, z) for rep j < R (words R .. 2 are
+// zero), so that a gather of entry z (g_x, h_y: entries in wiring order) is one 12-byte
+// load for every repetition.  eq(r, z) = prod_i (z_i ? r_i : 1 - r_i) (the
+// reference's _eq at y = 0 / 1, include/LinearGKR/sumcheck_common.hpp:9-53).  f1, f2 come
+// from ab[0], ab[1] when ab is given, else f1 = 1 and the second term is absent.  r1_j /
+// r2_j are chal + j kChalStride + r1_off / r2_off.  A block of BS threads writes z in
+// [chunk * CH * BS, (chunk + 1) * CH * BS): the low log2(BS) bits of z index a table of BS
+// partial products per rep, the high bits one of CH, both in LDS.
+constexpr int kEqChunk = 16;      // z values per thread in the batch engine's eq ops
+constexpr int kEqChunkOne = 4;    // in k_eq (one table per launch: more blocks)
+template <int BS, int CH = kEqChunk>
+__device__ __forceinline__ void eq_block(uint32_t *__restrict__ o, const uint32_t *__restrict__ chal, uint32_t r1_off,
+                                         uint32_t r2_off, const uint32_t *__restrict__ ab, uint32_t m, uint32_t chunk,
+                                         uint32_t R)
 {
     constexpr uint32_t LOG = BS == 64 ? 6 : BS == 128 ? 7 : BS == 256 ? 8 : 9;
     static_assert((1 << LOG) == BS, "block size must be 64, 128, 256 or 512");
-    __shared__ uint32_t lo1[BS], lo2[BS], hi1[kEqChunk], hi2[kEqChunk];
+    // the challenges r1_j, r2_j (at most kMaxVars each) are staged in LDS, so the product
+    // chains below read them without a memory round trip per factor
+    __shared__ uint32_t rv[2][3][kMaxVars];
+    __shared__ uint32_t lo1[3][BS], lo2[3][BS], hi1[3][CH], hi2[3][CH];
     const bool two = ab != nullptr;
     const uint32_t f1 = two ? ab[0] : 1u, f2 = two ? ab[1] : 0u;
     const uint32_t h = m < LOG ? m : LOG, hb = m - h;
     const uint32_t t = threadIdx.x;
-    const uint64_t base = (uint64_t)chunk * (kEqChunk * BS);
-    if (t < (1u << h)) {
-        uint32_t a = f1, b = f2;
-        for (uint32_t i = 0; i < h; i++) {
-            const uint32_t bit = (t >> i) & 1u;
-            a = m31_mul(a, bit ? r1[i] : m31_sub(1u, r1[i]));
-            if (two) b = m31_mul(b, bit ? r2[i] : m31_sub(1u, r2[i]));
-        }
-        lo1[t] = a;
-        lo2[t] = b;
+    const uint64_t base = (uint64_t)chunk * (CH * BS);
+    for (uint32_t u = t; u < 2 * 3 * kMaxVars; u += BS) {
+        const uint32_t w = u / (3 * kMaxVars), j = (u / kMaxVars) % 3, i = u % kMaxVars;
+        rv[w][j][i] = (j < R && i < m && (w == 0 || two)) ? chal[j * kChalStride + (w ? r2_off : r1_off) + i] : 0u;
     }
-    if (t < kEqChunk) {
-        const uint64_t hi = (base >> h) + t;
+    __syncthreads();
+    // low tables: entry e of rep j is f prod_{i < h} (e_i ? r_i : 1 - r_i)
+    for (uint32_t u = t; u < (R << h); u += BS) {
+        const uint32_t j = u >> h, e = u & ((1u << h) - 1u);
+        uint32_t a = f1, b = f2;
+#pragma unroll
+        for (uint32_t i = 0; i < LOG; i++)
+            if (i < h) {
+                const uint32_t bit = (e >> i) & 1u, x = rv[0][j][i], y = rv[1][j][i];
+                a = m31_mul(a, bit ? x : m31_sub(1u, x));
+                if (two) b = m31_mul(b, bit ? y : m31_sub(1u, y));
+            }
+        lo1[j][e] = a;
+        lo2[j][e] = b;
+    }
+    // high tables: the CH high indices of this block's z range, bits h .. m - 1
+    for (uint32_t u = t; u < R * CH; u += BS) {
+        const uint32_t j = u / CH, c = u % CH;
+        const uint64_t hi = (base >> h) + c;
         uint32_t a = 1u, b = 1u;
-        for (uint32_t i = 0; i < hb; i++) {
-            const uint32_t bit = (uint32_t)(hi >> i) & 1u;
-            a = m31_mul(a, bit ? r1[h + i] : m31_sub(1u, r1[h + i]));
-            if (two) b = m31_mul(b, bit ? r2[h + i] : m31_sub(1u, r2[h + i]));
-        }
-        hi1[t] = a;
-        hi2[t] = b;
+#pragma unroll
+        for (uint32_t i = 0; i < kMaxVars; i++)
+            if (i < hb) {
+                const uint32_t bit = (uint32_t)(hi >> i) & 1u, x = rv[0][j][h + i], y = rv[1][j][h + i];
+                a = m31_mul(a, bit ? x : m31_sub(1u, x));
+                if (two) b = m31_mul(b, bit ? y : m31_sub(1u, y));
+            }
+        hi1[j][c] = a;
+        hi2[j][c] = b;
     }
     __syncthreads();
     const uint64_t M = 1ull << m;
     const uint32_t mask = (1u << h) - 1u;
-    for (int c = 0; c < kEqChunk; c++) {
+    for (int c = 0; c < CH; c++) {
         const uint64_t z = base + (uint64_t)c * BS + t;
         if (z < M) {
             const uint32_t li = (uint32_t)z & mask;
             const uint32_t hidx = (uint32_t)((z >> h) - (base >> h));
-            uint32_t v = m31_mul(lo1[li], hi1[hidx]);
-            if (two) v = m31_add(v, m31_mul(lo2[li], hi2[hidx]));
-            o[z] = v;
+            uint32_t v[3] = {0u, 0u, 0u};
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++)
+                if (j < R) {
+                    v[j] = m31_mul(lo1[j][li], hi1[j][hidx]);
+                    if (two) v[j] = m31_add(v[j], m31_mul(lo2[j][li], hi2[j][hidx]));
+                }
+            st3(o + kEqWords * z, Eq3{v[0], v[1], v[2]});
         }
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, uint64_t out_rs,
-                                                 const uint32_t *__restrict__ chal, uint32_t r1_off, uint32_t r2_off,
-                                                 const uint32_t *__restrict__ ab, uint32_t m)
+__global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, const uint32_t *__restrict__ chal,
+                                                 uint32_t r1_off, uint32_t r2_off, const uint32_t *__restrict__ ab,
+                                                 uint32_t m, uint32_t R)
 {
-    const uint32_t rep = blockIdx.y;
-    eq_block<kThreads>(out + rep * out_rs, chal + (uint64_t)rep * kChalStride + r1_off,
-                       chal + (uint64_t)rep * kChalStride + r2_off, ab, m, blockIdx.x);
+    eq_block<kThreads, kEqChunkOne>(out, chal, r1_off, r2_off, ab, m, blockIdx.x, R);
 }
 
 // ------------------------------------------------------------------ g_x and h_y
@@ -248,7 +285,7 @@ constexpr uint32_t kGxAccWords = 3 * 1024;
 
 template <int L, int BS>
 __device__ __forceinline__ void gx_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ hg, uint64_t hg_rs,
-                        const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
+                        const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz,
                         const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N, uint32_t R,
                         uint32_t x0)
 {
@@ -268,8 +305,10 @@ __device__ __forceinline__ void gx_tile(unsigned long long *__restrict__ acc, ui
         const uint32_t g = (uint32_t)(w % LG);
         it.en = ld4(ent + kGxWords * e);
         it.vb = it.en.x != kAddMark ? ld4(v + (uint64_t)it.en.x * L + 4 * g) : make_uint4(1u, 1u, 1u, 1u);
-#pragma unroll
-        for (uint32_t j = 0; j < 3; j++) it.c[j] = j < R ? m31_mul(it.en.z, eqz[j * eqz_rs + it.en.y]) : 0u;
+        const Eq3 ez = ld3(eqz + kEqWords * (uint64_t)it.en.y);  // eqZ_j[z] of every rep
+        it.c[0] = m31_mul(it.en.z, ez.r0);
+        it.c[1] = R > 1 ? m31_mul(it.en.z, ez.r1) : 0u;
+        it.c[2] = R > 2 ? m31_mul(it.en.z, ez.r2) : 0u;
         return it;
     };
     auto add = [&](const Item &it, uint64_t w) {
@@ -307,14 +346,14 @@ __device__ __forceinline__ void gx_tile(unsigned long long *__restrict__ acc, ui
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint64_t hg_rs,
                                                  const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz,
-                                                 uint64_t eqz_rs, const uint32_t *__restrict__ ptr,
+                                                 const uint32_t *__restrict__ ptr,
                                                  const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
 {
     __shared__ unsigned long long acc[kGxAccWords];
     constexpr uint32_t ROWS = gx_rows<L, kThreads>();
     const uint64_t tiles = (N + ROWS - 1) / ROWS;
     for (uint64_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x)
-        gx_tile<L, kThreads>(acc, hg, hg_rs, v, eqz, eqz_rs, ptr, ent, N, R, (uint32_t)(tile * ROWS));
+        gx_tile<L, kThreads>(acc, hg, hg_rs, v, eqz, ptr, ent, N, R, (uint32_t)(tile * ROWS));
 }
 
 // s_j[y] = sum over the mul gates with second input y of eqZ_j[z] * eqX_j[x] * c, for every
@@ -323,9 +362,9 @@ constexpr uint32_t kHyAccWords = 3 * 1024;
 
 template <int BS>
 __device__ __forceinline__ void hy_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ s, uint64_t s_rs,
-                        const uint32_t *__restrict__ eqz, uint64_t eqz_rs, const uint32_t *__restrict__ eqx,
-                        uint64_t eqx_rs, const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
-                        uint32_t N, uint32_t R, uint32_t y0)
+                        const uint32_t *__restrict__ eqz, const uint32_t *__restrict__ eqx,
+                        const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N, uint32_t R,
+                        uint32_t y0)
 {
     constexpr uint32_t kHyRows = tile_lanes(BS);
     const uint32_t t = threadIdx.x;
@@ -334,8 +373,10 @@ __device__ __forceinline__ void hy_tile(unsigned long long *__restrict__ acc, ui
     // entry e's R products (coef kept apart: multiplied when added)
     auto load = [&](uint32_t w, uint4 &en, uint32_t (&q)[3]) {
         en = ld4(ent + kHyWords * (uint64_t)(e0 + w));  // {x, z, coef, y}
-#pragma unroll
-        for (uint32_t j = 0; j < 3; j++) q[j] = j < R ? m31_mul(eqz[j * eqz_rs + en.y], eqx[j * eqx_rs + en.x]) : 0u;
+        const Eq3 a = ld3(eqz + kEqWords * (uint64_t)en.y), b = ld3(eqx + kEqWords * (uint64_t)en.x);  // every rep
+        q[0] = m31_mul(a.r0, b.r0);
+        q[1] = R > 1 ? m31_mul(a.r1, b.r1) : 0u;
+        q[2] = R > 2 ? m31_mul(a.r2, b.r2) : 0u;
     };
     auto add = [&](const uint4 &en, const uint32_t (&q)[3]) {
 #pragma unroll
@@ -364,8 +405,8 @@ __device__ __forceinline__ void hy_tile(unsigned long long *__restrict__ acc, ui
 }
 
 __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint64_t s_rs,
-                                                 const uint32_t *__restrict__ eqz, uint64_t eqz_rs,
-                                                 const uint32_t *__restrict__ eqx, uint64_t eqx_rs,
+                                                 const uint32_t *__restrict__ eqz,
+                                                 const uint32_t *__restrict__ eqx,
                                                  const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
                                                  uint32_t N, uint32_t R)
 {
@@ -373,7 +414,7 @@ __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint6
     constexpr uint32_t ROWS = tile_lanes(kThreads);
     const uint64_t tiles = (N + ROWS - 1) / ROWS;
     for (uint64_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x)
-        hy_tile<kThreads>(acc, s, s_rs, eqz, eqz_rs, eqx, eqx_rs, ptr, ent, N, R, (uint32_t)(tile * ROWS));
+        hy_tile<kThreads>(acc, s, s_rs, eqz, eqx, ptr, ent, N, R, (uint32_t)(tile * ROWS));
 }
 
 // ------------------------------------------------------------------ reductions
@@ -1215,18 +1256,18 @@ __global__ void __launch_bounds__(kRepThreads) k_tail_rep(TailRepParams q)
 // (the p0 slot of the round layout; the other slots are zero) that k_dot_fin adds.
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_dot(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ eq,
-                                                  uint64_t eq_rs, uint32_t M, uint32_t *__restrict__ partials)
+                                                  uint32_t M, uint32_t *__restrict__ partials)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
     __shared__ uint32_t red[4][3 * L];
     const uint32_t rep = blockIdx.y;
     const uint32_t g = threadIdx.x % LG;
-    const uint32_t *e = eq + rep * eq_rs;
+    const uint32_t *e = eq + rep;  // interleaved eq table: rep j of entry z at 3 z + j
     uint64_t acc[3][4] = {};
     const uint64_t total = (uint64_t)M * LG;
     for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads) {
         const uint64_t z = w / LG;
-        const uint4 t = mul4s(ld4(vals + z * L + 4 * g), e[z]);
+        const uint4 t = mul4s(ld4(vals + z * L + 4 * g), e[kEqWords * z]);
         acc[0][0] += t.x; acc[0][1] += t.y; acc[0][2] += t.z; acc[0][3] += t.w;
     }
     group_reduce<L>(acc, red, partials + ((uint64_t)blockIdx.x * gridDim.y + rep) * Q, 1);
@@ -1256,8 +1297,9 @@ struct BatchJob {
     uint32_t D;
     const LayerDev *layers;  // device array of D layers
     uint32_t *values;
-    uint32_t *eqz, *eqx, *hg0, *hgA, *hgB, *fA, *fB, *s0, *sA, *sB;
-    uint64_t eqz_rs, eqx_rs, hg0_rs, hgA_rs, hgB_rs, fA_rs, fB_rs, s0_rs, sA_rs, sB_rs;
+    uint32_t *eqz, *eqx;     // interleaved eq tables (kEqWords words per entry)
+    uint32_t *hg0, *hgA, *hgB, *fA, *fB, *s0, *sA, *sB;
+    uint64_t hg0_rs, hgA_rs, hgB_rs, fA_rs, fB_rs, s0_rs, sA_rs, sB_rs;
     uint32_t *chal;          // (D + 1) * R * kChalStride
     uint32_t *ab;            // 2 (D + 1)
     uint32_t *vxy;           // 2 R L
@@ -1276,10 +1318,11 @@ __device__ void block_forward(const uint32_t *__restrict__ vin, uint32_t *__rest
     for (uint64_t w = threadIdx.x; w < (uint64_t)M * LG; w += blockDim.x) forward_item<L>(vin, vout, ptr, ent, w);
 }
 
-// out[j][z] = f1 eq(r1_j, z) + f2 eq(r2_j, z) for z < 2^m (m <= 20), rep by rep, from
-// 2^10-entry low / high tables in LDS.  ab == nullptr: f1 = 1 and no second term.
-__device__ void block_eq(uint32_t *__restrict__ out, uint64_t out_rs, const uint32_t *__restrict__ chal,
-                         uint32_t r1_off, uint32_t r2_off, const uint32_t *__restrict__ ab, uint32_t m, uint32_t R)
+// out[3 z + j] = f1 eq(r1_j, z) + f2 eq(r2_j, z) for z < 2^m (m <= 20), the interleaved
+// layout of eq_block, rep by rep, from 2^10-entry low / high tables in LDS.  ab == nullptr:
+// f1 = 1 and no second term.
+__device__ void block_eq(uint32_t *__restrict__ out, const uint32_t *__restrict__ chal, uint32_t r1_off,
+                         uint32_t r2_off, const uint32_t *__restrict__ ab, uint32_t m, uint32_t R)
 {
     __shared__ uint32_t lo1[1024], lo2[1024], hi1[1024], hi2[1024];
     const bool two = ab != nullptr;
@@ -1309,11 +1352,10 @@ __device__ void block_eq(uint32_t *__restrict__ out, uint64_t out_rs, const uint
         }
         __syncthreads();
         const uint32_t mask = (1u << h) - 1u;
-        uint32_t *o = out + j * out_rs;
         for (uint32_t z = threadIdx.x; z < (1u << m); z += blockDim.x) {
             uint32_t v = m31_mul(lo1[z & mask], hi1[z >> h]);
             if (two) v = m31_add(v, m31_mul(lo2[z & mask], hi2[z >> h]));
-            o[z] = v;
+            out[kEqWords * (uint64_t)z + j] = v;
         }
         __syncthreads();
     }
@@ -1321,8 +1363,8 @@ __device__ void block_eq(uint32_t *__restrict__ out, uint64_t out_rs, const uint
 
 // claimed_v[j] = sum_z eq_j[z] * out[z], group j on rep j
 template <int L>
-__device__ void block_claims(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ eq, uint64_t eq_rs,
-                             uint32_t M, uint32_t R, uint32_t *__restrict__ claims)
+__device__ void block_claims(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ eq, uint32_t M, uint32_t R,
+                             uint32_t *__restrict__ claims)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
     __shared__ uint32_t red[4][4][3 * L];
@@ -1330,10 +1372,10 @@ __device__ void block_claims(const uint32_t *__restrict__ vals, const uint32_t *
     const uint32_t grp = threadIdx.x >> 8, tg = threadIdx.x & 255, g = tg % LG;
     uint64_t acc[3][4] = {};
     if (grp < R) {
-        const uint32_t *e = eq + grp * eq_rs;
+        const uint32_t *e = eq + grp;  // interleaved eq table
         for (uint64_t w = tg; w < (uint64_t)M * LG; w += 256) {
             const uint64_t z = w / LG;
-            const uint4 t = mul4s(ld4(vals + z * L + 4 * g), e[z]);
+            const uint4 t = mul4s(ld4(vals + z * L + 4 * g), e[kEqWords * z]);
             acc[0][0] += t.x; acc[0][1] += t.y; acc[0][2] += t.z; acc[0][3] += t.w;
         }
     }
@@ -1406,10 +1448,10 @@ __device__ void prove_batch_job(const BatchJob &J, uint32_t R, int hash_kind, un
         const uint32_t n = ly.n, m = ly.m;
         const uint32_t *V = J.values + ly.v_off;
         uint32_t *chal_l = chal_slot(l);
-        block_eq(J.eqz, J.eqz_rs, chal_slot(l + 1), 0u, 32u, ab_slot(l + 1), m, R);
-        if (l == (int)D - 1) block_claims<L>(J.values + ly.out_off, J.eqz, J.eqz_rs, 1u << m, R, J.claims);
+        block_eq(J.eqz, chal_slot(l + 1), 0u, 32u, ab_slot(l + 1), m, R);
+        if (l == (int)D - 1) block_claims<L>(J.values + ly.out_off, J.eqz, 1u << m, R, J.claims);
         for (uint32_t x0 = 0; x0 < (1u << n); x0 += gx_rows<L, kBatchThreads>())
-            gx_tile<L, kBatchThreads>(tacc, J.hg0, J.hg0_rs, V, J.eqz, J.eqz_rs, ly.gx_ptr, ly.gx_ent, 1u << n, R, x0);
+            gx_tile<L, kBatchThreads>(tacc, J.hg0, J.hg0_rs, V, J.eqz, ly.gx_ptr, ly.gx_ent, 1u << n, R, x0);
         __syncthreads();
         TsArgs ts{};
         ts.R = R; ts.chal = chal_l; ts.vx = J.vxy; ts.vy = J.vxy + R * L;
@@ -1421,10 +1463,9 @@ __device__ void prove_batch_job(const BatchJob &J, uint32_t R, int hash_kind, un
         tp.ts = ts;
         tp.ts.phase = 1;
         phase_rounds<L, false>(tp);
-        block_eq(J.eqx, J.eqx_rs, chal_l, 0u, 0u, nullptr, n, R);
+        block_eq(J.eqx, chal_l, 0u, 0u, nullptr, n, R);
         for (uint32_t y0 = 0; y0 < (1u << n); y0 += tile_lanes(kBatchThreads))
-            hy_tile<kBatchThreads>(tacc, J.s0, J.s0_rs, J.eqz, J.eqz_rs, J.eqx, J.eqx_rs, ly.hy_ptr, ly.hy_ent, 1u << n,
-                                   R, y0);
+            hy_tile<kBatchThreads>(tacc, J.s0, J.s0_rs, J.eqz, J.eqx, ly.hy_ptr, ly.hy_ent, 1u << n, R, y0);
         __syncthreads();
         tp.h0 = J.s0; tp.h0_rs = J.s0_rs; tp.hA = J.sA; tp.hB = J.sB; tp.hA_rs = J.sA_rs; tp.hB_rs = J.sB_rs;
         tp.ts.phase = 2;
@@ -1446,11 +1487,11 @@ __device__ void prove_batch_job(const BatchJob &J, uint32_t R, int hash_kind, un
 enum LsOp : uint32_t {
     kOpSeed = 0,  // work, 1 block: transcript seed from the host prefix
     kOpFwd,       // work: forward pass of layer l
-    kOpEqz,       // work, R x nb blocks: eqZ of layer l
+    kOpEqz,       // work, nb blocks: eqZ of layer l (every rep, interleaved)
     kOpGx,        // work: g_x of layer l
     kOpDot,       // work, R x nb blocks: claimed_v partials
     kOpPair,      // work, R x nb blocks: grid rounds k, k+1 of a phase
-    kOpEqx,       // work, R x nb blocks: eqX of layer l
+    kOpEqx,       // work, nb blocks: eqX of layer l (every rep, interleaved)
     kOpHy,        // work: h_y of layer l
     kOpPairFin,   // fin, 1 block: partials and transcript steps of grid rounds k, k+1
     kOpDotFin,    // fin, 1 block: claimed_v
@@ -1468,30 +1509,25 @@ struct LsGx {
     uint32_t *hg;
     uint64_t hg_rs;
     const uint32_t *v, *eqz;
-    uint64_t eqz_rs;
     const uint32_t *ptr, *ent;
     uint32_t N;
 };
 struct LsHy {
     uint32_t *s;
     uint64_t s_rs;
-    const uint32_t *eqz;
-    uint64_t eqz_rs;
-    const uint32_t *eqx;
-    uint64_t eqx_rs;
+    const uint32_t *eqz, *eqx;
     const uint32_t *ptr, *ent;
     uint32_t N;
 };
 struct LsEq {
-    uint32_t *out;
-    uint64_t out_rs;
-    const uint32_t *r1, *r2;  // rep 0; rep stride kChalStride
+    uint32_t *out;            // interleaved (kEqWords words per entry)
+    const uint32_t *chal;     // r1 / r2 of rep j at chal + j kChalStride + r1_off / r2_off
+    uint32_t r1_off, r2_off;
     const uint32_t *ab;       // alpha, beta, or nullptr for a single eq table
     uint32_t m;
 };
 struct LsDot {
     const uint32_t *vals, *eq;
-    uint64_t eq_rs;
     uint32_t M;
     uint32_t *partials;
 };
@@ -1653,9 +1689,7 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
     case kOpEqz:
     case kOpEqx: {
         const LsEq &a = T.eq;
-        const uint32_t rep = lb / T.nb;
-        eq_block<BS>(a.out + rep * a.out_rs, a.r1 + (uint64_t)rep * kChalStride, a.r2 + (uint64_t)rep * kChalStride,
-                     a.ab, a.m, lb % T.nb);
+        eq_block<BS>(a.out, a.chal, a.r1_off, a.r2_off, a.ab, a.m, lb, R);
         break;
     }
     case kOpGx: {
@@ -1663,7 +1697,7 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
         constexpr uint32_t ROWS = gx_rows<L, BS>();
         const uint32_t tiles = (a.N + ROWS - 1) / ROWS;
         for (uint32_t tile = (uint32_t)xcd_tile(lb, T.nb); tile < tiles; tile += T.nb)
-            gx_tile<L, BS>(tacc, a.hg, a.hg_rs, a.v, a.eqz, a.eqz_rs, a.ptr, a.ent, a.N, R, tile * ROWS);
+            gx_tile<L, BS>(tacc, a.hg, a.hg_rs, a.v, a.eqz, a.ptr, a.ent, a.N, R, tile * ROWS);
         break;
     }
     case kOpHy: {
@@ -1671,17 +1705,17 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
         constexpr uint32_t ROWS = tile_lanes(BS);
         const uint32_t tiles = (a.N + ROWS - 1) / ROWS;
         for (uint32_t tile = (uint32_t)xcd_tile(lb, T.nb); tile < tiles; tile += T.nb)
-            hy_tile<BS>(tacc, a.s, a.s_rs, a.eqz, a.eqz_rs, a.eqx, a.eqx_rs, a.ptr, a.ent, a.N, R, tile * ROWS);
+            hy_tile<BS>(tacc, a.s, a.s_rs, a.eqz, a.eqx, a.ptr, a.ent, a.N, R, tile * ROWS);
         break;
     }
     case kOpDot: {
         const LsDot &a = T.dot;
         const uint32_t rep = lb / T.nb, bx = lb % T.nb, g = t % LG;
-        const uint32_t *e = a.eq + rep * a.eq_rs;
+        const uint32_t *e = a.eq + rep;  // interleaved eq table
         uint64_t acc[3][4] = {};
         for (uint64_t w = (uint64_t)bx * BS + t; w < (uint64_t)a.M * LG; w += (uint64_t)T.nb * BS) {
             const uint64_t z = w / LG;
-            const uint4 v = mul4s(ld4(a.vals + z * L + 4 * g), e[z]);
+            const uint4 v = mul4s(ld4(a.vals + z * L + 4 * g), e[kEqWords * z]);
             acc[0][0] += v.x; acc[0][1] += v.y; acc[0][2] += v.z; acc[0][3] += v.w;
         }
         block_reduce_to<L, BS>(acc, a.partials + ((uint64_t)bx * R + rep) * Q, 1);
@@ -1947,8 +1981,7 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
     const uint32_t D = dc.D;
-    const uint64_t maxN = 1ull << dc.max_n, maxM = 1ull << dc.max_m;
-    const uint64_t eqz_rs = maxM, eqx_rs = maxN;
+    const uint64_t maxN = 1ull << dc.max_n;
     const uint64_t hg0_rs = maxN * L, hgA_rs = scratch_rows(maxN / 2) * L, hgB_rs = scratch_rows(maxN / 4) * L;
     const uint64_t fA_rs = scratch_rows(maxN / 2) * L, fB_rs = scratch_rows(maxN / 4) * L;
     const uint64_t s0_rs = scratch_rows(maxN), sA_rs = scratch_rows(maxN / 2), sB_rs = scratch_rows(maxN / 4);
@@ -1964,17 +1997,17 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
         const uint32_t *chal_prev = chal_slot(l + 1);
         // eqZ = alpha * eq(rz1, .) + beta * eq(rz2, .), rz1 = rx and rz2 = ry of the layer above
         {
-            dim3 grid((uint32_t)((M + kEqChunk * 256 - 1) / (kEqChunk * 256)), R);
+            dim3 grid((uint32_t)((M + kEqChunkOne * kThreads - 1) / (kEqChunkOne * kThreads)));
             timed(pf, kKEq, st, [&] {
-                hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqz, eqz_rs, chal_prev, 0u, 32u,
-                                   (const uint32_t *)ab_slot(l + 1), m);
+                hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqz, (const uint32_t *)chal_prev, 0u, 32u,
+                                   (const uint32_t *)ab_slot(l + 1), m, R);
             });
         }
         if (l == (int)D - 1) {
             const uint32_t nbx = grid_for(M * LG);
             timed(pf, kKDot, st, [&] {
                 hipLaunchKernelGGL(k_dot<L>, dim3(nbx, R), dim3(kThreads), 0, st, dc.values + ly.out_off, s.eqz,
-                                   eqz_rs, (uint32_t)M, s.partials);
+                                   (uint32_t)M, s.partials);
                 hipLaunchKernelGGL(k_dot_fin<L>, dim3(1), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials, nbx,
                                    R, s.claims);
             });
@@ -1983,20 +2016,20 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
             // one item per thread: the CSR walk is a chain of dependent gathers, hidden by occupancy
             dim3 grid((uint32_t)std::min<uint64_t>((N + gx_rows<L, kThreads>() - 1) / gx_rows<L, kThreads>(), 65536));
             timed(pf, kKGx, st, [&] {
-                hipLaunchKernelGGL(k_gx<L>, grid, dim3(kThreads), 0, st, s.hg0, hg0_rs, V, s.eqz, eqz_rs, ly.gx_ptr,
+                hipLaunchKernelGGL(k_gx<L>, grid, dim3(kThreads), 0, st, s.hg0, hg0_rs, V, s.eqz, ly.gx_ptr,
                                    ly.gx_ent, (uint32_t)N, R);
             });
         }
         for (int phase = 1; phase <= 2; phase++) {
             if (phase == 2) {
-                dim3 grid((uint32_t)((N + kEqChunk * 256 - 1) / (kEqChunk * 256)), R);
+                dim3 grid((uint32_t)((N + kEqChunkOne * kThreads - 1) / (kEqChunkOne * kThreads)));
                 timed(pf, kKEq, st, [&] {
-                    hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqx, eqx_rs, (const uint32_t *)chal_l,
-                                       0u, 0u, (const uint32_t *)nullptr, n);
+                    hipLaunchKernelGGL(k_eq, grid, dim3(kThreads), 0, st, s.eqx, (const uint32_t *)chal_l, 0u, 0u,
+                                       (const uint32_t *)nullptr, n, R);
                 });
                 dim3 g2((uint32_t)std::min<uint64_t>((N + tile_lanes(kThreads) - 1) / tile_lanes(kThreads), 65536));
                 timed(pf, kKHy, st, [&] {
-                    hipLaunchKernelGGL(k_hy, g2, dim3(kThreads), 0, st, s.s0, s0_rs, s.eqz, eqz_rs, s.eqx, eqx_rs,
+                    hipLaunchKernelGGL(k_hy, g2, dim3(kThreads), 0, st, s.s0, s0_rs, s.eqz, s.eqx,
                                        ly.hy_ptr, ly.hy_ent, (uint32_t)N, R);
                 });
             }
@@ -2307,7 +2340,7 @@ uint64_t scratch_layout(DevScratch &s, const DevCircuit &dc, uint32_t R, uint64_
     const uint64_t N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4), N1 = scratch_rows(maxN);
     struct Piece { uint32_t **p; uint64_t u32; };
     Piece pieces[] = {
-        {&s.eqz, R * maxM}, {&s.eqx, R * maxN}, {&s.hg0, R * maxN * L}, {&s.hgA, R * N2 * L},
+        {&s.eqz, kEqWords * maxM}, {&s.eqx, kEqWords * maxN}, {&s.hg0, R * maxN * L}, {&s.hgA, R * N2 * L},
         {&s.hgB, R * N4 * L}, {&s.fA, R * N2 * L}, {&s.fB, R * N4 * L}, {&s.s0, R * N1},
         {&s.sA, R * N2}, {&s.sB, R * N4}, {&s.partials, (uint64_t)R * kMaxBlocksPerRep * kPairVals * L},
         {&s.chal, (uint64_t)(dc.D + 1) * R * kChalStride}, {&s.ab, 2ull * (dc.D + 1)}, {&s.claims, R * L},
@@ -2466,7 +2499,7 @@ uint64_t batch_tables_u32(const DevCircuit &dc, uint32_t R)
     const uint64_t N1 = scratch_rows(maxN), N2 = scratch_rows(maxN / 2), N4 = scratch_rows(maxN / 4);
     uint64_t u = 0;
     auto add = [&](uint64_t n) { u += (n + 63) & ~63ull; };
-    add(R * maxM); add(R * maxN); add(R * maxN * L); add(R * N2 * L); add(R * N4 * L); add(R * N2 * L);
+    add(kEqWords * maxM); add(kEqWords * maxN); add(R * maxN * L); add(R * N2 * L); add(R * N4 * L); add(R * N2 * L);
     add(R * N4 * L); add(R * N1); add(R * N2); add(R * N4); add((uint64_t)(dc.D + 1) * R * kChalStride);
     add(2ull * (dc.D + 1)); add(2ull * R * L); add(16); add((uint64_t)R * kPairVals * L * kMaxBlocksPerRep);
     return u;
@@ -2500,7 +2533,7 @@ uint32_t ls_tile_blocks(uint64_t tiles)
 
 uint32_t ls_work_blocks(const LsTask &t, uint32_t R)
 {
-    return (t.op == kOpEqz || t.op == kOpEqx || t.op == kOpDot || t.op == kOpPair) ? R * t.nb : t.nb;
+    return (t.op == kOpDot || t.op == kOpPair) ? R * t.nb : t.nb;
 }
 
 uint32_t ls_fin_blocks(const LsTask &t, uint32_t R) { return t.op == kOpTailRep || t.op == kOpPairFin ? R : 1u; }
@@ -2563,20 +2596,20 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
         const uint32_t *V = dc.values + ly.v_off;
         {
             LsTask t = ls_task(kOpEqz, job, l);
-            t.eq = LsEq{J.eqz, J.eqz_rs, chal_slot(l + 1), chal_slot(l + 1) + 32, J.ab + 2ull * (l + 1), m};
+            t.eq = LsEq{J.eqz, chal_slot(l + 1), 0u, 32u, J.ab + 2ull * (l + 1), m};
             t.nb = (uint32_t)((M + kEqChunk * BS - 1) / (kEqChunk * BS));
             push_w(t);
         }
         {
             LsEntry e;
             LsTask t = ls_task(kOpGx, job, l);
-            t.gx = LsGx{J.hg0, J.hg0_rs, V, J.eqz, J.eqz_rs, ly.gx_ptr, ly.gx_ent, (uint32_t)N};
+            t.gx = LsGx{J.hg0, J.hg0_rs, V, J.eqz, ly.gx_ptr, ly.gx_ent, (uint32_t)N};
             t.nb = ls_tile_blocks((N * L + tile_lanes(BS) - 1) / tile_lanes(BS));
             e.w[0] = t;
             e.nw = 1;
             if (l == D - 1) {
                 LsTask d = ls_task(kOpDot, job, l);
-                d.dot = LsDot{dc.values + ly.out_off, J.eqz, J.eqz_rs, (uint32_t)M, J.partials};
+                d.dot = LsDot{dc.values + ly.out_off, J.eqz, (uint32_t)M, J.partials};
                 d.nb = ls_grid(M * LG, BS);
                 e.w[1] = d;
                 e.nw = 2;
@@ -2591,11 +2624,11 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
             const bool p2 = phase == 2;
             if (p2) {
                 LsTask t = ls_task(kOpEqx, job, l);
-                t.eq = LsEq{J.eqx, J.eqx_rs, chal_slot(l), chal_slot(l), nullptr, n};
+                t.eq = LsEq{J.eqx, chal_slot(l), 0u, 0u, nullptr, n};
                 t.nb = (uint32_t)((N + kEqChunk * BS - 1) / (kEqChunk * BS));
                 push_w(t);
                 LsTask h = ls_task(kOpHy, job, l);
-                h.hy = LsHy{J.s0, J.s0_rs, J.eqz, J.eqz_rs, J.eqx, J.eqx_rs, ly.hy_ptr, ly.hy_ent, (uint32_t)N};
+                h.hy = LsHy{J.s0, J.s0_rs, J.eqz, J.eqx, ly.hy_ptr, ly.hy_ent, (uint32_t)N};
                 h.nb = ls_tile_blocks((N + tile_lanes(BS) - 1) / tile_lanes(BS));
                 push_w(h);
             }
@@ -2744,8 +2777,8 @@ void batch_job_table(const std::vector<BatchItem> &items, uint32_t R, uint32_t *
         J.D = dc.D;
         J.layers = dc.d_layers;
         J.values = dc.values;
-        J.eqz_rs = maxM; J.eqz = take(R * maxM);
-        J.eqx_rs = maxN; J.eqx = take(R * maxN);
+        J.eqz = take(kEqWords * maxM);
+        J.eqx = take(kEqWords * maxN);
         J.hg0_rs = maxN * L; J.hg0 = take(R * maxN * L);
         J.hgA_rs = N2 * L; J.hgA = take(R * N2 * L);
         J.hgB_rs = N4 * L; J.hgB = take(R * N4 * L);
