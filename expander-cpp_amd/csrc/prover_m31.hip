@@ -853,51 +853,6 @@ __device__ __forceinline__ void block_reduce_vals(uint32_t (&acc)[NV][V], uint32
     }
 }
 
-// Rounds k and k+1 of one phase for every repetition (blockIdx.y = rep), on many blocks
-// (pair_item).  Each block leaves its partial sums; k_pair_fin adds them and runs the two
-// transcript steps.  The launch boundary between the two orders the partials, so no
-// device-scope fence is needed (on a multi-XCD part a release fence writes back the whole
-// L2, once per block).
-template <int L, bool PH2, int MODE>
-__global__ void __launch_bounds__(kThreads) k_pair(RoundParams p)
-{
-    constexpr uint32_t LG = L / 4;
-    const uint2 rb_ = pair_block(blockIdx.x, p.R);
-    const uint32_t rep = rb_.x, bx = rb_.y, nb = gridDim.x / p.R, g = threadIdx.x % LG;
-    const uint32_t *fs = p.fsrc + rep * p.fsrc_rs;
-    const uint32_t *hs = p.hsrc + rep * p.hsrc_rs;
-    uint32_t *fd = MODE ? p.fdst + rep * p.fdst_rs : nullptr;
-    uint32_t *hd = MODE ? p.hdst + rep * p.hdst_rs : nullptr;
-    const uint32_t ra = MODE ? p.rprev[(uint64_t)rep * kChalStride] : 0u;
-    const uint32_t rb = MODE ? p.rprev[(uint64_t)rep * kChalStride + 1] : 0u;
-    uint32_t acc[kPairVals][4] = {};
-    const uint64_t total = (uint64_t)p.items * LG;
-    if constexpr (MODE == 0) {
-        // the first pair forms its products from rows that need no fold, so the multiplies
-        // dominate: 64-bit sums, folded every third item (2^34 + 3 * 2^62 < 2^64)
-        uint64_t a64[kPairVals][4] = {};
-        uint32_t since = 0;
-        for (uint64_t w = (uint64_t)bx * kThreads + threadIdx.x; w < total; w += (uint64_t)nb * kThreads) {
-            pair_item<L, PH2, MODE, 4, uint64_t>(fs, fd, hs, hd, w / LG, g, ra, rb, a64);
-            if (++since == 3) {
-                since = 0;
-#pragma unroll
-                for (int q = 0; q < kPairVals; q++)
-#pragma unroll
-                    for (int c = 0; c < 4; c++) pair_fold(a64[q][c]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kPairVals; q++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) acc[q][c] = m31_reduce64(a64[q][c]);
-    } else {
-        for (uint64_t w = (uint64_t)bx * kThreads + threadIdx.x; w < total; w += (uint64_t)nb * kThreads)
-            pair_item<L, PH2, MODE>(fs, fd, hs, hd, w / LG, g, ra, rb, acc);
-    }
-    block_reduce_vals<L, kThreads, kPairVals>(acc, p.partials + ((uint64_t)rep * nb + bx) * kPairVals * L, 1);
-}
-
 // Adds the block-major partials [nb][nvals] of k_pair (one repetition: nvals = kPairVals L)
 // or k_dot (nvals = R 3L) into sums[nvals] (LDS), with NT threads.  nvals is a multiple of 4
 // and at most kFinMaxVals; nb is at most kMaxBlocksPerRep.  Each block's partials are one
@@ -1653,6 +1608,22 @@ __device__ __forceinline__ void ls_pair(const RoundParams &p, uint32_t nb, uint3
     block_reduce_vals<L, BS, kPairVals, V>(acc, p.partials + ((uint64_t)rep * nb + bx) * kPairVals * L, 1);
 }
 
+// Rounds k and k+1 of one phase for every repetition, on many blocks (pair_item): each
+// block leaves its partial sums; k_pair_fin adds them and runs the two transcript steps.
+// The launch boundary between the two orders the partials, so no device-scope fence is
+// needed (on a multi-XCD part a release fence writes back the whole L2, once per block).
+// Block size and lanes per thread by phase, measured on the 24 x 2^20 M31x16 proof: phase
+// 1 (f and h both 16-lane rows) with 512 threads of one lane each (59 VGPRs, so 8 waves per
+// SIMD hide the row gathers: the double-fold pass 31.1 -> 27.3 us, 0.54 -> 0.61 of HBM
+// peak); phase 2 (h a scalar per row) with 256 threads of four lanes (one lane per thread
+// was slower there: 1.82 -> 1.85 ms of pairs, 0.68 -> 0.75 ms of first pairs per proof).
+template <int L, bool PH2, int MODE, int BS, int V>
+__global__ void __launch_bounds__(BS) k_pair(RoundParams p)
+{
+    ls_pair<L, BS, PH2, MODE, V>(p, gridDim.x / p.R, blockIdx.x);
+}
+constexpr uint32_t kPairBS1 = 512, kPairV1 = 1, kPairBS2 = 256, kPairV2 = 4;
+
 template <int L, int BS>
 __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ jobs, const LsTask *__restrict__ tasks,
                                                 const uint32_t *__restrict__ start, uint32_t ntasks, uint32_t R,
@@ -2070,17 +2041,20 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
                     rp.hdst = a ? hA : hB; rp.hdst_rs = a ? hA_rs : hB_rs;
                     rp.rprev = chal_l + (phase == 2 ? 32 : 0) + (k - 2);
                 }
-                const uint32_t nbx = (grid_for((uint64_t)rp.items * LG) + 7) & ~7u;  // a multiple of 8
+                const uint32_t pbs = phase == 1 ? kPairBS1 : kPairBS2, pv = phase == 1 ? kPairV1 : kPairV2;
+                const uint64_t pw = (uint64_t)rp.items * (L / pv);  // work items: (item, lane group)
+                const uint32_t nbx = (uint32_t)((std::min<uint64_t>(std::max<uint64_t>(1, (pw + pbs - 1) / pbs),
+                                                                    kMaxBlocksPerRep) + 7) & ~7ull);  // a multiple of 8
                 rp.R = R;
                 dim3 grid(nbx * R);
                 const int cls = phase == 1 ? (j == 0 ? kKPairPh1First : kKPairPh1) : (j == 0 ? kKPairPh2First : kKPairPh2);
-                const dim3 blk(kThreads);
+                const dim3 blk(pbs);
                 if (phase == 1) {
-                    if (j == 0) launch_timed(pf, cls, k_pair<L, false, 0>, grid, blk, st, rp);
-                    else launch_timed(pf, cls, k_pair<L, false, 2>, grid, blk, st, rp);
+                    if (j == 0) launch_timed(pf, cls, k_pair<L, false, 0, kPairBS1, kPairV1>, grid, blk, st, rp);
+                    else launch_timed(pf, cls, k_pair<L, false, 2, kPairBS1, kPairV1>, grid, blk, st, rp);
                 } else {
-                    if (j == 0) launch_timed(pf, cls, k_pair<L, true, 0>, grid, blk, st, rp);
-                    else launch_timed(pf, cls, k_pair<L, true, 2>, grid, blk, st, rp);
+                    if (j == 0) launch_timed(pf, cls, k_pair<L, true, 0, kPairBS2, kPairV2>, grid, blk, st, rp);
+                    else launch_timed(pf, cls, k_pair<L, true, 2, kPairBS2, kPairV2>, grid, blk, st, rp);
                 }
                 timed(pf, kKPairFin, st, [&] {
                     hipLaunchKernelGGL(k_pair_fin<L>, dim3(R), dim3(kFinThreads), 0, st, (const uint32_t *)s.partials,
