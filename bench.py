@@ -221,7 +221,7 @@ def batch_leg(gkrx, fs, count, steps, warmup, plain=True, fixed=None):
            "plain_latency_ms": {"p50": round(percentile(plain_lat, 50), 3),
                                 "p99": round(percentile(plain_lat, 99), 3)} if plain else None,
            "identical_to_plain": same, "verified_sample": ok, "engine": names[engine], "block": block,
-           "lanes_per_thread": lanes, "autotune_device_ms": tuned,
+           "lanes_per_thread": lanes, "autotune_ms": tuned,
            "batched_split_ms": {k: round(float(np.mean([s[k] for s in splits])), 3) for k in splits[0]},
            "roofline": roof}
     # the context and circuits stay resident for the serving leg (the caller closes ctx)

@@ -1099,17 +1099,17 @@ try {
     std::lock_guard<std::mutex> lk(ctx->mu);
     GKRX_HIP_CHECK(hipSetDevice(ctx->device));
     const int kind = ctx->cfg.fs_hash == GKRX_FS_KECCAK256 ? 1 : 0;
-    // placeholder transcript prefixes: the device work of a proof does not depend on them
-    const uint32_t ts[10] = {32, 32, 0, 0, 0, 0, 0, 0, 0, 0};
-    const uint8_t grind[32] = {0};
-    std::vector<std::vector<uint32_t>> rz(n);
+    // each candidate proves the batch as gkrx_prove_batch does: the host transcript prefixes
+    // run on host threads while the batch is enqueued, so a schedule that needs prefixes
+    // early pays for the wait here too (the proof bytes go to scratch buffers)
+    std::vector<BatchPrefix> pre(n);
     std::vector<std::vector<uint8_t>> tails(n);
     std::vector<BatchItem> items(n);
     for (uint32_t i = 0; i < n; i++) {
         gkrx_circuit *c = circuits[i];
-        rz[i].assign((size_t)ctx->R * c->m[c->D - 1], 1u);
+        pre[i].rz1.resize((size_t)ctx->R * c->m[c->D - 1]);
         tails[i].resize(tail_bytes(ctx, c));
-        items[i] = BatchItem{&c->dev, rz[i].data(), ts, grind, tails[i].size(), tails[i].data()};
+        items[i] = BatchItem{&c->dev, pre[i].rz1.data(), pre[i].tstate, pre[i].grind, tails[i].size(), tails[i].data()};
     }
     struct Cand {
         int engine;
@@ -1122,16 +1122,17 @@ try {
             for (uint32_t v : {1u, 2u, 4u}) cands.push_back({e, b, v});
     const int saved_engine = ctx->batch.engine;
     const uint32_t saved_block = ctx->batch.block, saved_lanes = ctx->batch.lanes;
-    double best = 1e300;
-    Cand pick{saved_engine, saved_block, saved_lanes};
+    std::vector<double> times(cands.size());
     for (size_t k = 0; k < cands.size(); k++) {
         ctx->batch.engine = cands[k].engine;
         ctx->batch.block = cands[k].block;
         ctx->batch.lanes = cands[k].lanes;
-        double ms = 0;
-        for (int rep = 0; rep < 2; rep++) {  // the first run allocates and loads code
+        double ms = 1e300;
+        for (int rep = 0; rep < 3; rep++) {  // the first run allocates and loads code; the best of the other two
             const auto t0 = std::chrono::steady_clock::now();
-            int rc = dev_prove_batch(items, ctx->R, kind, ctx->batch, ctx->stream, nullptr);
+            PrefixRunner runner(ctx, circuits, n, kind, pre);
+            int rc = dev_prove_batch(items, ctx->R, kind, ctx->batch, ctx->stream, nullptr, &runner);
+            runner.join();
             if (rc == GKRX_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = GKRX_ERR_DEVICE;
             if (rc) {
                 ctx->batch.engine = saved_engine;
@@ -1139,14 +1140,21 @@ try {
                 ctx->batch.lanes = saved_lanes;
                 return rc;
             }
-            ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (rep > 0) ms = std::min(ms, t);
         }
+        times[k] = ms;
         if (ms_each) ms_each[k] = ms;
-        if (ms < best) {
-            best = ms;
-            pick = cands[k];
-        }
     }
+    // settings within 1% of the fastest are tied (run-to-run noise on a batch is about that
+    // much); the first of them in the fixed order above wins, so the pick repeats
+    const double best = *std::min_element(times.begin(), times.end());
+    Cand pick{saved_engine, saved_block, saved_lanes};
+    for (size_t k = 0; k < cands.size(); k++)
+        if (times[k] <= best * 1.01) {
+            pick = cands[k];
+            break;
+        }
     ctx->batch.engine = pick.engine;
     ctx->batch.block = pick.block;
     ctx->batch.lanes = pick.lanes;

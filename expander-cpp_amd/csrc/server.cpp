@@ -386,7 +386,10 @@ try {
     for (uint32_t i = 0; i < total; i++) {
         auto sl = std::make_unique<Slot>();
         sl->single = ctx->bn254 || i == slots;
-        sl->bs.engine = ctx->batch.engine;  // the context's batch engine (gkrx_ctx_set_batch_engine)
+        // the context's batch engine (gkrx_ctx_set_batch_engine); a slot runs its batch on one
+        // stream, where the grouped lockstep engine would only serialize its groups' separate
+        // launches, so that engine runs as the plain lockstep engine here
+        sl->bs.engine = ctx->batch.engine == kBatchLockstepGrouped ? kBatchLockstep : ctx->batch.engine;
         sl->bs.block = ctx->batch.block;
         sl->bs.lanes = ctx->batch.lanes;
         sl->bs.chain_stream = false;  // slots already overlap; stay within the hardware queues

@@ -340,8 +340,10 @@ class Context:
         return e.value, b.value
 
     def batch_autotune(self, circuits):
-        """Times every batch engine / block size / lanes per thread on these circuits and keeps
-        the fastest.  Returns (engine, block, lanes, {(engine, block, lanes): ms})."""
+        """Times every batch engine / block size / lanes per thread on these circuits (host
+        prefixes included, as prove_batch runs) and keeps the fastest: among settings within 1%
+        of the fastest, the first in the order of the returned dict.  Returns (engine, block,
+        lanes, {(engine, block, lanes): ms})."""
         n = len(circuits)
         hs = (C.c_void_p * n)(*[c._h.value for c in circuits])
         e, b, v = C.c_int(), C.c_uint32(), C.c_uint32()

@@ -156,11 +156,13 @@ int gkrx_ctx_get_graphs(const gkrx_ctx *ctx, int *on);
  * (one 4-, 8- or 16-byte access per thread; L / lanes threads per entry).  Default 4. */
 int gkrx_ctx_set_batch_lanes(gkrx_ctx *ctx, uint32_t lanes);
 int gkrx_ctx_get_batch_lanes(const gkrx_ctx *ctx, uint32_t *lanes);
-/* Autotuner: times the device part of a batch of these circuits (witnesses set) under
- * every engine, block size and lanes per thread, keeps the fastest in the context, and
- * reports it (lanes is 4 for PER_PROOF).  ms_each (optional, 25 entries): PER_PROOF, then
- * LOCKSTEP with block 64, 128, 256, 512, each with lanes 1, 2, 4 (block-major), then
- * LOCKSTEP_GROUPS in the same order. */
+/* Autotuner: times a batched prove of these circuits (witnesses set) as gkrx_prove_batch
+ * runs it -- the host transcript prefixes on host threads while the device works -- under
+ * every engine, block size and lanes per thread (best of two timed runs after a warmup),
+ * keeps the fastest in the context -- among settings within 1% of the fastest, the first
+ * in the order below -- and reports it (lanes is 4 for PER_PROOF).  ms_each (optional, 25
+ * entries): PER_PROOF, then LOCKSTEP with block 64, 128, 256, 512, each with lanes 1, 2, 4
+ * (block-major), then LOCKSTEP_GROUPS in the same order. */
 int gkrx_batch_autotune(gkrx_ctx *ctx, uint32_t n, gkrx_circuit *const *circuits, int *engine, uint32_t *block,
                         uint32_t *lanes, double *ms_each);
 /* Verifier::verify (LinearGKR.hpp:104-149), on the host.  Returns 1 = accepted,

@@ -136,7 +136,10 @@ def test_autotune_picks_an_engine_and_keeps_results(ctx, mixed_plain):
     try:
         engine, block, lanes, ms = ctx.batch_autotune(circs)
         assert (engine, block) in ENGINES and ctx.batch_engine() == (engine, block) and ctx.batch_lanes() == lanes
-        assert len(ms) == 25 and all(v > 0 for v in ms.values()) and ms[(engine, block, lanes)] == min(ms.values())
+        assert len(ms) == 25 and all(v > 0 for v in ms.values())
+        # the first setting (in the order of the dict) within 1% of the fastest
+        tied = [k for k, v in ms.items() if v <= min(ms.values()) * 1.01]
+        assert (engine, block, lanes) == tied[0]
         batch = gkrx.Prover(ctx).prove_batch(circs)
     finally:
         ctx.set_batch_engine(*saved)
