@@ -558,7 +558,8 @@ __device__ __forceinline__ void round_item_dbl(const uint32_t *__restrict__ fs, 
 // Once r_k is drawn the transcript kernel evaluates them: the same field values as a pass
 // over the folded round-(k+1) tables, which are never stored.  T is the layer input (MODE 0:
 // rounds 0 and 1) or is made here by folding rows 16i .. 16i+15 of table k - 2 by
-// ra = r_{k-2} and then rb = r_{k-1} (MODE 2), and stored for the next pair.  Sums are
+// ra = r_{k-2} and then rb = r_{k-1} (MODE 2), or rows 8i .. 8i+7 of table k - 1 by ra =
+// r_{k-1} (MODE 1, the small-table rounds), and stored for the next pass.  Sums are
 // canonical (u32 accumulators keep the register count, and with it the occupancy, up).
 constexpr uint32_t kPairVals = 9;
 
@@ -588,6 +589,27 @@ __device__ __forceinline__ void pair_item(const uint32_t *__restrict__ fs, uint3
 #pragma unroll
             for (int r = 0; r < 4; r++) h[r] = ldv<V>(hs + (4 * i + r) * L + V * g);
         }
+    } else if (MODE == 1) {
+        // table k from rows 8i .. 8i+7 of table k - 1, folded once by ra
+        uint4 b[2];
+        if (PH2) {
+            b[0] = *reinterpret_cast<const uint4 *>(hs + 8 * i);
+            b[1] = *reinterpret_cast<const uint4 *>(hs + 8 * i + 4);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t *src = fs + (8 * i + 2 * r) * L + V * g;
+            f[r] = foldv(ldv<V>(src), ldv<V>(src + L), ra);
+            stv(fd + (4 * i + r) * L + V * g, f[r]);
+            if (PH2) {
+                sv[r] = (r & 1) ? fold1(b[r >> 1].z, b[r >> 1].w, ra) : fold1(b[r >> 1].x, b[r >> 1].y, ra);
+            } else {
+                const uint32_t *hsrc = hs + (8 * i + 2 * r) * L + V * g;
+                h[r] = foldv(ldv<V>(hsrc), ldv<V>(hsrc + L), ra);
+                stv(hd + (4 * i + r) * L + V * g, h[r]);
+            }
+        }
+        if (PH2 && g == 0) *reinterpret_cast<uint4 *>(hd + 4 * i) = make_uint4(sv[0], sv[1], sv[2], sv[3]);
     } else {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -1112,87 +1134,132 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(TailParams p)
 // len0 and hs0 (the proof layout is fixed by the circuit shape), so this kernel reads no
 // transcript state; the workgroup of the last rep writes the final state.
 constexpr uint32_t kRepThreads = 512;
+constexpr int kTailV = 2;  // lanes per thread in the small-table passes
 struct TailRepParams {
     TailParams t;
     uint32_t kend;
     uint32_t len0, hs0;  // transcript length and hash start (bytes of the tail) at round k0
 };
 
-// The body of k_tail_rep for repetition `rep`; blockDim is NT (a multiple of 64, >= 3L).
-// One pass per round: with tables this small a round pair's pass (double folds, nine
-// products, a 9L-value reduction) costs about two single passes, measured on the 24 x 2^20
-// proof (20.63 against 20.31 ms of device time), so the rounds here run one at a time.
-template <int L, bool PH2, uint32_t NT = kRepThreads>
+// Adds the nine product sums of a wave's threads (V lanes per thread; the threads sharing
+// t % (L / V) hold the same lanes) into out[q L + lane] (canonical).
+template <int L, int V>
+__device__ __forceinline__ void wave_reduce_pairs(uint32_t (&acc)[kPairVals][V], uint32_t *out)
+{
+    constexpr uint32_t LG = L / V;
+#pragma unroll
+    for (uint32_t off = LG; off < 64; off <<= 1)
+#pragma unroll
+        for (int q = 0; q < (int)kPairVals; q++)
+#pragma unroll
+            for (int c = 0; c < V; c++) acc[q][c] = m31_add(acc[q][c], __shfl_xor(acc[q][c], off, 64));
+    const uint32_t lane = threadIdx.x & 63;
+    if (lane < LG) {
+#pragma unroll
+        for (int q = 0; q < (int)kPairVals; q++)
+#pragma unroll
+            for (int c = 0; c < V; c++) out[q * L + V * lane + c] = acc[q][c];
+    }
+}
+
+// The body of k_tail_rep for repetition `rep`; blockDim is NT (a multiple of 64, >= 128);
+// k0 < kend <= n - 1.
+// Each round's sums come from the nine products of a pass over the table's groups of four
+// rows (the round-pair products above): round k's sums are their first combination, round
+// k+1's a quadratic in r_k.  So the pass that makes table k+1 and its products (for round
+// k+2) needs only r_k and runs on waves 1 .. NT/64 - 1 while wave 0 hashes round k+1's
+// message, whose sums it evaluates from the previous pass at r_k; a round costs one hash
+// and one barrier instead of a pass, a reduction and a hash in turn.  Before round k0 the
+// whole workgroup makes table k0 (from the layer input, or folding table k0 - 1 once or
+// table k0 - 2 twice) and its products.  The product sums of a pass live per wave in
+// red[buffer][wave], two buffers used in turn (wave 0 reads one while the pass fills the
+// other).  Table k (k >= 1) is stored in the A buffers for odd k, B for even k, where the
+// last round (k_tail) finds table kend - 1.
+template <int L, bool PH2, uint32_t NT = kRepThreads, int V = kTailV>
 __device__ __forceinline__ void tail_rep_body(const TailRepParams &q, uint32_t rep)
 {
-    constexpr uint32_t LG = L / 4, Q = 3 * L, W = NT / 64;
-    static_assert(NT % 64 == 0 && NT >= Q, "tail workgroup shape");
+    constexpr uint32_t LG = L / V, Q = 3 * L, NV = kPairVals * L, W = NT / 64;
+    static_assert(NT % 64 == 0 && NT >= 128, "tail workgroup shape");
     const TailParams &p = q.t;
-    __shared__ uint32_t red[W][3 * L];
-    __shared__ uint32_t sums[Q], rr, hb[L + Q];
-    const uint32_t R = p.ts.R;
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63, g = t % LG;
+    __shared__ uint32_t red[2][W][NV];
+    __shared__ uint32_t sums[Q], rr[2], hb[L + Q];
+    const uint32_t R = p.ts.R, k0 = p.k0;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     uint32_t *chal = p.ts.chal + rep * kChalStride + (PH2 ? 32u : 0u);
-    if (t == 0) rr = p.k0 > 0 ? chal[p.k0 - 1] : 0u;
+    const uint32_t *f0 = p.f0 + rep * p.f0_rs, *h0 = p.h0 + rep * p.h0_rs;
+    auto tab_f = [&](uint32_t kk) { return ((kk & 1) ? p.fA : p.fB) + rep * ((kk & 1) ? p.fA_rs : p.fB_rs); };
+    auto tab_h = [&](uint32_t kk) { return ((kk & 1) ? p.hA : p.hB) + rep * ((kk & 1) ? p.hA_rs : p.hB_rs); };
+    {
+        uint32_t acc[kPairVals][V] = {};
+        const uint64_t items = (1ull << (p.n - k0 - 2)) * LG;
+        if (k0 == 0) {
+            for (uint64_t w = t; w < items; w += NT)
+                pair_item<L, PH2, 0, V>(f0, nullptr, h0, nullptr, w / LG, (uint32_t)(w % LG), 0u, 0u, acc);
+        } else if (p.dbl) {
+            const uint32_t ra = chal[k0 - 2], rb = chal[k0 - 1];
+            for (uint64_t w = t; w < items; w += NT)
+                pair_item<L, PH2, 2, V>(f0, tab_f(k0), h0, tab_h(k0), w / LG, (uint32_t)(w % LG), ra, rb, acc);
+        } else {
+            const uint32_t ra = chal[k0 - 1];
+            for (uint64_t w = t; w < items; w += NT)
+                pair_item<L, PH2, 1, V>(f0, tab_f(k0), h0, tab_h(k0), w / LG, (uint32_t)(w % LG), ra, 0u, acc);
+        }
+        wave_reduce_pairs<L, V>(acc, red[0][wave]);
+    }
     __syncthreads();
     uint32_t d[8] = {};
-    for (uint32_t k = p.k0; k < q.kend; k++) {
-        const uint64_t items = 1ull << (p.n - k - 1);
-        const bool first = (k == p.k0) || (k == 1);
-        const uint32_t *fs = (first ? p.f0 : ((k - 1) & 1 ? p.fA : p.fB)) + rep * (first ? p.f0_rs : ((k - 1) & 1 ? p.fA_rs : p.fB_rs));
-        const uint32_t *hs = (first ? p.h0 : ((k - 1) & 1 ? p.hA : p.hB)) + rep * (first ? p.h0_rs : ((k - 1) & 1 ? p.hA_rs : p.hB_rs));
-        uint32_t *fd = ((k & 1) ? p.fA : p.fB) + rep * ((k & 1) ? p.fA_rs : p.fB_rs);
-        uint32_t *hd = ((k & 1) ? p.hA : p.hB) + rep * ((k & 1) ? p.hA_rs : p.hB_rs);
-        const uint32_t r = rr;
-        uint64_t acc[3][4] = {};
-        if (k == p.k0 && p.dbl) {
-            const uint32_t ra = chal[k - 2];
-            for (uint64_t w = t; w < items * LG; w += NT) round_item_dbl<L, PH2>(fs, fd, hs, hd, w / LG, g, ra, r, acc);
-        } else {
-            for (uint64_t w = t; w < items * LG; w += NT) {
-                if (k == 0) round_item<L, PH2, false>(fs, nullptr, hs, nullptr, w / LG, g, 0u, acc);
-                else round_item<L, PH2, true>(fs, fd, hs, hd, w / LG, g, r, acc);
-            }
-        }
-        // block reduce: lanes sharing t % LG within the wave, then the W waves
-        uint32_t v[3][4];
-#pragma unroll
-        for (int pk = 0; pk < 3; pk++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) v[pk][c] = m31_reduce64(acc[pk][c]);
-#pragma unroll
-        for (uint32_t off = LG; off < 64; off <<= 1)
-#pragma unroll
-            for (int pk = 0; pk < 3; pk++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) v[pk][c] = m31_add(v[pk][c], __shfl_xor(v[pk][c], off, 64));
-        if (lane < LG) {
-#pragma unroll
-            for (int pk = 0; pk < 3; pk++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) red[wave][pk * L + 4 * lane + c] = v[pk][c];
-        }
-        __syncthreads();
-        if (t < Q) {
-            uint32_t x = 0;
-#pragma unroll
-            for (uint32_t w = 0; w < W; w++) x = m31_add(x, red[w][t]);
-            sums[t] = x;
-        }
-        __syncthreads();
+    for (uint32_t k = k0; k < q.kend; k++) {
+        const uint32_t rprev = rr[(k - k0 + 1) & 1];  // r_{k-1}, for k > k0
         if (wave == 0) {
-            const uint32_t base = q.len0 + (k - p.k0) * 4 * Q * R + 4 * Q * rep;
-            const uint32_t start = (rep == 0 && k == p.k0) ? q.hs0 : base;
+            // round k's sums from the products of table k0 (k = k0) or table k - 1 at r_{k-1}
+            const uint32_t (*P)[NV] = red[(k - k0 + 1) & 1];
+            if (k == k0) P = red[0];
+            for (uint32_t x = lane; x < Q; x += 64) {
+                const uint32_t sidx = x / L, c = x % L;
+                uint32_t v[6];
+#pragma unroll
+                for (int j = 0; j < 6; j++) v[j] = 0u;
+                const uint32_t q0 = k == k0 ? 0u : 3 * sidx;  // first: P0..P5; later: P(3s)..P(3s+2)
+                const int nq = k == k0 ? 6 : 3;
+                for (uint32_t w = 0; w < W; w++)
+#pragma unroll
+                    for (int j = 0; j < 6; j++)
+                        if (j < nq) v[j] = m31_add(v[j], P[w][(q0 + j) * L + c]);
+                uint32_t sv;
+                if (k == k0) {
+                    const uint32_t u = m31_add(v[0], v[3]), vv = m31_add(v[1], v[4]), ww = m31_add(v[2], v[5]);
+                    sv = sidx == 0 ? u : sidx == 1 ? vv : m31_sub(m31_add(m31_add(u, u), m31_add(vv, vv)), ww);
+                } else {
+                    const uint32_t c1 = m31_sub(v[1], m31_add(v[0], v[2]));
+                    sv = m31_add(v[0], m31_mul(rprev, m31_add(c1, m31_mul(rprev, v[2]))));
+                }
+                sums[x] = sv;
+            }
+            // wave 0 takes no part in the pass: its slot of the buffer the pass fills is zero
+            if (k > k0)
+                for (uint32_t x = lane; x < NV; x += 64) red[(k - k0) & 1][0][x] = 0u;
+            __threadfence_block();
+            const uint32_t base = q.len0 + (k - k0) * 4 * Q * R + 4 * Q * rep;
+            const uint32_t start = (rep == 0 && k == k0) ? q.hs0 : base;
             const uint32_t c = rep_step<L>(sums, p.ts, PH2 ? 2 : 1, rep, base, start, hb, d);
             if (lane == 0) {
                 chal[k] = c;
-                rr = c;
+                rr[(k - k0) & 1] = c;  // the other slot may still be read by a pass wave
             }
+        } else if (k > k0) {
+            // table k from table k - 1 by r_{k-1}, and its products for round k + 1
+            const uint32_t *fs = k == 1 ? f0 : tab_f(k - 1);
+            const uint32_t *hs = k == 1 ? h0 : tab_h(k - 1);
+            uint32_t acc[kPairVals][V] = {};
+            const uint64_t items = (1ull << (p.n - k - 2)) * LG;
+            for (uint64_t w = t - 64; w < items; w += NT - 64)
+                pair_item<L, PH2, 1, V>(fs, tab_f(k), hs, tab_h(k), w / LG, (uint32_t)(w % LG), rprev, 0u, acc);
+            if (k + 1 < q.kend) wave_reduce_pairs<L, V>(acc, red[(k - k0) & 1][wave]);
         }
         __syncthreads();
     }
     if (rep == R - 1 && t == 0) {
-        const uint32_t len = q.len0 + (q.kend - p.k0) * 4 * Q * R;
+        const uint32_t len = q.len0 + (q.kend - k0) * 4 * Q * R;
         p.ts.tstate[0] = len;
         p.ts.tstate[1] = len;
         for (int i = 0; i < 8; i++) p.ts.tstate[2 + i] = d[i];
