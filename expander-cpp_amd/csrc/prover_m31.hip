@@ -283,64 +283,122 @@ template <int L, int BS>
 constexpr uint32_t gx_rows() { return tile_lanes(BS) / L; }
 constexpr uint32_t kGxAccWords = 3 * 1024;
 
+// The tiles tfirst, tfirst + tstep, ... (< tiles) of g_x, software-pipelined: while a tile's
+// contributions are added and its rows written, the gathers of the next tile, the CSR
+// entries of the one after and the entry range of the third are in flight, so a block
+// waits for one memory latency per tile instead of three dependent ones (row range, then
+// entries, then the v rows and eqZ triples they name).  The first kGxPipe items of a tile
+// per thread go through the pipeline; a tile with more work adds the rest directly.
+constexpr int kGxPipe = 2;
+constexpr uint32_t kGxBlocks = 1536;
 template <int L, int BS>
-__device__ __forceinline__ void gx_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ hg, uint64_t hg_rs,
-                        const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz,
-                        const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N, uint32_t R,
-                        uint32_t x0)
+__device__ __forceinline__ void gx_tiles(unsigned long long *__restrict__ acc, uint32_t *__restrict__ hg, uint64_t hg_rs,
+                                         const uint32_t *__restrict__ v, const uint32_t *__restrict__ eqz,
+                                         const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N,
+                                         uint32_t R, uint32_t tfirst, uint32_t tstep, uint32_t tiles)
 {
     constexpr uint32_t LG = L / 4, ROWS = gx_rows<L, BS>();
+    constexpr int NI = kGxPipe;
     const uint32_t t = threadIdx.x;
-    const uint32_t rows = N - x0 < ROWS ? N - x0 : ROWS;
-    const uint32_t e0 = ptr[x0], e1 = ptr[x0 + rows];
-    const uint64_t work = (uint64_t)(e1 - e0) * LG;
-    // one work item: entry e's R contributions to lanes 4g .. 4g+3 of its row
-    struct Item {
-        uint4 en, vb;  // {in1 or kAddMark, z, coef, x}; v[in1] (lanes), or ones for an add gate
-        uint32_t c[3];  // coef * eqZ_j[z]
+    // The loads below are issued without branches (an item out of range loads from the
+    // start of v or eqZ, always valid, and is not used), so that the compiler can count
+    // them and let them stay in flight across the tile's work.
+    auto range = [&](uint32_t tile) {  // entry range of a tile; tiles past the end read row 0
+        const uint32_t x0 = tile < tiles ? tile * ROWS : 0u, x1 = N - x0 < ROWS ? N : x0 + ROWS;
+        return make_uint2(ptr[x0], ptr[x1]);
     };
-    auto load = [&](uint64_t w) {
-        Item it;
-        const uint64_t e = e0 + w / LG;
-        const uint32_t g = (uint32_t)(w % LG);
-        it.en = ld4(ent + kGxWords * e);
-        it.vb = it.en.x != kAddMark ? ld4(v + (uint64_t)it.en.x * L + 4 * g) : make_uint4(1u, 1u, 1u, 1u);
-        const Eq3 ez = ld3(eqz + kEqWords * (uint64_t)it.en.y);  // eqZ_j[z] of every rep
-        it.c[0] = m31_mul(it.en.z, ez.r0);
-        it.c[1] = R > 1 ? m31_mul(it.en.z, ez.r1) : 0u;
-        it.c[2] = R > 2 ? m31_mul(it.en.z, ez.r2) : 0u;
-        return it;
+    auto count = [&](uint32_t tile, uint2 r) { return tile < tiles ? (uint64_t)(r.y - r.x) * LG : 0ull; };
+    auto entries = [&](uint64_t cnt, uint2 r, uint4 (&en)[NI]) {
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const uint64_t w = t + (uint64_t)i * BS;
+            en[i] = ld4(w < cnt ? ent + kGxWords * (r.x + w / LG) : v);
+        }
     };
-    auto add = [&](const Item &it, uint64_t w) {
+    auto gathers = [&](uint64_t cnt, const uint4 (&en)[NI], uint4 (&vb)[NI], Eq3 (&ez)[NI]) {
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const uint64_t w = t + (uint64_t)i * BS;
+            const bool ok = w < cnt;
+            vb[i] = ld4(ok && en[i].x != kAddMark ? v + (uint64_t)en[i].x * L + 4 * (uint32_t)(w % LG) : v);
+            ez[i] = ld3(ok ? eqz + kEqWords * (uint64_t)en[i].y : eqz);
+        }
+    };
+    auto add = [&](uint32_t x0, const uint4 &en, uint4 vb, const Eq3 &ez, uint64_t w) {
         const uint32_t g = (uint32_t)(w % LG);
-        unsigned long long *a = acc + (uint64_t)(it.en.w - x0) * L + 4 * g;
+        if (en.x == kAddMark) vb = make_uint4(1u, 1u, 1u, 1u);
+        const uint32_t c[3] = {m31_mul(en.z, ez.r0), R > 1 ? m31_mul(en.z, ez.r1) : 0u, R > 2 ? m31_mul(en.z, ez.r2) : 0u};
+        unsigned long long *a = acc + (uint64_t)(en.w - x0) * L + 4 * g;
 #pragma unroll
         for (uint32_t j = 0; j < 3; j++)
             if (j < R) {
-                // products below 2^32, reduced once when the row is written
                 unsigned long long *aj = a + j * ROWS * L;
-                atomicAdd(aj, (unsigned long long)m31_mul_wide(it.vb.x, it.c[j]));
-                atomicAdd(aj + 1, (unsigned long long)m31_mul_wide(it.vb.y, it.c[j]));
-                atomicAdd(aj + 2, (unsigned long long)m31_mul_wide(it.vb.z, it.c[j]));
-                atomicAdd(aj + 3, (unsigned long long)m31_mul_wide(it.vb.w, it.c[j]));
+                atomicAdd(aj, (unsigned long long)m31_mul_wide(vb.x, c[j]));
+                atomicAdd(aj + 1, (unsigned long long)m31_mul_wide(vb.y, c[j]));
+                atomicAdd(aj + 2, (unsigned long long)m31_mul_wide(vb.z, c[j]));
+                atomicAdd(aj + 3, (unsigned long long)m31_mul_wide(vb.w, c[j]));
             }
     };
-    // the thread's first item is gathered before the accumulators are cleared: its chain of
-    // dependent loads (entry, then the v row and eqZ) overlaps the clear and the barrier
-    Item first{};
-    if (t < work) first = load(t);
-    for (uint32_t i = t; i < R * ROWS * L; i += BS) acc[i] = 0;
-    __syncthreads();
-    if (t < work) add(first, t);
-    for (uint64_t w = t + BS; w < work; w += BS) add(load(w), w);
-    __syncthreads();
-    for (uint32_t i = t; i < R * rows * LG; i += BS) {
-        const uint32_t j = i / (rows * LG), rem = i % (rows * LG), r = rem / LG, g = rem % LG;
-        const unsigned long long *aj = acc + (j * ROWS + r) * L + 4 * g;
-        st4(hg + j * hg_rs + (uint64_t)(x0 + r) * L + 4 * g,
-            make_uint4(m31_reduce64(aj[0]), m31_reduce64(aj[1]), m31_reduce64(aj[2]), m31_reduce64(aj[3])));
+    // prologue: the ranges of the first three tiles, the entries of the first two, the
+    // gathers of the first
+    uint32_t tg = tfirst, te = tfirst + tstep, tp = te + tstep;
+    uint2 rg = range(tg), re = range(te), rp = range(tp);
+    uint4 eng[NI], ene[NI], vbg[NI];
+    Eq3 ezg[NI];
+    entries(count(tg, rg), rg, eng);
+    entries(count(te, re), re, ene);
+    gathers(count(tg, rg), eng, vbg, ezg);
+    while (tg < tiles) {
+        const uint32_t x0 = tg * ROWS;
+        const uint32_t rows = N - x0 < ROWS ? N - x0 : ROWS;
+#pragma unroll
+        for (uint32_t k = 0; k < (3 * ROWS * L + BS - 1) / BS; k++)
+            if (t + k * BS < R * ROWS * L) acc[t + k * BS] = 0;
+        __syncthreads();
+        const uint64_t work = count(tg, rg);
+        // items past the pipelined ones (a tile with many entries), loaded here directly:
+        // before the next tiles' loads are issued, so that waiting for these does not wait
+        // for those
+        for (uint64_t w = t + (uint64_t)NI * BS; w < work; w += BS) {
+            const uint4 en = ld4(ent + kGxWords * (rg.x + w / LG));
+            const uint4 vb = en.x != kAddMark ? ld4(v + (uint64_t)en.x * L + 4 * (uint32_t)(w % LG)) : make_uint4(1u, 1u, 1u, 1u);
+            add(x0, en, vb, ld3(eqz + kEqWords * (uint64_t)en.y), w);
+        }
+        // the next tiles' loads, in flight through this tile's additions and writes
+        const uint32_t tq = tp + tstep;
+        uint4 vbn[NI], enp[NI];
+        Eq3 ezn[NI];
+        gathers(count(te, re), ene, vbn, ezn);
+        entries(count(tp, rp), rp, enp);
+        const uint2 rq = range(tq);
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const uint64_t w = t + (uint64_t)i * BS;
+            if (w < work) add(x0, eng[i], vbg[i], ezg[i], w);
+        }
+        __syncthreads();
+        // straight-line code (no loop), so that the loads above stay in flight through it
+#pragma unroll
+        for (uint32_t k = 0; k < (3 * ROWS * LG + BS - 1) / BS; k++) {
+            const uint32_t i = t + k * BS;
+            if (i < R * rows * LG) {
+                const uint32_t j = i / (rows * LG), rem = i % (rows * LG), r = rem / LG, g = rem % LG;
+                const unsigned long long *aj = acc + (j * ROWS + r) * L + 4 * g;
+                st4(hg + j * hg_rs + (uint64_t)(x0 + r) * L + 4 * g,
+                    make_uint4(m31_reduce64(aj[0]), m31_reduce64(aj[1]), m31_reduce64(aj[2]), m31_reduce64(aj[3])));
+            }
+        }
+        __syncthreads();
+        tg = te; te = tp; tp = tq;
+        rg = re; re = rp; rp = rq;
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            eng[i] = ene[i];
+            ene[i] = enp[i];
+            vbg[i] = vbn[i];
+            ezg[i] = ezn[i];
+        }
     }
-    __syncthreads();
 }
 
 template <int L>
@@ -351,57 +409,113 @@ __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint
 {
     __shared__ unsigned long long acc[kGxAccWords];
     constexpr uint32_t ROWS = gx_rows<L, kThreads>();
-    const uint64_t tiles = (N + ROWS - 1) / ROWS;
-    for (uint64_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x)
-        gx_tile<L, kThreads>(acc, hg, hg_rs, v, eqz, ptr, ent, N, R, (uint32_t)(tile * ROWS));
+    const uint32_t tiles = (N + ROWS - 1) / ROWS;
+    gx_tiles<L, kThreads>(acc, hg, hg_rs, v, eqz, ptr, ent, N, R, xcd_tile(blockIdx.x, gridDim.x), gridDim.x, tiles);
 }
 
 // s_j[y] = sum over the mul gates with second input y of eqZ_j[z] * eqX_j[x] * c, for every
 // rep j at once (phase-2 hg_j = vx_j * s_j); a tile is tile_lanes(BS) rows.
 constexpr uint32_t kHyAccWords = 3 * 1024;
 
+// The tiles tfirst, tfirst + tstep, ... (< tiles) of h_y, pipelined like gx_tiles: while a
+// tile's products are added and its rows written, the eq gathers of the next tile, the CSR
+// entries of the one after and the entry range of the third are in flight.
+constexpr int kHyPipe = 2;
 template <int BS>
-__device__ __forceinline__ void hy_tile(unsigned long long *__restrict__ acc, uint32_t *__restrict__ s, uint64_t s_rs,
-                        const uint32_t *__restrict__ eqz, const uint32_t *__restrict__ eqx,
-                        const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N, uint32_t R,
-                        uint32_t y0)
+__device__ __forceinline__ void hy_tiles(unsigned long long *__restrict__ acc, uint32_t *__restrict__ s, uint64_t s_rs,
+                                         const uint32_t *__restrict__ eqz, const uint32_t *__restrict__ eqx,
+                                         const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N,
+                                         uint32_t R, uint32_t tfirst, uint32_t tstep, uint32_t tiles)
 {
-    constexpr uint32_t kHyRows = tile_lanes(BS);
+    constexpr uint32_t ROWS = tile_lanes(BS);
+    constexpr int NI = kHyPipe;
     const uint32_t t = threadIdx.x;
-    const uint32_t rows = N - y0 < kHyRows ? N - y0 : kHyRows;
-    const uint32_t e0 = ptr[y0], e1 = ptr[y0 + rows];
-    // entry e's R products (coef kept apart: multiplied when added)
-    auto load = [&](uint32_t w, uint4 &en, uint32_t (&q)[3]) {
-        en = ld4(ent + kHyWords * (uint64_t)(e0 + w));  // {x, z, coef, y}
-        const Eq3 a = ld3(eqz + kEqWords * (uint64_t)en.y), b = ld3(eqx + kEqWords * (uint64_t)en.x);  // every rep
-        q[0] = m31_mul(a.r0, b.r0);
-        q[1] = R > 1 ? m31_mul(a.r1, b.r1) : 0u;
-        q[2] = R > 2 ? m31_mul(a.r2, b.r2) : 0u;
+    // branch-free loads: an item out of range reads the start of eqX (always valid) instead
+    auto range = [&](uint32_t tile) {
+        const uint32_t y0 = tile < tiles ? tile * ROWS : 0u, y1 = N - y0 < ROWS ? N : y0 + ROWS;
+        return make_uint2(ptr[y0], ptr[y1]);
     };
-    auto add = [&](const uint4 &en, const uint32_t (&q)[3]) {
+    auto count = [&](uint32_t tile, uint2 r) { return tile < tiles ? r.y - r.x : 0u; };
+    auto entries = [&](uint32_t cnt, uint2 r, uint4 (&en)[NI]) {  // {x, z, coef, y}
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const uint32_t w = t + i * BS;
+            en[i] = ld4(w < cnt ? ent + kHyWords * (uint64_t)(r.x + w) : eqx);
+        }
+    };
+    auto gathers = [&](uint32_t cnt, const uint4 (&en)[NI], Eq3 (&a)[NI], Eq3 (&b)[NI]) {
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            const bool ok = t + i * BS < cnt;
+            a[i] = ld3(ok ? eqz + kEqWords * (uint64_t)en[i].y : eqz);
+            b[i] = ld3(ok ? eqx + kEqWords * (uint64_t)en[i].x : eqx);
+        }
+    };
+    auto add = [&](uint32_t y0, const uint4 &en, const Eq3 &a, const Eq3 &b) {
+        const uint32_t q[3] = {m31_mul(a.r0, b.r0), R > 1 ? m31_mul(a.r1, b.r1) : 0u, R > 2 ? m31_mul(a.r2, b.r2) : 0u};
 #pragma unroll
         for (uint32_t j = 0; j < 3; j++)
-            if (j < R) atomicAdd(acc + j * kHyRows + (en.w - y0), (unsigned long long)m31_mul_wide(q[j], en.z));
+            if (j < R) atomicAdd(acc + j * ROWS + (en.w - y0), (unsigned long long)m31_mul_wide(q[j], en.z));
     };
-    // the first entry's gathers overlap the clear of the accumulators and the barrier
-    uint4 en0 = make_uint4(0u, 0u, 0u, 0u);
-    uint32_t q0[3] = {};
-    if (t < e1 - e0) load(t, en0, q0);
-    for (uint32_t i = t; i < R * kHyRows; i += BS) acc[i] = 0;
-    __syncthreads();
-    if (t < e1 - e0) add(en0, q0);
-    for (uint32_t w = t + BS; w < e1 - e0; w += BS) {
-        uint4 en;
-        uint32_t q[3];
-        load(w, en, q);
-        add(en, q);
+    uint32_t tg = tfirst, te = tfirst + tstep, tp = te + tstep;
+    uint2 rg = range(tg), re = range(te), rp = range(tp);
+    uint4 eng[NI], ene[NI];
+    Eq3 ag[NI], bg[NI];
+    entries(count(tg, rg), rg, eng);
+    entries(count(te, re), re, ene);
+    gathers(count(tg, rg), eng, ag, bg);
+    while (tg < tiles) {
+        const uint32_t y0 = tg * ROWS;
+        const uint32_t rows = N - y0 < ROWS ? N - y0 : ROWS;
+#pragma unroll
+        for (uint32_t k = 0; k < (3 * ROWS + BS - 1) / BS; k++)
+            if (t + k * BS < R * ROWS) acc[t + k * BS] = 0;
+        __syncthreads();
+        const uint32_t work = count(tg, rg);
+        // items past the pipelined ones, before the next tiles' loads are issued
+        for (uint32_t w = t + NI * BS; w < work; w += BS) {
+            const uint4 en = ld4(ent + kHyWords * (uint64_t)(rg.x + w));
+            add(y0, en, ld3(eqz + kEqWords * (uint64_t)en.y), ld3(eqx + kEqWords * (uint64_t)en.x));
+        }
+        const uint32_t tq = tp + tstep;
+        uint4 enp[NI];
+        Eq3 an[NI], bn[NI];
+        gathers(count(te, re), ene, an, bn);
+        entries(count(tp, rp), rp, enp);
+        const uint2 rq = range(tq);
+#pragma unroll
+        for (int i = 0; i < NI; i++)
+            if (t + i * BS < work) add(y0, eng[i], ag[i], bg[i]);
+        __syncthreads();
+        // four rows per store where the tile is whole in fours (rows is a power of two or 1024)
+        if ((rows & 3) == 0) {
+#pragma unroll
+            for (uint32_t k = 0; k < (3 * ROWS / 4 + BS - 1) / BS; k++) {
+                const uint32_t i = t + k * BS;
+                if (i < R * rows / 4) {
+                    const uint32_t j = i / (rows / 4), r = 4 * (i % (rows / 4));
+                    const unsigned long long *aj = acc + j * ROWS + r;
+                    st4(s + j * s_rs + y0 + r, make_uint4(m31_reduce64(aj[0]), m31_reduce64(aj[1]),
+                                                          m31_reduce64(aj[2]), m31_reduce64(aj[3])));
+                }
+            }
+        } else {
+            for (uint32_t i = t; i < R * rows; i += BS) {
+                const uint32_t j = i / rows, r = i % rows;
+                s[j * s_rs + y0 + r] = m31_reduce64(acc[j * ROWS + r]);
+            }
+        }
+        __syncthreads();
+        tg = te; te = tp; tp = tq;
+        rg = re; re = rp; rp = rq;
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            eng[i] = ene[i];
+            ene[i] = enp[i];
+            ag[i] = an[i];
+            bg[i] = bn[i];
+        }
     }
-    __syncthreads();
-    for (uint32_t i = t; i < R * rows; i += BS) {
-        const uint32_t j = i / rows, r = i % rows;
-        s[j * s_rs + y0 + r] = m31_reduce64(acc[j * kHyRows + r]);
-    }
-    __syncthreads();
 }
 
 __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint64_t s_rs,
@@ -412,9 +526,8 @@ __global__ void __launch_bounds__(kThreads) k_hy(uint32_t *__restrict__ s, uint6
 {
     __shared__ unsigned long long acc[kHyAccWords];
     constexpr uint32_t ROWS = tile_lanes(kThreads);
-    const uint64_t tiles = (N + ROWS - 1) / ROWS;
-    for (uint64_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < tiles; tile += gridDim.x)
-        hy_tile<kThreads>(acc, s, s_rs, eqz, eqx, ptr, ent, N, R, (uint32_t)(tile * ROWS));
+    const uint32_t tiles = (N + ROWS - 1) / ROWS;
+    hy_tiles<kThreads>(acc, s, s_rs, eqz, eqx, ptr, ent, N, R, xcd_tile(blockIdx.x, gridDim.x), gridDim.x, tiles);
 }
 
 // ------------------------------------------------------------------ reductions
@@ -1472,8 +1585,8 @@ __device__ void prove_batch_job(const BatchJob &J, uint32_t R, int hash_kind, un
         uint32_t *chal_l = chal_slot(l);
         block_eq(J.eqz, chal_slot(l + 1), 0u, 32u, ab_slot(l + 1), m, R);
         if (l == (int)D - 1) block_claims<L>(J.values + ly.out_off, J.eqz, 1u << m, R, J.claims);
-        for (uint32_t x0 = 0; x0 < (1u << n); x0 += gx_rows<L, kBatchThreads>())
-            gx_tile<L, kBatchThreads>(tacc, J.hg0, J.hg0_rs, V, J.eqz, ly.gx_ptr, ly.gx_ent, 1u << n, R, x0);
+        gx_tiles<L, kBatchThreads>(tacc, J.hg0, J.hg0_rs, V, J.eqz, ly.gx_ptr, ly.gx_ent, 1u << n, R, 0u, 1u,
+                                   ((1u << n) + gx_rows<L, kBatchThreads>() - 1) / gx_rows<L, kBatchThreads>());
         __syncthreads();
         TsArgs ts{};
         ts.R = R; ts.chal = chal_l; ts.vx = J.vxy; ts.vy = J.vxy + R * L;
@@ -1486,8 +1599,8 @@ __device__ void prove_batch_job(const BatchJob &J, uint32_t R, int hash_kind, un
         tp.ts.phase = 1;
         phase_rounds<L, false>(tp);
         block_eq(J.eqx, chal_l, 0u, 0u, nullptr, n, R);
-        for (uint32_t y0 = 0; y0 < (1u << n); y0 += tile_lanes(kBatchThreads))
-            hy_tile<kBatchThreads>(tacc, J.s0, J.s0_rs, J.eqz, J.eqx, ly.hy_ptr, ly.hy_ent, 1u << n, R, y0);
+        hy_tiles<kBatchThreads>(tacc, J.s0, J.s0_rs, J.eqz, J.eqx, ly.hy_ptr, ly.hy_ent, 1u << n, R, 0u, 1u,
+                                ((1u << n) + tile_lanes(kBatchThreads) - 1) / tile_lanes(kBatchThreads));
         __syncthreads();
         tp.h0 = J.s0; tp.h0_rs = J.s0_rs; tp.hA = J.sA; tp.hB = J.sB; tp.hA_rs = J.sA_rs; tp.hB_rs = J.sB_rs;
         tp.ts.phase = 2;
@@ -1734,16 +1847,14 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
         const LsGx &a = T.gx;
         constexpr uint32_t ROWS = gx_rows<L, BS>();
         const uint32_t tiles = (a.N + ROWS - 1) / ROWS;
-        for (uint32_t tile = (uint32_t)xcd_tile(lb, T.nb); tile < tiles; tile += T.nb)
-            gx_tile<L, BS>(tacc, a.hg, a.hg_rs, a.v, a.eqz, a.ptr, a.ent, a.N, R, tile * ROWS);
+        gx_tiles<L, BS>(tacc, a.hg, a.hg_rs, a.v, a.eqz, a.ptr, a.ent, a.N, R, xcd_tile(lb, T.nb), T.nb, tiles);
         break;
     }
     case kOpHy: {
         const LsHy &a = T.hy;
         constexpr uint32_t ROWS = tile_lanes(BS);
         const uint32_t tiles = (a.N + ROWS - 1) / ROWS;
-        for (uint32_t tile = (uint32_t)xcd_tile(lb, T.nb); tile < tiles; tile += T.nb)
-            hy_tile<BS>(tacc, a.s, a.s_rs, a.eqz, a.eqx, a.ptr, a.ent, a.N, R, tile * ROWS);
+        hy_tiles<BS>(tacc, a.s, a.s_rs, a.eqz, a.eqx, a.ptr, a.ent, a.N, R, xcd_tile(lb, T.nb), T.nb, tiles);
         break;
     }
     case kOpDot: {
@@ -2051,8 +2162,9 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
             });
         }
         {
-            // one item per thread: the CSR walk is a chain of dependent gathers, hidden by occupancy
-            dim3 grid((uint32_t)std::min<uint64_t>((N + gx_rows<L, kThreads>() - 1) / gx_rows<L, kThreads>(), 65536));
+            // a resident grid (six blocks per CU, limited by the LDS sums) walks the tiles with
+            // the next tiles' loads in flight
+            dim3 grid((uint32_t)std::min<uint64_t>((N + gx_rows<L, kThreads>() - 1) / gx_rows<L, kThreads>(), kGxBlocks));
             timed(pf, kKGx, st, [&] {
                 hipLaunchKernelGGL(k_gx<L>, grid, dim3(kThreads), 0, st, s.hg0, hg0_rs, V, s.eqz, ly.gx_ptr,
                                    ly.gx_ent, (uint32_t)N, R);
@@ -2548,7 +2660,9 @@ uint64_t batch_tables_u32(const DevCircuit &dc, uint32_t R)
 
 constexpr uint32_t kLsGatherItems = 4;  // work items per thread of a gather task
 constexpr uint32_t kLsMaxJobs = 2048;   // at most two work tasks per proof in a step, 4096 per step
-constexpr uint32_t kLsGroups = 3;       // grouped lockstep engine: groups of proofs (one stream each)
+constexpr uint32_t kLsGroups = 3;
+constexpr uint32_t kLsGxTilesPerBlock = 4;  // g_x tiles per workgroup (pipelined: the next tiles' loads in flight)
+constexpr uint32_t kLsHyTilesPerBlock = 4;  // h_y tiles per workgroup, likewise       // grouped lockstep engine: groups of proofs (one stream each)
 
 uint32_t ls_grid(uint64_t work, uint32_t bs)
 {
@@ -2645,7 +2759,7 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
             LsEntry e;
             LsTask t = ls_task(kOpGx, job, l);
             t.gx = LsGx{J.hg0, J.hg0_rs, V, J.eqz, ly.gx_ptr, ly.gx_ent, (uint32_t)N};
-            t.nb = ls_tile_blocks((N * L + tile_lanes(BS) - 1) / tile_lanes(BS));
+            t.nb = ls_tile_blocks(((N * L + tile_lanes(BS) - 1) / tile_lanes(BS) + kLsGxTilesPerBlock - 1) / kLsGxTilesPerBlock);
             e.w[0] = t;
             e.nw = 1;
             if (l == D - 1) {
@@ -2670,7 +2784,8 @@ void ls_job_ops(const DevCircuit &dc, const BatchJob &J, uint32_t job, uint32_t 
                 push_w(t);
                 LsTask h = ls_task(kOpHy, job, l);
                 h.hy = LsHy{J.s0, J.s0_rs, J.eqz, J.eqx, ly.hy_ptr, ly.hy_ent, (uint32_t)N};
-                h.nb = ls_tile_blocks((N + tile_lanes(BS) - 1) / tile_lanes(BS));
+                h.nb = ls_tile_blocks(((N + tile_lanes(BS) - 1) / tile_lanes(BS) + kLsHyTilesPerBlock - 1) /
+                                      kLsHyTilesPerBlock);
                 push_w(h);
             }
             uint32_t *h0 = p2 ? J.s0 : J.hg0, *hA = p2 ? J.sA : J.hgA, *hB = p2 ? J.sB : J.hgB;
