@@ -281,7 +281,15 @@ __global__ void __launch_bounds__(kThreads) k_eq(uint32_t *__restrict__ out, con
 constexpr uint32_t tile_lanes(uint32_t bs) { return 4 * bs < 1024 ? 4 * bs : 1024; }
 template <int L, int BS>
 constexpr uint32_t gx_rows() { return tile_lanes(BS) / L; }
-constexpr uint32_t kGxAccWords = 3 * 1024;
+// A row of the g_x accumulators is L + 1 u64 (one of padding): a wave's 64 atomics of one
+// instruction hit up to 16 rows at the same lane offset, and with 8L-byte rows (a multiple
+// of the 128 bytes the 32 LDS banks span) they all fell on the same two banks; the padding
+// shifts each row by two banks (the bank-conflict cycles of k_gx were 57 % of its LDS
+// cycles before it).
+template <int L, int BS>
+constexpr uint32_t gx_acc_words() { return 3 * gx_rows<L, BS>() * (L + 1); }
+template <int L, int BS>
+constexpr uint32_t tile_acc_words() { return gx_acc_words<L, BS>() > 3 * tile_lanes(BS) ? gx_acc_words<L, BS>() : 3 * tile_lanes(BS); }
 
 // The tiles tfirst, tfirst + tstep, ... (< tiles) of g_x, software-pipelined: while a tile's
 // contributions are added and its rows written, the gathers of the next tile, the CSR
@@ -297,7 +305,7 @@ __device__ __forceinline__ void gx_tiles(unsigned long long *__restrict__ acc, u
                                          const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent, uint32_t N,
                                          uint32_t R, uint32_t tfirst, uint32_t tstep, uint32_t tiles)
 {
-    constexpr uint32_t LG = L / 4, ROWS = gx_rows<L, BS>();
+    constexpr uint32_t LG = L / 4, ROWS = gx_rows<L, BS>(), RS = L + 1;
     constexpr int NI = kGxPipe;
     const uint32_t t = threadIdx.x;
     // The loads below are issued without branches (an item out of range loads from the
@@ -328,11 +336,11 @@ __device__ __forceinline__ void gx_tiles(unsigned long long *__restrict__ acc, u
         const uint32_t g = (uint32_t)(w % LG);
         if (en.x == kAddMark) vb = make_uint4(1u, 1u, 1u, 1u);
         const uint32_t c[3] = {m31_mul(en.z, ez.r0), R > 1 ? m31_mul(en.z, ez.r1) : 0u, R > 2 ? m31_mul(en.z, ez.r2) : 0u};
-        unsigned long long *a = acc + (uint64_t)(en.w - x0) * L + 4 * g;
+        unsigned long long *a = acc + (uint64_t)(en.w - x0) * RS + 4 * g;
 #pragma unroll
         for (uint32_t j = 0; j < 3; j++)
             if (j < R) {
-                unsigned long long *aj = a + j * ROWS * L;
+                unsigned long long *aj = a + j * ROWS * RS;
                 atomicAdd(aj, (unsigned long long)m31_mul_wide(vb.x, c[j]));
                 atomicAdd(aj + 1, (unsigned long long)m31_mul_wide(vb.y, c[j]));
                 atomicAdd(aj + 2, (unsigned long long)m31_mul_wide(vb.z, c[j]));
@@ -352,8 +360,8 @@ __device__ __forceinline__ void gx_tiles(unsigned long long *__restrict__ acc, u
         const uint32_t x0 = tg * ROWS;
         const uint32_t rows = N - x0 < ROWS ? N - x0 : ROWS;
 #pragma unroll
-        for (uint32_t k = 0; k < (3 * ROWS * L + BS - 1) / BS; k++)
-            if (t + k * BS < R * ROWS * L) acc[t + k * BS] = 0;
+        for (uint32_t k = 0; k < (3 * ROWS * RS + BS - 1) / BS; k++)
+            if (t + k * BS < R * ROWS * RS) acc[t + k * BS] = 0;
         __syncthreads();
         const uint64_t work = count(tg, rg);
         // items past the pipelined ones (a tile with many entries), loaded here directly:
@@ -383,7 +391,7 @@ __device__ __forceinline__ void gx_tiles(unsigned long long *__restrict__ acc, u
             const uint32_t i = t + k * BS;
             if (i < R * rows * LG) {
                 const uint32_t j = i / (rows * LG), rem = i % (rows * LG), r = rem / LG, g = rem % LG;
-                const unsigned long long *aj = acc + (j * ROWS + r) * L + 4 * g;
+                const unsigned long long *aj = acc + (j * ROWS + r) * RS + 4 * g;
                 st4(hg + j * hg_rs + (uint64_t)(x0 + r) * L + 4 * g,
                     make_uint4(m31_reduce64(aj[0]), m31_reduce64(aj[1]), m31_reduce64(aj[2]), m31_reduce64(aj[3])));
             }
@@ -407,7 +415,7 @@ __global__ void __launch_bounds__(kThreads) k_gx(uint32_t *__restrict__ hg, uint
                                                  const uint32_t *__restrict__ ptr,
                                                  const uint32_t *__restrict__ ent, uint32_t N, uint32_t R)
 {
-    __shared__ unsigned long long acc[kGxAccWords];
+    __shared__ unsigned long long acc[gx_acc_words<L, kThreads>()];
     constexpr uint32_t ROWS = gx_rows<L, kThreads>();
     const uint32_t tiles = (N + ROWS - 1) / ROWS;
     gx_tiles<L, kThreads>(acc, hg, hg_rs, v, eqz, ptr, ent, N, R, xcd_tile(blockIdx.x, gridDim.x), gridDim.x, tiles);
@@ -1520,8 +1528,6 @@ __device__ void block_claims(const uint32_t *__restrict__ vals, const uint32_t *
     __syncthreads();
 }
 
-// LDS accumulators shared by the g_x and h_y tiles of one workgroup
-constexpr uint32_t kTileAccWords = kGxAccWords > kHyAccWords ? kGxAccWords : kHyAccWords;
 
 // A persistent grid over a device work queue: each workgroup takes the next slot of `order`
 // (largest proofs first) from the queue head until the slots run out, and proves that job
@@ -1534,7 +1540,7 @@ __global__ void __launch_bounds__(kBatchThreads) k_prove_batch(const BatchJob *_
                                                                const uint32_t *__restrict__ order, uint32_t nj,
                                                                uint32_t *queue, uint32_t R, int hash_kind)
 {
-    __shared__ unsigned long long tacc[kTileAccWords];
+    __shared__ unsigned long long tacc[tile_acc_words<L, kBatchThreads>()];  // g_x and h_y tiles
     __shared__ uint32_t slot;
     for (;;) {
         if (threadIdx.x == 0) slot = atomicAdd(queue, 1u);
@@ -1810,7 +1816,7 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
                                                 uint32_t block_base)
 {
     constexpr uint32_t LG = L / 4, Q = 3 * L;
-    __shared__ unsigned long long tacc[3 * tile_lanes(BS)];
+    __shared__ unsigned long long tacc[tile_acc_words<L, BS>()];
     // block_base: the launch covers the step's workgroups from block_base on (a multiple of
     // 8, so that blockIdx.x and the step's workgroup index sit on the same XCD)
     const uint2 f = ls_find(start, ntasks, blockIdx.x + block_base);
