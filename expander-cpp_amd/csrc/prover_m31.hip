@@ -1256,6 +1256,7 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(TailParams p)
 // transcript state; the workgroup of the last rep writes the final state.
 constexpr uint32_t kRepThreads = 512;
 constexpr int kTailV = 2;  // lanes per thread in the small-table passes
+constexpr int kTailPreV = 4;  // in the first pass of k_tail_rep (table k0 from a table up to 4x larger)
 struct TailRepParams {
     TailParams t;
     uint32_t kend;
@@ -1311,21 +1312,25 @@ __device__ __forceinline__ void tail_rep_body(const TailRepParams &q, uint32_t r
     auto tab_f = [&](uint32_t kk) { return ((kk & 1) ? p.fA : p.fB) + rep * ((kk & 1) ? p.fA_rs : p.fB_rs); };
     auto tab_h = [&](uint32_t kk) { return ((kk & 1) ? p.hA : p.hB) + rep * ((kk & 1) ? p.hA_rs : p.hB_rs); };
     {
-        uint32_t acc[kPairVals][V] = {};
-        const uint64_t items = (1ull << (p.n - k0 - 2)) * LG;
+        // table k0 and its products, with PV lanes per thread: one workgroup reads up to 16
+        // rows per item of a table four times larger, so it keeps wider loads in flight
+        constexpr int PV = NT >= kRepThreads ? kTailPreV : V;
+        constexpr uint32_t PLG = L / PV;
+        uint32_t acc[kPairVals][PV] = {};
+        const uint64_t items = (1ull << (p.n - k0 - 2)) * PLG;
         if (k0 == 0) {
             for (uint64_t w = t; w < items; w += NT)
-                pair_item<L, PH2, 0, V>(f0, nullptr, h0, nullptr, w / LG, (uint32_t)(w % LG), 0u, 0u, acc);
+                pair_item<L, PH2, 0, PV>(f0, nullptr, h0, nullptr, w / PLG, (uint32_t)(w % PLG), 0u, 0u, acc);
         } else if (p.dbl) {
             const uint32_t ra = chal[k0 - 2], rb = chal[k0 - 1];
             for (uint64_t w = t; w < items; w += NT)
-                pair_item<L, PH2, 2, V>(f0, tab_f(k0), h0, tab_h(k0), w / LG, (uint32_t)(w % LG), ra, rb, acc);
+                pair_item<L, PH2, 2, PV>(f0, tab_f(k0), h0, tab_h(k0), w / PLG, (uint32_t)(w % PLG), ra, rb, acc);
         } else {
             const uint32_t ra = chal[k0 - 1];
             for (uint64_t w = t; w < items; w += NT)
-                pair_item<L, PH2, 1, V>(f0, tab_f(k0), h0, tab_h(k0), w / LG, (uint32_t)(w % LG), ra, 0u, acc);
+                pair_item<L, PH2, 1, PV>(f0, tab_f(k0), h0, tab_h(k0), w / PLG, (uint32_t)(w % PLG), ra, 0u, acc);
         }
-        wave_reduce_pairs<L, V>(acc, red[0][wave]);
+        wave_reduce_pairs<L, PV>(acc, red[0][wave]);
     }
     __syncthreads();
     uint32_t d[8] = {};
