@@ -41,7 +41,7 @@ TAIL_ROWS = 2048  # tables with at most this many rows end a phase in the small-
 # Keys: the workload and the batch engine setting the profiled run used.
 # k_pair<16, phase 1, double-fold> of the single 24 x 2^20 proof
 # (profiles/pmc_traffic_m31x16_24x2e20_keccak256.md):
-SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 137421175}
+SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 137744556}
 # BN254 fused round k_bn_round<true> of the 12 x 2^18 proof: int32 VALU wave instructions per
 # dispatch (rocprofv3 --pmc SQ_INSTS_VALU_INT32, averaged over the proof's dispatches of both
 # phases; profiles/valu_bn254_12x2e18_keccak256.md).  The int32 VALU peak is derived from the
@@ -54,14 +54,14 @@ INT32_VALU_PEAK_TOPS = 39.32
 # {(proofs, engine, block, lanes per thread): bytes per dispatch}; engine 4 (proof groups on
 # their own streams) launches about three times as many, smaller, round-pair kernels
 BATCH_PAIR_TRAFFIC = {
-    (512, 2, 64, 1): 1252691336, (512, 2, 64, 2): 1216063907, (512, 2, 64, 4): 1214141397,
-    (512, 2, 128, 1): 1215803902, (512, 2, 128, 2): 1189142499, (512, 2, 128, 4): 1180104679,
-    (512, 2, 256, 1): 1184597465, (512, 2, 256, 2): 1170121267, (512, 2, 256, 4): 1167875802,
-    (512, 2, 512, 1): 1171106264, (512, 2, 512, 2): 1163164573, (512, 2, 512, 4): 1164080179,
-    (512, 4, 64, 1): 422220552, (512, 4, 64, 2): 412015620, (512, 4, 64, 4): 411162970,
-    (512, 4, 128, 1): 411678381, (512, 4, 128, 2): 404511076, (512, 4, 128, 4): 402241307,
-    (512, 4, 256, 1): 402472958, (512, 4, 256, 2): 398202361, (512, 4, 256, 4): 397999671,
-    (512, 4, 512, 1): 398269810, (512, 4, 512, 2): 395821151, (512, 4, 512, 4): 397116025,
+    (512, 2, 64, 1): 1252212537, (512, 2, 64, 2): 1215790856, (512, 2, 64, 4): 1213910904,
+    (512, 2, 128, 1): 1215148902, (512, 2, 128, 2): 1188869878, (512, 2, 128, 4): 1179831817,
+    (512, 2, 256, 1): 1184559781, (512, 2, 256, 2): 1170093154, (512, 2, 256, 4): 1167763312,
+    (512, 2, 512, 1): 1171026623, (512, 2, 512, 2): 1163177863, (512, 2, 512, 4): 1164018785,
+    (512, 4, 64, 1): 422053602, (512, 4, 64, 2): 411957245, (512, 4, 64, 4): 411074671,
+    (512, 4, 128, 1): 411592693, (512, 4, 128, 2): 404512912, (512, 4, 128, 4): 402235282,
+    (512, 4, 256, 1): 402466210, (512, 4, 256, 2): 398174594, (512, 4, 256, 4): 397908388,
+    (512, 4, 512, 1): 398239225, (512, 4, 512, 2): 395819784, (512, 4, 512, 4): 397058799,
 }
 
 
