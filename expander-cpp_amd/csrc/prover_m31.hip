@@ -1261,6 +1261,10 @@ struct TailRepParams {
     TailParams t;
     uint32_t kend;
     uint32_t len0, hs0;  // transcript length and hash start (bytes of the tail) at round k0
+    // when not null: table k0 and its nine products were made by a grid pass (k_pair,
+    // block-major partials [rep][pre_nb][kPairVals L]); the kernel only adds the partials
+    const uint32_t *pre_part;
+    uint32_t pre_nb;
 };
 
 // Adds the nine product sums of a wave's threads (V lanes per thread; the threads sharing
@@ -1311,7 +1315,17 @@ __device__ __forceinline__ void tail_rep_body(const TailRepParams &q, uint32_t r
     const uint32_t *f0 = p.f0 + rep * p.f0_rs, *h0 = p.h0 + rep * p.h0_rs;
     auto tab_f = [&](uint32_t kk) { return ((kk & 1) ? p.fA : p.fB) + rep * ((kk & 1) ? p.fA_rs : p.fB_rs); };
     auto tab_h = [&](uint32_t kk) { return ((kk & 1) ? p.hA : p.hB) + rep * ((kk & 1) ? p.hA_rs : p.hB_rs); };
-    {
+    if (q.pre_part) {
+        // the products of table k0 from the grid pass: its few partial rows added (all
+        // loads in flight at once) into slot 0; the other slots zero
+        const uint32_t *pp = q.pre_part + (uint64_t)rep * q.pre_nb * NV;
+        for (uint32_t v = t; v < W * NV; v += NT) {
+            uint32_t x = 0;
+            if (v < NV)
+                for (uint32_t b = 0; b < q.pre_nb; b++) x = m31_add(x, pp[(uint64_t)b * NV + v]);
+            red[0][v / NV][v % NV] = x;
+        }
+    } else {
         // table k0 and its products, with PV lanes per thread: one workgroup reads up to 16
         // rows per item of a table four times larger, so it keeps wider loads in flight
         constexpr int PV = NT >= kRepThreads ? kTailPreV : V;
@@ -2288,7 +2302,31 @@ int prove_impl(const DevCircuit &dc, DevScratch &s, uint32_t R, int hash_kind, u
                 q.kend = klast;
                 q.len0 = (uint32_t)ts_len;
                 q.hs0 = (uint32_t)ts_hs;
-                timed(pf, phase == 1 ? kKTailPh1 : kKTailPh2, st, [&] {
+                const int cls = phase == 1 ? kKTailPh1 : kKTailPh2;
+                if (K >= 2) {
+                    // table K (double-folded from table K - 2) and its nine products on the
+                    // whole GPU: the round-pair pass of rounds K, K + 1 without its transcript
+                    // end; k_tail_rep adds its partials instead of making the table in one
+                    // workgroup per repetition (19 us there, a memory latency per row group)
+                    RoundParams rp{};
+                    rp.items = (uint32_t)(N >> (K + 2));
+                    rp.partials = s.partials;
+                    rp.fsrc = q.t.f0; rp.fsrc_rs = q.t.f0_rs;
+                    rp.hsrc = q.t.h0; rp.hsrc_rs = q.t.h0_rs;
+                    rp.fdst = s.fB; rp.fdst_rs = fB_rs;  // table K (even): the B buffers
+                    rp.hdst = hB; rp.hdst_rs = hB_rs;
+                    rp.rprev = chal_l + (phase == 2 ? 32 : 0) + (K - 2);
+                    rp.R = R;
+                    const uint32_t pbs = phase == 1 ? kPairBS1 : kPairBS2, pv = phase == 1 ? kPairV1 : kPairV2;
+                    const uint64_t pw = (uint64_t)rp.items * (L / pv);
+                    const uint32_t nbx = (uint32_t)((std::min<uint64_t>(std::max<uint64_t>(1, (pw + pbs - 1) / pbs),
+                                                                        kMaxBlocksPerRep) + 7) & ~7ull);
+                    if (phase == 1) launch_timed(pf, cls, k_pair<L, false, 2, kPairBS1, kPairV1>, dim3(nbx * R), dim3(pbs), st, rp);
+                    else launch_timed(pf, cls, k_pair<L, true, 2, kPairBS2, kPairV2>, dim3(nbx * R), dim3(pbs), st, rp);
+                    q.pre_part = s.partials;
+                    q.pre_nb = nbx;
+                }
+                timed(pf, cls, st, [&] {
                     if (phase == 1) hipLaunchKernelGGL((k_tail_rep<L, false>), dim3(R), dim3(kRepThreads), 0, st, q);
                     else hipLaunchKernelGGL((k_tail_rep<L, true>), dim3(R), dim3(kRepThreads), 0, st, q);
                 });
