@@ -41,7 +41,7 @@ TAIL_ROWS = 2048  # tables with at most this many rows end a phase in the small-
 # Keys: the workload and the batch engine setting the profiled run used.
 # k_pair<16, phase 1, double-fold> of the single 24 x 2^20 proof
 # (profiles/pmc_traffic_m31x16_24x2e20_keccak256.md):
-SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 137744556}
+SINGLE_PAIR_TRAFFIC = {(24, 20, 16): 137754769}
 # BN254 fused round k_bn_round<true> of the 12 x 2^18 proof: int32 VALU wave instructions per
 # dispatch (rocprofv3 --pmc SQ_INSTS_VALU_INT32, averaged over the proof's dispatches of both
 # phases; profiles/valu_bn254_12x2e18_keccak256.md).  The int32 VALU peak is derived from the

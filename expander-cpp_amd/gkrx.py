@@ -474,14 +474,24 @@ class Prover:
         n = len(circuits)
         R, L = self.ctx.config.get_num_repetitions(), self.ctx.config.words
         bufs = outs if outs is not None else [np.empty(self.proof_size(c), dtype=np.uint8) for c in circuits]
-        claims = [np.empty(R * L, dtype=np.uint32) for _ in circuits]
-        hs = (C.c_void_p * n)(*[c._h.value for c in circuits])
-        ps = (C.c_void_p * n)(*[b.ctypes.data for b in bufs])
-        caps = (C.c_size_t * n)(*[b.size for b in bufs])
-        lens = (C.c_size_t * n)()
-        cs = (C.c_void_p * n)(*[c.ctypes.data for c in claims])
-        _check(lib().gkrx_prove_batch(self.ctx._h, n, hs, ps, caps, lens, cs), "gkrx_prove_batch")
-        return [(claims[i].reshape(R, L), bufs[i][:lens[i]]) for i in range(n)]
+        for b in bufs:
+            if b.dtype != np.uint8 or not b.flags.c_contiguous:
+                raise ValueError("proof buffers must be contiguous uint8 arrays")
+        # the pointer tables as numpy arrays (one allocation each, no per-proof ctypes
+        # objects); claimed values of proof i are row i of one [n, R, L] array
+        claims = np.empty((n, R, L), dtype=np.uint32)
+        hs = np.fromiter((c._h.value for c in circuits), dtype=np.uint64, count=n)
+        ps = np.fromiter((b.ctypes.data for b in bufs), dtype=np.uint64, count=n)
+        caps = np.fromiter((b.size for b in bufs), dtype=np.uint64, count=n)
+        lens = np.zeros(n, dtype=np.uint64)
+        cs = np.uint64(claims.ctypes.data) + np.arange(n, dtype=np.uint64) * np.uint64(R * L * 4)
+        _check(lib().gkrx_prove_batch(self.ctx._h, n, hs.ctypes.data_as(C.POINTER(C.c_void_p)),
+                                      ps.ctypes.data_as(C.POINTER(C.c_void_p)),
+                                      caps.ctypes.data_as(C.POINTER(C.c_size_t)),
+                                      lens.ctypes.data_as(C.POINTER(C.c_size_t)),
+                                      cs.ctypes.data_as(C.POINTER(C.c_void_p))), "gkrx_prove_batch")
+        ln = lens.tolist()
+        return [(claims[i], bufs[i][:ln[i]]) for i in range(n)]
 
     def last_timing(self):
         ms = (C.c_double * 3)()
