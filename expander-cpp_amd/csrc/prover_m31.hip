@@ -9,8 +9,9 @@
 //   k_pair     <- receive_challenge (folds of rounds k-2, k-1) fused with poly_eval_at
 //                 (rounds k and k+1, the latter as a quadratic in r_k)
 //                                                     sumcheck_helper.hpp:68-133, on the whole GPU
-//   k_tail_rep, k_tail <- the same one round at a time for the small last rounds of a
-//                 phase, in one workgroup
+//   k_tail_rep, k_tail <- the small last rounds of a phase one at a time, one workgroup
+//                 per repetition (each round's pass behind the previous round's hash),
+//                 then the last round of the phase in one workgroup
 //   k_ls_*     <- all of the above for a batch of independent proofs, one step of every
 //                 proof per launch (no reference counterpart: the reference proves one
 //                 proof per thread, bench_keccak.cpp:32-42)
@@ -21,7 +22,8 @@
 //
 // Layout: every table keeps the reference's natural index order, element-major and
 // lane-minor (one table entry = L consecutive u32 lanes = the bytes F::to_bytes writes).
-// A thread owns 4 lanes of one entry (16-byte loads); L/4 threads cover one entry.
+// A thread owns V = 1, 2 or 4 lanes of one entry (4-, 8- or 16-byte loads); L/V threads
+// cover one entry.
 // Scatters of the reference become gathers over CSR lists built at circuit load, so no
 // modular atomics are needed and every sum is exact field arithmetic (order-free).
 // Folding is ping-pong between buffers (the reference folds in place, which is only
