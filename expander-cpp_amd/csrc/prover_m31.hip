@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(kThreads) k_forward(const uint32_t *__restrict
 // [chunk * CH * BS, (chunk + 1) * CH * BS): the low log2(BS) bits of z index a table of BS
 // partial products per rep, the high bits one of CH, both in LDS.
 constexpr int kEqChunk = 16;      // z values per thread in the batch engine's eq ops
-constexpr int kEqChunkOne = 4;    // in k_eq (one table per launch: more blocks)
+constexpr int kEqChunkOne = 8;    // in k_eq (one table per launch: more blocks; 2, 4, 16: slower)
 template <int BS, int CH = kEqChunk>
 __device__ __forceinline__ void eq_block(uint32_t *__restrict__ o, const uint32_t *__restrict__ chal, uint32_t r1_off,
                                          uint32_t r2_off, const uint32_t *__restrict__ ab, uint32_t m, uint32_t chunk,
