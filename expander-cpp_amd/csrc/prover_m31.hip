@@ -1299,7 +1299,8 @@ __device__ __forceinline__ void wave_reduce_pairs(uint32_t (&acc)[kPairVals][V],
 // message, whose sums it evaluates from the previous pass at r_k; a round costs one hash
 // and one barrier instead of a pass, a reduction and a hash in turn.  Before round k0 the
 // whole workgroup makes table k0 (from the layer input, or folding table k0 - 1 once or
-// table k0 - 2 twice) and its products.  The product sums of a pass live per wave in
+// table k0 - 2 twice) and its products, unless a grid pass made them already (pre_part:
+// then it only adds that pass's partial rows).  The product sums of a pass live per wave in
 // red[buffer][wave], two buffers used in turn (wave 0 reads one while the pass fills the
 // other).  Table k (k >= 1) is stored in the A buffers for odd k, B for even k, where the
 // last round (k_tail) finds table kend - 1.
