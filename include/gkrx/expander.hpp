@@ -428,15 +428,26 @@ class Proof {
         return f;
     }
     // transcript.hpp:54-85: [u32 byte count][bytes]; the compressed copy is made with tar as
-    // the reference does
+    // the reference does.  Both paths reach the shell single-quoted (a quote inside one is
+    // written '\''), so a path cannot add words or commands to the tar line.
     void write_and_compress(const char *proof_loc = "/tmp/proof.txt",
                             const char *compressed_proof_loc = "/tmp/compressed_proof.tar.gz")
     {
         if (gkrx_proof_write(proof_loc, bytes.data(), bytes.size()) != GKRX_OK) std::abort();
         if (compressed_proof_loc) {
-            const std::string cmd = std::string("tar -czf ") + compressed_proof_loc + " --absolute-names " + proof_loc;
+            const std::string cmd = std::string("tar -czf ") + shell_quote(compressed_proof_loc) +
+                                    " --absolute-names -- " + shell_quote(proof_loc);
             if (std::system(cmd.c_str()) != 0) std::abort();
         }
+    }
+    static std::string shell_quote(const char *s)
+    {
+        std::string q = "'";
+        for (; *s; ++s) {
+            if (*s == '\'') q += "'\\''";
+            else q += *s;
+        }
+        return q + "'";
     }
     void read_proof(const char *proof_loc = "/tmp/proof.txt")
     {

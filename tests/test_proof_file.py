@@ -44,3 +44,40 @@ def test_reference_reads_and_writes_the_same_files(tmp_path):
                    capture_output=True)
     assert gkrx.proof_read(theirs) == data
     assert open(theirs, "rb").read() == open(ours, "rb").read()
+
+
+LIBDIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "expander-cpp_amd")
+INCDIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+
+SHIM_WRITER = r"""
+#include "gkrx/expander.hpp"
+int main(int argc, char **argv)
+{
+    gkr::Proof<gkr::M31_field::VectorizedM31> p;
+    const unsigned char b[5] = {1, 2, 3, 4, 5};
+    p.append_bytes(b, 5);
+    p.write_and_compress(argv[1], argv[2]);
+    return 0;
+}
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(LIBDIR, "libgkrx.so")), reason="libgkrx.so not built")
+def test_shim_compress_quotes_paths(tmp_path):
+    """Proof::write_and_compress hands both paths to tar through the shell; paths holding
+    spaces, quotes, ';' and '$(...)' must name files, never run anything."""
+    src = tmp_path / "w.cpp"
+    src.write_text(SHIM_WRITER)
+    exe = str(tmp_path / "w")
+    subprocess.run(["g++", "-std=c++17", "-I" + INCDIR, "-o", exe, str(src), "-L" + LIBDIR, "-lgkrx",
+                    "-Wl,-rpath," + LIBDIR], check=True, capture_output=True)
+    d = tmp_path / "d ir"
+    d.mkdir()
+    marker = tmp_path / "ran"
+    proof = str(d / "p'; touch ran; '.bin")
+    tgz = str(d / "c $(touch ran) '.tar.gz")
+    subprocess.run([exe, proof, tgz], check=True, capture_output=True, timeout=60, cwd=str(tmp_path))
+    assert not marker.exists()
+    assert gkrx.proof_read(proof) == bytes([1, 2, 3, 4, 5])
+    listing = subprocess.run(["tar", "-tzf", tgz], check=True, capture_output=True, text=True).stdout
+    assert listing.strip().endswith(os.path.basename(proof))
