@@ -166,16 +166,79 @@ __device__ __forceinline__ void forward_item(const uint32_t *__restrict__ vin, u
     st4(vout + (uint64_t)o * L + 4 * g, acc);
 }
 
+// The work items w0 + u BS (u < U) of one thread.  Each of an item's three dependent loads
+// (its entry range, its first CSR entry, the input rows that entry names) is issued for all
+// U items before any is used, so a thread has U gathers in flight instead of one.  The loads
+// take no branches: an item past the end, or an output with no entry, reads output 0's
+// range, an entry from vin and input row 0 (valid addresses, and row 0 stays cached) and is
+// dropped; an add gate's second row is row 0 too.  Outputs with more than one entry add the
+// others afterwards.
+template <int L, int U, int BS>
+__device__ __forceinline__ void forward_items(const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
+                                              const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
+                                              uint64_t w0, uint64_t total)
+{
+    constexpr uint32_t LG = L / 4;
+    uint32_t o[U], g[U], e0[U], e1[U], a[U], b[U], c[U];
+    uint4 va[U], vb[U], acc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint64_t w = w0 + (uint64_t)u * BS;
+        o[u] = w < total ? (uint32_t)(w / LG) : 0u;
+        g[u] = (uint32_t)(w % LG);
+        e0[u] = ptr[o[u]];
+        e1[u] = ptr[o[u] + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        if (w0 + (uint64_t)u * BS >= total) e1[u] = e0[u];
+        const bool has = e0[u] < e1[u];
+        const uint32_t *pe = has ? ent + 3 * (uint64_t)e0[u] : vin;
+        const uint32_t x = pe[0], y = pe[1], z = pe[2];
+        a[u] = has ? x : 0u;
+        b[u] = has ? y : kAddMark;
+        c[u] = has ? z : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        va[u] = ld4(vin + (uint64_t)a[u] * L + 4 * g[u]);
+        vb[u] = ld4(vin + (uint64_t)(b[u] != kAddMark ? b[u] : 0u) * L + 4 * g[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u] = mul4s(b[u] != kAddMark ? mul4(va[u], vb[u]) : va[u], c[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        for (uint32_t e = e0[u] + 1; e < e1[u]; e++) {
+            const uint32_t x = ent[3 * (uint64_t)e], y = ent[3 * (uint64_t)e + 1], z = ent[3 * (uint64_t)e + 2];
+            uint4 v = ld4(vin + (uint64_t)x * L + 4 * g[u]);
+            if (y != kAddMark) v = mul4(v, ld4(vin + (uint64_t)y * L + 4 * g[u]));
+            acc[u] = add4(acc[u], mul4s(v, z));
+        }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (w0 + (uint64_t)u * BS < total) st4(vout + (uint64_t)o[u] * L + 4 * g[u], acc[u]);
+}
+
+// The forward pass of one layer by nb blocks of BS threads: block `tile` takes the items
+// [tile U BS, (tile + 1) U BS), then the same range nb U BS further on, and so on.
+template <int L, int U, int BS>
+__device__ __forceinline__ void forward_range(const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
+                                              const uint32_t *__restrict__ ptr, const uint32_t *__restrict__ ent,
+                                              uint64_t total, uint32_t tile, uint32_t nb)
+{
+    for (uint64_t w0 = (uint64_t)tile * U * BS + threadIdx.x; w0 < total; w0 += (uint64_t)nb * U * BS)
+        forward_items<L, U, BS>(vin, vout, ptr, ent, w0, total);
+}
+
+constexpr int kFwdItems = 4;       // work items per thread in flight in the forward pass
+constexpr int kLsGatherItems = 4;  // the same in the batch engine's forward ops
 template <int L>
 __global__ void __launch_bounds__(kThreads) k_forward(const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout,
                                                       const uint32_t *__restrict__ ptr,
                                                       const uint32_t *__restrict__ ent, uint32_t M)
 {
-    constexpr uint32_t LG = L / 4;
-    const uint64_t total = (uint64_t)M * LG;
-    const uint64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    for (uint64_t w = tile * kThreads + threadIdx.x; w < total; w += (uint64_t)gridDim.x * kThreads)
-        forward_item<L>(vin, vout, ptr, ent, w);
+    forward_range<L, kFwdItems, kThreads>(vin, vout, ptr, ent, (uint64_t)M * (L / 4), xcd_tile(blockIdx.x, gridDim.x),
+                                          gridDim.x);
 }
 
 // ------------------------------------------------------------------ eq tables
@@ -1860,9 +1923,7 @@ __global__ void __launch_bounds__(BS) k_ls_work(const BatchJob *__restrict__ job
     }
     case kOpFwd: {
         const LsFwd &a = T.fwd;
-        const uint64_t total = (uint64_t)a.M * LG, tile = xcd_tile(lb, T.nb);
-        for (uint64_t w = tile * BS + t; w < total; w += (uint64_t)T.nb * BS)
-            forward_item<L>(a.vin, a.vout, a.ptr, a.ent, w);
+        forward_range<L, kLsGatherItems, BS>(a.vin, a.vout, a.ptr, a.ent, (uint64_t)a.M * LG, xcd_tile(lb, T.nb), T.nb);
         break;
     }
     case kOpEqz:
@@ -2123,7 +2184,7 @@ int forward_impl(const DevCircuit &dc, hipStream_t st, Profiler *pf)
         const LayerDev &ly = dc.layers[l];
         const uint64_t M = 1ull << ly.m;
         const uint64_t work = M * (L / 4);
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>((work + kThreads - 1) / kThreads, 65536);
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((work + kThreads * kFwdItems - 1) / (kThreads * kFwdItems), 65536);
         timed(pf, kKForward, st, [&] {
             hipLaunchKernelGGL(k_forward<L>, dim3(blocks), dim3(kThreads), 0, st, dc.values + ly.v_off,
                                dc.values + ly.out_off, ly.fw_ptr, ly.fw_ent, (uint32_t)M);
@@ -2710,11 +2771,10 @@ uint64_t batch_tables_u32(const DevCircuit &dc, uint32_t R)
     return u;
 }
 
-constexpr uint32_t kLsGatherItems = 4;  // work items per thread of a gather task
 constexpr uint32_t kLsMaxJobs = 2048;   // at most two work tasks per proof in a step, 4096 per step
-constexpr uint32_t kLsGroups = 3;
+constexpr uint32_t kLsGroups = 3;           // grouped lockstep engine: groups of proofs (one stream each)
 constexpr uint32_t kLsGxTilesPerBlock = 4;  // g_x tiles per workgroup (pipelined: the next tiles' loads in flight)
-constexpr uint32_t kLsHyTilesPerBlock = 4;  // h_y tiles per workgroup, likewise       // grouped lockstep engine: groups of proofs (one stream each)
+constexpr uint32_t kLsHyTilesPerBlock = 4;  // h_y tiles per workgroup, likewise
 
 uint32_t ls_grid(uint64_t work, uint32_t bs)
 {
