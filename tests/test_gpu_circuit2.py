@@ -30,3 +30,32 @@ def test_circuit2_proof_matches_oracle(case, tmp_path):
     assert bytes(proof) == op and np.array_equal(claimed, oc)
     assert gkrx.Verifier(ctx).verify(circ, claimed, proof)
     ctx.close()
+
+
+@pytest.mark.parametrize("engine,block,lanes", [(gkrx.BATCH_LOCKSTEP, 64, 1), (gkrx.BATCH_LOCKSTEP, 256, 4),
+                                                (gkrx.BATCH_PER_PROOF, 256, 4)],
+                         ids=["lockstep_b64_v1", "lockstep_b256_v4", "per_proof"])
+def test_circuit2_batch_equals_plain(engine, block, lanes, tmp_path):
+    """Outputs with two gates (the CIRCUIT2 circuits have up to about a thousand per layer)
+    through the batch engines' forward, g_x and h_y ops: every batched proof equals the
+    proof of the same circuit made alone."""
+    print(f"[gpu] circuit2 batch engine={engine} block={block} lanes={lanes}", flush=True)
+    ctx = gkrx.Context(gkrx.Config(fs_hash=gkrx.FS_KECCAK256, nb_parallel=16))
+    circs = []
+    for seed, depth, k in [(12, 4, 9), (13, 2, 12), (14, 3, 11)]:
+        segs, layers, sentinel = cg.make(seed, depth, k)
+        path = str(tmp_path / f"c{seed}.circuit2")
+        gkrx.write_circuit2(path, segs, layers, sentinel)
+        c = gkrx.Circuit.load_circuit2(ctx, path)
+        c.set_witness(gkrx.synth_witness(seed, 1 << k, 16))
+        circs.append(c)
+    p = gkrx.Prover(ctx)
+    alone = [p.prove(c) for c in circs]
+    alone = [(np.array(cl), bytes(pr)) for cl, pr in alone]
+    ctx.set_batch_engine(engine, block)
+    ctx.set_batch_lanes(lanes)
+    for _ in range(2):
+        res = p.prove_batch(circs)
+        for (cl, pr), (ca, pa) in zip(res, alone):
+            assert np.array_equal(cl, ca) and bytes(pr) == pa
+    ctx.close()
